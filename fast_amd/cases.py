@@ -1,0 +1,79 @@
+"""The benchmark workloads as host zones (meshes, numerics, reference states, BCs, fields).
+
+* Taylor-Green vortex: periodic box [0, 2 pi]^3, Mach 0.1, Re 1600, Pr 0.72, gamma 1.4
+  (``Fast/FastS/test/explicitLocalTgv_t1.py:86-90`` for the initial field, autoperiodic BCs as
+  ``Fast/FastS/test/tgvMpi.py:66-74``).
+* Channel with a bump: lower wall y_w(x) = 0.05 exp(-((x-1.5)/0.2)^2), walls stretched from a
+  first-cell height of 1e-4 with ratio 1.1, +-1 % interior node jitter (seed 0), Mach 0.5,
+  Re 1e5, viscous walls in j, periodic in k.
+"""
+import math
+import numpy as np
+
+from . import params as P
+from . import mesh as M
+from .zone import HostZone
+
+TWO_PI = 2.0 * math.pi
+ALL_PERIODIC = [("BCautoperiod", f, None) for f in ("imin", "imax", "jmin", "jmax", "kmin", "kmax")]
+
+
+def tgv_state():
+    # mu at T0 equals rho0 U0 L / Re with L = 1, U0 = 1
+    return P.reference_state(mach=0.1, reynolds=1600.0, prandtl=0.72, gamma=1.4)
+
+
+def tgv_zone(n, numerics=None, perturb=0.0, seed=0, flow="lamin", name="tgv"):
+    """TGV zone of n^3 (or n = (ni,nj,nk)) cells; ``perturb`` adds a seeded relative noise to
+    every primitive (used by the parity tests so that no term of the fluxes vanishes)."""
+    if np.isscalar(n):
+        n = (n, n, n)
+    num = P.Numerics() if numerics is None else numerics
+    st = tgv_state()
+    lo, hi = (0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI)
+    x, y, z = M.cart_vertices(n, lo, hi)
+    zone = HostZone(name, n[0], n[1], n[2], num, st, x, y, z, bcs=ALL_PERIODIC, flow=flow)
+    xc, yc, zc = M.cell_centers_cart(n, lo, hi)
+    w = M.tgv_primitives(xc, yc, zc, st)
+    if perturb > 0.0:
+        rng = np.random.default_rng(seed)
+        nd = zone.nd
+        scale = np.repeat([st["ro"], st["u"], st["u"], st["u"], st["T"]], nd)
+        w = w + perturb * scale * rng.uniform(-1.0, 1.0, w.size)
+    zone.ro[:] = w
+    zone.ro_m1[:] = w
+    zone.ro_p1[:] = w
+    return zone
+
+
+def tgv_dt(zone, cfl=0.5):
+    """Explicit time step dt = cfl * h / max(|u| + c) from the field."""
+    st = zone.state
+    nd = zone.nd
+    ro = zone.interior(zone.ro)
+    c = np.sqrt(st["gamma"] * st["cv"] * (st["gamma"] - 1.0) * ro[4])
+    vmax = np.max(np.abs(ro[1:4]).max(axis=0) + c)
+    h = TWO_PI / zone.dims[0]
+    return cfl * h / vmax
+
+
+def channel_state(mach=0.5, reynolds=1.0e5):
+    return P.reference_state(mach=mach, reynolds=reynolds, prandtl=0.72, gamma=1.4,
+                             length=1.0, direction=(1.0, 0.0, 0.0))
+
+
+def channel_zone(ni, nj, nk, numerics=None, jitter=0.01, seed=0, name="channel",
+                 inflow="BCFarfield", outflow="BCFarfield"):
+    num = P.Numerics(scheme="ausmpred", slope="o3", temporal_scheme="implicit",
+                     ss_iteration=1, time_step_nature="local", cfl=10.0) if numerics is None else numerics
+    st = channel_state()
+    x, y, z = M.channel_bump_vertices(ni, nj, nk, jitter=jitter, seed=seed)
+    bcs = [(inflow, "imin", None), (outflow, "imax", None),
+           ("BCWallViscous", "jmin", None), ("BCWallViscous", "jmax", None),
+           ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+    zone = HostZone(name, ni, nj, nk, num, st, x, y, z, bcs=bcs)
+    nd = zone.nd
+    w = np.concatenate([np.full(nd, st["ro"]), np.full(nd, st["vx"]), np.full(nd, st["vy"]),
+                        np.full(nd, st["vz"]), np.full(nd, st["T"])])
+    zone.ro[:] = w; zone.ro_m1[:] = w; zone.ro_p1[:] = w
+    return zone

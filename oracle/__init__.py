@@ -1,0 +1,125 @@
+"""Test oracle for the FastS structured NS iteration.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of bench.py, and only as the checker.  The shipped package never imports
+or loads anything from here.
+
+Two backends with the same call contract (``substep``, ``init_metric``, ``apply_bc``):
+
+* :class:`Oracle` -- ``liboracle.so``, the C restatement in ``fastoracle.c`` (always built);
+* :class:`Reference` -- ``_ref/libfastref.so``, the reference FastS Fortran kernels compiled
+  from their own sources by ``make ref`` plus the harness ``ref_driver.c``.  Only present where
+  the reference sources were available at build time.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libfastref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(ref=None):
+    """Build liboracle.so and, when the reference sources are present, _ref/libfastref.so."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+    if ref is None:
+        ref = os.path.isdir("/root/reference/Fast")
+    if ref:
+        subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "ref"])
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class _Backend:
+    prefix = None
+
+    def __init__(self, path):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+
+    def substep(self, z, nitcfg, nitrun=1, first_it=0, lssiter_verif=0, lexit_lu=0, rdm=None,
+                stop=0):
+        cfl = np.zeros(3)
+        f = getattr(self.lib, self.prefix + "substep")
+        f.restype = ctypes.c_int
+        rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nitcfg), ctypes.c_int(nitrun),
+               ctypes.c_int(first_it), ctypes.c_int(lssiter_verif), ctypes.c_int(lexit_lu),
+               _d(z.x), _d(z.y), _d(z.z), _d(z.ro), _d(z.ro_m1), _d(z.ro_p1), _d(z.xmut),
+               _d(z.ti), _d(z.tj), _d(z.tk), _d(z.vol), _d(z.drodm), _d(z.coe), _d(z.wig),
+               _d(rdm), _d(cfl), ctypes.c_int(stop))
+        if rc != 0:
+            raise RuntimeError("%s substep failed (%d)" % (self.prefix, rc))
+        return cfl
+
+    def apply_bc(self, z, rop, nstep=1):
+        f = getattr(self.lib, self.prefix + "apply_bc")
+        rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nstep), _d(rop), _d(z.ti),
+               _d(z.tj), _d(z.tk), _d(z.x), _d(z.y), _d(z.z), _d(z.xmut))
+        if rc != 0:
+            raise RuntimeError("%s apply_bc failed (%d)" % (self.prefix, rc))
+
+
+class Oracle(_Backend):
+    """C restatement (liboracle.so)."""
+    prefix = "fo_"
+
+    def __init__(self, path=LIB):
+        if not os.path.exists(path):
+            build(ref=False)
+        super().__init__(path)
+
+    def init_metric(self, z):
+        rc = self.lib.fo_init_metric(_i(z.param_int), _d(z.x), _d(z.y), _d(z.z), _d(z.ti),
+                                     _d(z.tj), _d(z.tk), _d(z.vol))
+        if rc != 0:
+            raise RuntimeError("fo_init_metric failed")
+
+    def lu_sgs(self, z, ind, rotmp, rop, drodm, mjrnewton=1):
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        rc = self.lib.fo_lu_sgs(_i(z.param_int), _d(z.param_real), _i(ind), _d(rotmp), _d(rop),
+                                _d(drodm), _d(z.ti), _d(z.tj), _d(z.tk), _d(z.coe),
+                                ctypes.c_int(mjrnewton))
+        if rc != 0:
+            raise RuntimeError("fo_lu_sgs failed")
+
+
+class Reference(_Backend):
+    """Reference Fortran kernels (oracle/_ref/libfastref.so)."""
+    prefix = "ref_"
+
+    @staticmethod
+    def available(path=REFLIB):
+        return os.path.exists(path)
+
+    def __init__(self, path=REFLIB):
+        super().__init__(path)
+
+    def classify(self, z):
+        """Run nature_geom_dom on a copy of the zone's param_int; return it."""
+        p = z.param_int.copy()
+        degen = np.zeros(int(p[42]), dtype=np.int32)
+        self.lib.ref_nature_geom_dom(_i(p), _d(z.param_real), _d(z.x), _d(z.y), _d(z.z), _i(degen))
+        return p, degen
+
+    def init_metric(self, z):
+        degen = np.zeros(int(z.param_int[42]), dtype=np.int32)
+        rc = self.lib.ref_init_metric(_i(z.param_int), _d(z.x), _d(z.y), _d(z.z), _i(degen),
+                                      _d(z.ti), _d(z.tj), _d(z.tk), _d(z.vol))
+        if rc != 0:
+            raise RuntimeError("ref_init_metric failed")
