@@ -1,0 +1,153 @@
+"""Benchmark: FastS NS residual + LU-SGS iteration, Taylor-Green vortex 256^3, FP64, one GPU.
+
+One step = one implicit time step with one Newton sub-iteration: viscosity and LU
+coefficients, fused inviscid (AUSM+ o3) + viscous face fluxes with the BDF residual, LU-SGS
+lower and upper sweeps with the Newton update, and the ghost-cell fill (periodic faces and
+corners).  Residual-only verification sub-iterations are not part of a step.
+
+Prints one JSON line.  Timing: W untimed steps, then exactly K steps between two device
+synchronisations (wall clock), cross-checked with HIP events on the solver stream.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# compulsory bytes per interior cell-iteration on a Cartesian zone (FP64, 5 equations):
+# residual 80 (read W, write R), LU lower 120, LU upper + update 120
+BYTES_PER_CELL = {"residual": 80.0, "lu": 240.0, "iteration": 320.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scheme", default="ausmpred")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--phase-reps", type=int, default=5)
+    return ap.parse_args()
+
+
+def make_zone(n, scheme):
+    from fast_amd import cases, params as P
+    num = P.Numerics(scheme=scheme, slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="global", time_step=1.0)
+    z = cases.tgv_zone(n, num)
+    z.param_real[P.DTC] = cases.tgv_dt(z, cfl=2.0)
+    return z
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_tgv256.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(kernel, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(n, scheme, iters):
+    """The reference FastS Fortran kernels (compiled from source into oracle/_ref) on the host,
+    one thread, on the same workload; falls back to the oracle's C restatement if the
+    reference library is not present."""
+    import oracle
+    from fast_amd import params as P
+    kind = "reference"
+    if oracle.Reference.available():
+        be = oracle.Reference()
+    else:
+        be = oracle.Oracle()
+        kind = "port"
+    z = make_zone(n, scheme)
+    be.init_metric(z)
+    be.apply_bc(z, z.ro)
+    z.ro_m1[:] = z.ro
+    z.ro_p1[:] = z.ro
+    t0 = time.perf_counter()
+    for s in range(iters):
+        be.substep(z, 1, first_it=0 if s == 0 else 1)
+        m1 = z.ro_m1.copy(); z.ro_m1[:] = z.ro; z.ro[:] = z.ro_p1; z.ro_p1[:] = m1
+    dt = time.perf_counter() - t0
+    ncell = n ** 3
+    return {"value": ncell * iters / dt / 1e6, "unit": "Mcell-iter/s", "cores": 1, "kind": kind,
+            "sample": "%d implicit iterations of the same TGV %d^3 %s case, single thread, %s"
+                      % (iters, n, scheme, platform.processor() or platform.machine())}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    from fast_amd import _lib
+    z = make_zone(a.n, a.scheme)
+    ctx = _lib.Context(0)
+    zid = ctx.add_zone(z)
+    ctx.apply_bc(zid, 0)
+    buf = np.zeros_like(z.ro)
+    ctx.get(zid, _lib.FIELD_RO, buf)
+    ctx.set(zid, _lib.FIELD_RO_M1, buf)
+    ctx.set(zid, _lib.FIELD_RO_P1, buf)
+    del buf
+    ncell = a.n ** 3
+    # warm-up (first step BDF1, then BDF2)
+    ctx.compute(0, 1, 1, nit_c=a.warmup, modulo_verif=0, first_it=0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ms_ev = ctx.compute_timed(a.warmup, 1, 1, nit_c=a.steps, modulo_verif=0, first_it=1)
+    ctx.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = ncell * a.steps / wall / 1e6
+    # health check of the state after the timed steps
+    tke = ctx.tke(zid, 0)
+    if not np.isfinite(tke).all():
+        raise RuntimeError("non-finite state after the benchmark")
+    # per-phase timings (HIP events on the solver stream)
+    phases = {}
+    for name, ph in (("coe", _lib.PHASE_COE), ("residual", _lib.PHASE_RESIDUAL),
+                     ("lu", _lib.PHASE_LU), ("bc", _lib.PHASE_BC)):
+        phases[name] = ctx.time_phase(zid, ph, reps=a.phase_reps)
+    ctx.close()
+    step_ms = wall / a.steps * 1e3
+    res_ms = phases["residual"]
+    achieved = BYTES_PER_CELL["residual"] * ncell / (res_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_residual", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles("k_residual"),
+                "bytes_per_cell": BYTES_PER_CELL["residual"],
+                "launch_ms": round(res_ms, 4)}
+    it_gbs = value * 1e6 * BYTES_PER_CELL["iteration"] / 1e9
+    out = {"metric": "Mcell-iter/s NS residual+LU-SGS at 256^3", "value": round(value, 2),
+           "unit": "Mcell-iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(step_ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
+           "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
+                      % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic"},
+           "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),
+           "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+           "event_ms_per_step": round(ms_ev / a.steps, 3),
+           "roofline": roofline}
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.n, a.scheme, a.cpu_iters)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

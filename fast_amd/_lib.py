@@ -1,0 +1,170 @@
+"""ctypes binding of libfastmi.so (the C ABI declared in include/fastmi.h).
+
+The library is required: there is no CPU fallback.  ``load()`` raises if the built library is
+missing, and every call that returns a non-zero status raises :class:`FastMIError` with the
+library's message.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfastmi.so")
+
+FIELD_RO, FIELD_RO_M1, FIELD_RO_P1, FIELD_MUT, FIELD_DRODM, FIELD_COE, FIELD_METRIC, FIELD_WIG = range(8)
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# exported symbols (checked by the CPU test suite against include/fastmi.h)
+SIGNATURES = {
+    "fm_last_error": (ctypes.c_char_p, []),
+    "fm_ctx_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "fm_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fm_zone_add": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                   _dp, _ip]),
+    "fm_zone_metric": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fm_zone_set_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long]),
+    "fm_zone_get_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long]),
+    "fm_zone_set_param_real": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int]),
+    "fm_zone_substep": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 7),
+    "fm_apply_bc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fm_compute": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 6),
+    "fm_compute_timed": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_float)]),
+    "fm_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "fm_residual_norms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "fm_enstrophy_tke": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
+    "fm_time_phase": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]),
+}
+PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
+
+
+class FastMIError(RuntimeError):
+    pass
+
+
+def load(path=LIBPATH):
+    """Load libfastmi.so (built by fast_amd.build); raise if it is not there."""
+    global _lib
+    if _lib is not None and path == LIBPATH:
+        return _lib
+    if not os.path.exists(path):
+        raise FastMIError("libfastmi.so not built (run __graft_entry__.build() or python -m fast_amd.build)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path == LIBPATH:
+        _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _lib.fm_last_error().decode() if _lib is not None else ""
+        raise FastMIError("fastmi error %d: %s" % (rc, msg))
+
+
+def dptr(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("float64 contiguous array required")
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a.dtype != np.int32 or not a.flags.c_contiguous:
+        raise TypeError("int32 contiguous array required")
+    return a.ctypes.data_as(_ip)
+
+
+class Context:
+    """A device context with its zones."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.fm_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        self.zones = []
+
+    def close(self):
+        if self.h:
+            self.lib.fm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_zone(self, hz):
+        """Register a HostZone, compute its metric on the device and upload its time levels."""
+        zid = ctypes.c_int()
+        p = np.ascontiguousarray(hz.param_int, dtype=np.int32)
+        r = np.ascontiguousarray(hz.param_real, dtype=np.float64)
+        check(self.lib.fm_zone_add(self.h, iptr(p), p.size, dptr(r), r.size, dptr(hz.x),
+                                   dptr(hz.y), dptr(hz.z), ctypes.byref(zid)))
+        check(self.lib.fm_zone_metric(self.h, zid.value))
+        for f, a in ((FIELD_RO, hz.ro), (FIELD_RO_M1, hz.ro_m1), (FIELD_RO_P1, hz.ro_p1)):
+            self.set(zid.value, f, a)
+        self.zones.append(hz)
+        return zid.value
+
+    def set(self, zone, field, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(self.lib.fm_zone_set_field(self.h, zone, field, dptr(a), a.size))
+
+    def get(self, zone, field, out):
+        check(self.lib.fm_zone_get_field(self.h, zone, field, dptr(out), out.size))
+        return out
+
+    def set_param_real(self, zone, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        check(self.lib.fm_zone_set_param_real(self.h, zone, dptr(r), r.size))
+
+    def substep(self, zone, nitcfg, nitrun=1, first_it=0, lssiter_verif=0, lexit_lu=0, stop=0):
+        check(self.lib.fm_zone_substep(self.h, zone, nitcfg, nitrun, first_it, lssiter_verif,
+                                       lexit_lu, stop))
+
+    def apply_bc(self, zone, level=0, nstep=1):
+        check(self.lib.fm_apply_bc(self.h, zone, level, nstep))
+
+    def compute(self, nitrun, nstep_deb, nstep_fin, nit_c=1, modulo_verif=0, first_it=1):
+        check(self.lib.fm_compute(self.h, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it))
+
+    def compute_timed(self, nitrun, nstep_deb, nstep_fin, nit_c=1, modulo_verif=0, first_it=1):
+        ms = ctypes.c_float()
+        check(self.lib.fm_compute_timed(self.h, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif,
+                                        first_it, ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(self.lib.fm_sync(self.h))
+
+    def residual_norms(self, zone):
+        out = np.zeros(10)
+        check(self.lib.fm_residual_norms(self.h, zone, dptr(out)))
+        return out
+
+    def tke(self, zone, level=0):
+        out = np.zeros(2)
+        check(self.lib.fm_enstrophy_tke(self.h, zone, level, dptr(out)))
+        return out
+
+    def time_phase(self, zone, phase, reps=10):
+        ms = ctypes.c_float()
+        check(self.lib.fm_time_phase(self.h, zone, phase, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def pull(self, zone, hz):
+        """Copy every device field of a zone into its HostZone arrays."""
+        for f, a in ((FIELD_RO, hz.ro), (FIELD_RO_M1, hz.ro_m1), (FIELD_RO_P1, hz.ro_p1),
+                     (FIELD_MUT, hz.xmut), (FIELD_DRODM, hz.drodm), (FIELD_COE, hz.coe),
+                     (FIELD_METRIC, hz.metric), (FIELD_WIG, hz.wig)):
+            self.get(zone, f, a)
+        return hz

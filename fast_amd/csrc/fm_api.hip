@@ -1,0 +1,557 @@
+// C ABI of the fastmi library: contexts, zones, and the sub-iteration / time-step drivers.
+// Host-side orchestration follows the reference drivers (Compute/computePT.cpp,
+// Compute/gsdr3.cpp, Compute/lhs.cpp, BC/BCzone.cpp) for the implemented options; all
+// numerics run in the HIP kernels of fm_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <string>
+#include <vector>
+#include "fastmi.h"
+#include "fm_kernels.hpp"
+
+using namespace fm;
+
+// param_solver.h slots
+enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 27, ILES = 28,
+       ITYPCP = 29, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
+       NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
+       EXPLOC = 54, NSSITER = 58, SLOPE = 64, PT_BC = 70, NB_RELAX = 71, IMPLICITSOLVER = 74,
+       LU_MATCH = 75, SENSORTYPE = 85, BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2 };
+enum { R_DTC = 0, R_GAMMA = 1, R_CVINF = 2, R_ROINF = 3, R_PINF = 4, R_VINF = 5, R_TINF = 6,
+       R_PRANDT = 10, R_XMUL0 = 11, R_TEMP0 = 12, R_CS = 13, R_CFL = 15, R_PSIROE = 34,
+       R_WIG_AMP = 72 };
+
+static thread_local std::string g_err;
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap; va_start(ap, fmt); vsnprintf(buf, sizeof buf, fmt, ap); va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail(FM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
+
+struct BcRec { int type, idir, fen[6]; };
+
+struct Zone {
+    std::vector<int> p;
+    std::vector<double> pr;
+    std::vector<BcRec> bcs;
+    KZ k;
+    double* ro[3];           // current N, N-1, N+1
+    double* ro_scratch;      // Newton target when the LU sweep works in place
+    double* norm_part; double* norm_out; int norm_blocks;
+    double h_norms[10]; int have_norms;
+    long n_metric;
+};
+
+struct fm_ctx {
+    int device;
+    hipStream_t stream;
+    std::vector<Zone*> zones;
+    hipEvent_t e0, e1;
+};
+
+extern "C" const char* fm_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fm_ctx_create(fm_ctx** out, int device) {
+    if (!out) return fail(FM_E_ARG, "null output");
+    HIPCHK(hipSetDevice(device));
+    fm_ctx* c = new fm_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
+    HIPCHK(hipEventCreate(&c->e0));
+    HIPCHK(hipEventCreate(&c->e1));
+    *out = c;
+    return FM_OK;
+}
+
+static void free_zone(Zone* z) {
+    double* bufs[] = {z->ro[0], z->ro[1], z->ro[2], z->ro_scratch, z->k.xmut, z->k.drodm,
+                      z->k.coe, z->k.wig, z->k.ti, (double*)z->k.x, (double*)z->k.y,
+                      (double*)z->k.z, z->norm_part, z->norm_out};
+    for (double* b : bufs) if (b) (void)hipFree(b);
+    delete z;
+}
+
+extern "C" int fm_ctx_destroy(fm_ctx* c) {
+    if (!c) return FM_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (Zone* z : c->zones) free_zone(z);
+    (void)hipEventDestroy(c->e0);
+    (void)hipEventDestroy(c->e1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FM_OK;
+}
+
+static void fill_kz_params(Zone* z) {
+    const int* p = z->p.data();
+    const double* r = z->pr.data();
+    KZ& k = z->k;
+    k.dtc = r[R_DTC]; k.gam = r[R_GAMMA]; k.cv = r[R_CVINF]; k.roinf = r[R_ROINF];
+    k.pinf = r[R_PINF]; k.vinf = r[R_VINF]; k.tinf = r[R_TINF]; k.prandt = r[R_PRANDT];
+    k.xmul0 = r[R_XMUL0]; k.temp0 = r[R_TEMP0]; k.cs = r[R_CS]; k.cfl = r[R_CFL];
+    k.psiroe = r[R_PSIROE]; k.wig_amp = r[R_WIG_AMP];
+    k.kflu = p[KFLUDOM]; k.slope = p[SLOPE]; k.iflow = p[IFLOW]; k.itypcp = p[ITYPCP];
+    k.dtloc = p[DTLOC]; k.rk = p[RK]; k.nssiter = p[NSSITER]; k.sensortype = p[SENSORTYPE];
+}
+
+static int dmalloc(double** p, long n) {
+    if (n <= 0) n = 1;
+    if (hipMalloc((void**)p, sizeof(double) * n) != hipSuccess)
+        return fail(FM_E_NOMEM, "hipMalloc of %ld doubles failed", n);
+    if (hipMemset(*p, 0, sizeof(double) * n) != hipSuccess) return fail(FM_E_HIP, "hipMemset");
+    return FM_OK;
+}
+
+extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr, int n_real,
+                           const double* x, const double* y, const double* zc, int* zone_id) {
+    if (!c || !p || !pr || !x || !y || !zc || n_int < 137 || n_real < 73)
+        return fail(FM_E_ARG, "fm_zone_add: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    int typ = p[ITYPZONE];
+    if (typ != 0 && typ != 2) return fail(FM_E_UNSUPPORTED, "zone type %d (3dhomo/2d) not implemented", typ);
+    if (p[NEQ] != 5 || p[IFLOW] < 1 || p[IFLOW] > 2 || p[ILES] != 0)
+        return fail(FM_E_UNSUPPORTED, "only 5-equation Euler / laminar NS is implemented");
+    if (p[LALE] != 0) return fail(FM_E_UNSUPPORTED, "moving meshes are not implemented");
+    if (p[EXPLOC] != 0) return fail(FM_E_UNSUPPORTED, "explicit local time stepping is not implemented");
+    if (p[ITYPCP] <= 1 && (p[IMPLICITSOLVER] != 0 || p[NB_RELAX] != 1 || p[LU_MATCH] != 0))
+        return fail(FM_E_UNSUPPORTED, "implicit solver must be LU-SGS without relaxation or overlap");
+    if (p[ITYPCP] == 2 && p[DTLOC] == 1) return fail(FM_E_UNSUPPORTED, "explicit scheme with local time step");
+    int kf = p[KFLUDOM], sl = p[SLOPE];
+    if (!((kf == 5 && sl >= 1 && sl <= 3) || ((kf == 1 || kf == 2) && sl == 2)))
+        return fail(FM_E_UNSUPPORTED, "scheme %d / slope %d not implemented", kf, sl);
+    if (p[NIJK + 3] != 2 || p[NIJK + 4] != 2) return fail(FM_E_UNSUPPORTED, "two ghost layers required");
+    Zone* z = new Zone();
+    memset(&z->k, 0, sizeof(KZ));
+    z->p.assign(p, p + n_int);
+    z->pr.assign(pr, pr + n_real);
+    z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
+    z->norm_part = z->norm_out = nullptr; z->have_norms = 0;
+    KZ& k = z->k;
+    k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
+    k.nit = p[NIJK]; k.njt = p[NIJK + 1]; k.nkt = p[NIJK + 2];
+    k.nf = p[NIJK + 3]; k.nfk = p[NIJK + 4];
+    k.nd = p[NDIMDX]; k.incj = p[NIJK]; k.inck = (long)p[NIJK] * p[NIJK + 1];
+    k.typ = typ;
+    k.mi = p[NIJK_MTR]; k.mj = p[NIJK_MTR + 1]; k.mk = p[NIJK_MTR + 2];
+    k.mf = p[NIJK_MTR + 3]; k.mfk = p[NIJK_MTR + 4]; k.ndm = p[NDIMDX_MTR];
+    k.xi = p[NIJK_XYZ]; k.xj = p[NIJK_XYZ + 1]; k.xf = p[NIJK_XYZ + 3]; k.xfk = p[NIJK_XYZ + 4];
+    fill_kz_params(z);
+    // boundary-condition table (FastC._BCcompact layout)
+    int pt_bcs = p[PT_BC];
+    int nb = pt_bcs > 0 ? p[pt_bcs] : 0;
+    for (int n = 0; n < nb; n++) {
+        int pt = p[pt_bcs + 1 + n];
+        BcRec b; b.type = p[pt + BC_TYPE]; b.idir = p[pt + BC_IDIR];
+        for (int q = 0; q < 6; q++) b.fen[q] = p[pt + BC_FEN + q];
+        if (!(b.type == 0 || b.type == 11 || ((b.type == 4 || b.type == 6) && p[IFLOW] > 1))) {
+            free_zone(z);
+            return fail(FM_E_UNSUPPORTED, "boundary condition type %d not implemented", b.type);
+        }
+        if (b.idir < 1 || b.idir > 6) { free_zone(z); return fail(FM_E_ARG, "BC direction %d", b.idir); }
+        if ((b.type == 4 || b.type == 6) && kf == 5) {
+            free_zone(z);
+            return fail(FM_E_UNSUPPORTED, "Roe flux with wall boundaries (wall-flux correction) not implemented");
+        }
+        z->bcs.push_back(b);
+    }
+    long nd = k.nd, nxyz = p[NDIMDX_XYZ];
+    long ndm = k.ndm, nmtr = ndm * (2 * p[NEQ_IJ] + p[NEQ_K] + 1);
+    z->n_metric = nmtr;
+    int rc = FM_OK;
+    double *xd, *yd, *zd, *mtr;
+    if ((rc = dmalloc(&z->ro[0], 5 * nd)) || (rc = dmalloc(&z->ro[1], 5 * nd)) ||
+        (rc = dmalloc(&z->ro[2], 5 * nd)) || (rc = dmalloc(&z->ro_scratch, 5 * nd)) ||
+        (rc = dmalloc(&k.xmut, nd)) || (rc = dmalloc(&k.drodm, 5 * nd)) ||
+        (rc = dmalloc(&k.coe, (long)p[NEQ_COE] * nd)) || (rc = dmalloc(&k.wig, 3 * nd)) ||
+        (rc = dmalloc(&mtr, nmtr)) || (rc = dmalloc(&xd, nxyz)) || (rc = dmalloc(&yd, nxyz)) ||
+        (rc = dmalloc(&zd, nxyz))) {
+        free_zone(z);
+        return rc;
+    }
+    k.ti = mtr; k.tj = mtr + ndm * p[NEQ_IJ]; k.tk = mtr + 2 * ndm * p[NEQ_IJ];
+    k.vol = mtr + ndm * (2 * p[NEQ_IJ] + p[NEQ_K]);
+    k.x = xd; k.y = yd; k.z = zd;
+    z->norm_blocks = 512;
+    if ((rc = dmalloc(&z->norm_part, 10 * z->norm_blocks)) || (rc = dmalloc(&z->norm_out, 16))) {
+        free_zone(z);
+        return rc;
+    }
+    HIPCHK(hipMemcpy(xd, x, sizeof(double) * nxyz, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(yd, y, sizeof(double) * nxyz, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(zd, zc, sizeof(double) * nxyz, hipMemcpyHostToDevice));
+    c->zones.push_back(z);
+    if (zone_id) *zone_id = (int)c->zones.size() - 1;
+    return FM_OK;
+}
+
+static Zone* get_zone(fm_ctx* c, int zone) {
+    if (!c || zone < 0 || zone >= (int)c->zones.size()) return nullptr;
+    return c->zones[zone];
+}
+
+extern "C" int fm_zone_set_param_real(fm_ctx* c, int zone, const double* pr, int n_real) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !pr || n_real < 73) return fail(FM_E_ARG, "fm_zone_set_param_real: bad arguments");
+    z->pr.assign(pr, pr + n_real);
+    fill_kz_params(z);
+    return FM_OK;
+}
+
+extern "C" int fm_zone_metric(fm_ctx* c, int zone) {
+    Zone* z = get_zone(c, zone);
+    if (!z) return fail(FM_E_ARG, "bad zone %d", zone);
+    HIPCHK(hipSetDevice(c->device));
+    launch_metric(z->k, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
+static double* field_ptr(Zone* z, int field, long* n) {
+    long nd = z->k.nd;
+    switch (field) {
+    case FM_FIELD_RO: *n = 5 * nd; return z->ro[0];
+    case FM_FIELD_RO_M1: *n = 5 * nd; return z->ro[1];
+    case FM_FIELD_RO_P1: *n = 5 * nd; return z->ro[2];
+    case FM_FIELD_MUT: *n = nd; return z->k.xmut;
+    case FM_FIELD_DRODM: *n = 5 * nd; return z->k.drodm;
+    case FM_FIELD_COE: *n = (long)z->p[NEQ_COE] * nd; return z->k.coe;
+    case FM_FIELD_METRIC: *n = z->n_metric; return z->k.ti;
+    case FM_FIELD_WIG: *n = 3 * nd; return z->k.wig;
+    }
+    return nullptr;
+}
+
+extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* host, long n) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !host) return fail(FM_E_ARG, "fm_zone_set_field: bad arguments");
+    long sz; double* d = field_ptr(z, field, &sz);
+    if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
+extern "C" int fm_zone_get_field(fm_ctx* c, int zone, int field, double* host, long n) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !host) return fail(FM_E_ARG, "fm_zone_get_field: bad arguments");
+    long sz; double* d = field_ptr(z, field, &sz);
+    if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, d, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// BC windows: indice_cl_sdm.for for a thread box equal to the zone, npass = 0
+static bool cl_window(const Zone* z, int idir, const int* fen, int w[6]) {
+    const KZ& k = z->k;
+    int sdm[6] = {1, k.ni, 1, k.nj, 1, k.nk};
+    if (!(sdm[0] <= fen[1] && sdm[1] >= fen[0] && sdm[2] <= fen[3] && sdm[3] >= fen[2] &&
+          sdm[4] <= fen[5] && sdm[5] >= fen[4])) return false;
+    for (int a = 0; a < 3; a++) {
+        w[2 * a] = fen[2 * a] > sdm[2 * a] ? fen[2 * a] : sdm[2 * a];
+        w[2 * a + 1] = fen[2 * a + 1] < sdm[2 * a + 1] ? fen[2 * a + 1] : sdm[2 * a + 1];
+    }
+    int f = k.nf, fk = k.nfk;
+    switch (idir) {
+    case 1: w[1] = w[0] - 1; w[0] = w[0] - f; break;
+    case 2: w[1] = w[0] + f; w[0] = w[0] + 1; break;
+    case 3: w[3] = w[2] - 1; w[2] = w[2] - f; break;
+    case 4: w[3] = w[2] + f; w[2] = w[2] + 1; break;
+    case 5: w[5] = w[4] - 1; w[4] = w[4] - fk; break;
+    default: w[5] = w[4] + fk; w[4] = w[4] + 1; break;
+    }
+    return true;
+}
+
+// BCzone.cpp dispatch (lrhs = 1: RHS of the implicit step, returns the LU range;
+// lrhs = 0: ghost-cell fill of a state).
+static void bczone(fm_ctx* c, Zone* z, int lrhs, double* rop, int shift_lu[6], int* lcorner_out) {
+    const KZ& k = z->k;
+    int ind_dm[6] = {1, k.ni, 1, k.nj, 1, k.nk};
+    int lskip[6] = {1, 1, 1, 1, 1, 1}, ificmax[6] = {0, 0, 0, 0, 0, 0};
+    int lcorner = 1;
+    for (const BcRec& b : z->bcs) {
+        int ip = b.idir - 1, t = b.type;
+        if (lrhs == 1 && ((t >= 3 && t <= 7) || t == 12 || t == 18 || t == 30)) {
+            lskip[ip] = 0; lcorner = 0;
+            if ((t >= 3 && t <= 7) || t == 12) ificmax[ip] = 1;
+        }
+    }
+    if (lrhs == 1) {
+        for (int a = 0; a < 3; a++) {
+            shift_lu[2 * a] = ind_dm[2 * a] - (1 - lskip[2 * a]) * ificmax[2 * a];
+            shift_lu[2 * a + 1] = ind_dm[2 * a + 1] + (1 - lskip[2 * a + 1]) * ificmax[2 * a + 1];
+        }
+        for (int idir = 0; idir < 6; idir++) {
+            if (lskip[idir] != 0) continue;
+            int fen[6];
+            for (int l = 0; l < 6; l++) fen[l] = ind_dm[l];
+            if (idir % 2 == 0) { fen[idir] = 1; fen[idir + 1] = 1; } else fen[idir - 1] = fen[idir];
+            BcArgs a; a.idir = idir + 1; a.kind = 0; a.lrhs = 1;
+            if (cl_window(z, idir + 1, fen, a.w)) launch_bc_face(k, rop, a, c->stream);
+        }
+    }
+    for (const BcRec& b : z->bcs) {
+        if (!(lskip[b.idir - 1] == 0 || lrhs == 0)) continue;
+        int t = b.type;
+        if (lrhs == 1 && (t == 0 || t == 1 || t == 2 || t == 10 || t == 11 || t == 14 || t == 16 ||
+                          t == 17 || t == 20 || t == 21)) t = 0;
+        BcArgs a; a.idir = b.idir; a.lrhs = lrhs;
+        a.kind = (t == 0) ? 0 : (t == 11) ? 2 : 1;
+        if (cl_window(z, b.idir, b.fen, a.w)) launch_bc_face(k, rop, a, c->stream);
+    }
+    if (lcorner_out) *lcorner_out = lcorner;
+}
+
+static void coins(fm_ctx* c, Zone* z, double* rop, const int* ind) {
+    Box L;
+    const KZ& k = z->k;
+    L.i0 = ind[0] > 1 ? ind[0] : 1; L.j0 = ind[2] > 1 ? ind[2] : 1; L.k0 = ind[4] > 1 ? ind[4] : 1;
+    L.i1 = ind[1] < k.ni ? ind[1] : k.ni; L.j1 = ind[3] < k.nj ? ind[3] : k.nj; L.k1 = ind[5] < k.nk ? ind[5] : k.nk;
+    launch_coins(k, rop, L, c->stream);
+}
+
+// One sub-iteration of one zone (gsdr3.cpp:152-444 with navier_stokes_struct.for and lhs.cpp).
+static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_verif, int lexit_lu, int stop) {
+    KZ& k = z->k;
+    hipStream_t st = c->stream;
+    int implicit = k.itypcp <= 1;
+    double* ro = z->ro[0]; double* ro_m1 = z->ro[1]; double* ro_p1 = z->ro[2];
+    double *ro_ssiter, *ro_cl;
+    if (nitcfg == 1) { ro_ssiter = ro; ro_cl = ro_p1; }
+    else if (implicit) { ro_ssiter = ro_p1; ro_cl = ro_p1; }
+    else if (nitcfg % 2 == 0) { ro_ssiter = ro_p1; ro_cl = ro; }
+    else { ro_ssiter = ro; ro_cl = ro_p1; }
+    int sdm[6] = {1, k.ni, 1, k.nj, 1, k.nk};
+    int lamin = k.iflow == 2;
+    if (nitcfg == 1) {
+        if (implicit) {
+            Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
+            launch_coe(k, ro_ssiter, first_it, k.dtloc ? 2 : 1, b, lamin, st);
+        } else {
+            Box b = {0, k.ni + 1, 0, k.nj + 1, 0, k.nk + 1};
+            launch_coe(k, ro_ssiter, first_it, 0, b, lamin, st);
+        }
+    }
+    if (k.kflu == 2 && nitcfg == 1) launch_wiggle(k, ro_ssiter, st);
+    RhsArgs a;
+    memset(&a, 0, sizeof a);
+    a.rop = ro_ssiter; a.rop_n = ro; a.rop_n1 = ro_m1;
+    a.implicit = implicit; a.nitcfg = nitcfg;
+    a.zero_init = (implicit || nitcfg == 1);
+    a.coefH = 0.;
+    if (k.rk == 3) { if (nitcfg == 2) a.coefH = -0.6830127018922193; else if (nitcfg == 3) a.coefH = -4. / 3.; }
+    else if (k.rk == 2) { if (nitcfg == 2) a.coefH = -1.; }
+    a.coefW = 1.;
+    if (k.rk == 2 && nitcfg == 2) a.coefW = 0.5;
+    if (k.rk == 3) a.coefW = nitcfg == 1 ? 0.5 : nitcfg == 2 ? 0.9106836025229591 : 0.3660254037844387;
+    if (first_it == 0 || k.dtloc == 1) { a.c1 = -1.; a.c2 = 1.; a.c3 = 0.; }
+    else { a.c1 = -1.5; a.c2 = 2.; a.c3 = -0.5; }
+    if (nitcfg % 2 == 1) { a.rk_out = ro_p1; a.rk_in = ro; }
+    else { a.rk_out = ro; a.rk_in = ro_p1; }
+    if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
+    HIPCHK(hipGetLastError());
+    if (stop == 1) return FM_OK;
+    if (implicit) {
+        int shift_lu[6], lcorner = 1;
+        bczone(c, z, 1, k.drodm, shift_lu, &lcorner);
+        if (lcorner == 0) coins(c, z, k.drodm, shift_lu);
+        if (lssiter_verif == 1) {
+            launch_norms(k, k.drodm, z->norm_part, z->norm_blocks, z->norm_out, st);
+            z->have_norms = 1;
+        }
+        if (stop == 2) return FM_OK;
+        if (lexit_lu == 0) {
+            LuArgs la;
+            la.i0 = shift_lu[0]; la.i1 = shift_lu[1]; la.j0 = shift_lu[2]; la.j1 = shift_lu[3];
+            la.k0 = shift_lu[4]; la.k1 = shift_lu[5];
+            la.dw = k.drodm; la.newton = 1;
+            la.rop = ro_ssiter;
+            if (ro_ssiter == ro_p1) {
+                // in place (sub-iterations >= 2): the K terms of the upper sweep see the state
+                // already updated on plane k+1 (invlu_u.for + mjr_newton per k plane)
+                la.rop_k = z->ro_scratch; la.out = z->ro_scratch;
+            } else {
+                la.rop_k = ro_ssiter; la.out = ro_p1;
+            }
+            launch_lu(k, la, st);
+            HIPCHK(hipGetLastError());
+            if (la.out == z->ro_scratch) {
+                // copy the updated cells back (whole LU box: ghost layers the range extends into)
+                HIPCHK(hipMemcpyAsync(ro_p1, z->ro_scratch, sizeof(double) * 5 * k.nd, hipMemcpyDeviceToDevice, st));
+            }
+        }
+    }
+    if (stop == 3) return FM_OK;
+    if (lexit_lu == 0) {
+        int shift[6];
+        bczone(c, z, 0, ro_cl, shift, nullptr);
+        coins(c, z, ro_cl, sdm);
+    }
+    HIPCHK(hipGetLastError());
+    return FM_OK;
+}
+
+extern "C" int fm_zone_substep(fm_ctx* c, int zone, int nitcfg, int nitrun, int first_it,
+                               int lssiter_verif, int lexit_lu, int stop) {
+    (void)nitrun;
+    Zone* z = get_zone(c, zone);
+    if (!z) return fail(FM_E_ARG, "bad zone %d", zone);
+    if (nitcfg < 1 || nitcfg > z->k.nssiter) return fail(FM_E_ARG, "nitcfg %d out of range", nitcfg);
+    HIPCHK(hipSetDevice(c->device));
+    int rc = substep(c, z, nitcfg, first_it, lssiter_verif, lexit_lu, stop);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
+extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
+    (void)nstep;
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2) return fail(FM_E_ARG, "fm_apply_bc: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    int shift[6], sdm[6] = {1, z->k.ni, 1, z->k.nj, 1, z->k.nk};
+    bczone(c, z, 0, z->ro[level], shift, nullptr);
+    coins(c, z, z->ro[level], sdm);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
+// computePT.cpp:534-679 for the implemented options: nit_c steps; in each step the
+// sub-iterations nstep_deb..nstep_fin of every zone; the last implicit sub-iteration only
+// computes the residual (lexit_lu) and only on verification steps; time levels rotate.
+static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
+    for (int it = 0; it < nit_c; it++) {
+        int nitrun_loc = nitrun * nit_c + it;
+        int verif = modulo_verif > 0 && (nitrun_loc % modulo_verif == 0);
+        for (int nstep = nstep_deb; nstep <= nstep_fin; nstep++) {
+            for (Zone* z : c->zones) {
+                int implicit = z->k.itypcp <= 1;
+                int nssiter = z->k.nssiter;
+                int lssiter_verif = 0, lexit_lu = 0;
+                if (implicit && nstep == nssiter) {
+                    if (!verif) continue;           // skip (computePT.cpp:581-582)
+                    lssiter_verif = 1; lexit_lu = 1;
+                }
+                int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 0);
+                if (rc) return rc;
+            }
+        }
+        first_it = 1;
+        for (Zone* z : c->zones) {
+            double* m1 = z->ro[1]; z->ro[1] = z->ro[0]; z->ro[0] = z->ro[2]; z->ro[2] = m1;
+        }
+    }
+    return FM_OK;
+}
+
+extern "C" int fm_compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c,
+                          int modulo_verif, int first_it) {
+    if (!c) return fail(FM_E_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    return compute(c, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it);
+}
+
+extern "C" int fm_compute_timed(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c,
+                                int modulo_verif, int first_it, float* ms) {
+    if (!c) return fail(FM_E_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->e0, c->stream));
+    int rc = compute(c, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->e1, c->stream));
+    HIPCHK(hipEventSynchronize(c->e1));
+    if (ms) HIPCHK(hipEventElapsedTime(ms, c->e0, c->e1));
+    return FM_OK;
+}
+
+extern "C" int fm_sync(fm_ctx* c) {
+    if (!c) return fail(FM_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
+extern "C" int fm_residual_norms(fm_ctx* c, int zone, double* out) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !out) return fail(FM_E_ARG, "fm_residual_norms: bad arguments");
+    if (!z->have_norms) return fail(FM_E_ARG, "no residual computed yet");
+    double h[10];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(h, z->norm_out, sizeof h, hipMemcpyDeviceToHost));
+    double ncell = (double)z->k.ni * z->k.nj * z->k.nk;
+    double dtc = z->k.dtc;
+    for (int e = 0; e < 5; e++) {
+        out[e] = sqrt(h[e] / ncell) / dtc;
+        out[5 + e] = h[5 + e] / dtc;
+    }
+    return FM_OK;
+}
+
+extern "C" int fm_enstrophy_tke(fm_ctx* c, int zone, int level, double* out) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !out || level < 0 || level > 2) return fail(FM_E_ARG, "fm_enstrophy_tke: bad arguments");
+    int nblk = z->norm_blocks;
+    launch_tke(z->k, z->ro[level], z->norm_part, nblk, c->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(2 * nblk);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(h.data(), z->norm_part, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost));
+    double sk = 0., sv = 0.;
+    for (int b = 0; b < nblk; b++) { sk += h[2 * b]; sv += h[2 * b + 1]; }
+    out[0] = sk; out[1] = sv;
+    return FM_OK;
+}
+
+// Phase timing (benchmark support): the kernels of one phase of sub-iteration 1, repeated.
+extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms) {
+    Zone* z = get_zone(c, zone);
+    if (!z || reps < 1 || !ms) return fail(FM_E_ARG, "fm_time_phase: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    KZ& k = z->k;
+    hipStream_t st = c->stream;
+    int implicit = k.itypcp <= 1;
+    int lamin = k.iflow == 2;
+    double* ro = z->ro[0];
+    RhsArgs a;
+    memset(&a, 0, sizeof a);
+    a.rop = ro; a.rop_n = ro; a.rop_n1 = z->ro[1]; a.implicit = implicit; a.nitcfg = 1;
+    a.zero_init = 1; a.coefW = 0.5; a.rk_in = ro; a.rk_out = z->ro_scratch;
+    a.c1 = -1.5; a.c2 = 2.; a.c3 = -0.5;
+    LuArgs la;
+    la.i0 = 1; la.i1 = k.ni; la.j0 = 1; la.j1 = k.nj; la.k0 = 1; la.k1 = k.nk;
+    la.dw = k.drodm; la.rop = ro; la.rop_k = ro; la.out = z->ro_scratch; la.newton = 1;
+    Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipEventRecord(c->e0, st));
+    for (int r = 0; r < reps; r++) {
+        if (phase == FM_PHASE_COE) launch_coe(k, ro, 1, implicit ? (k.dtloc ? 2 : 1) : 0, b, lamin, st);
+        else if (phase == FM_PHASE_RESIDUAL) {
+            if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel");
+        } else if (phase == FM_PHASE_LU) {
+            if (!implicit) return fail(FM_E_ARG, "LU phase needs an implicit zone");
+            launch_lu(k, la, st);
+        } else if (phase == FM_PHASE_BC) {
+            int shift[6], sdm[6] = {1, k.ni, 1, k.nj, 1, k.nk};
+            bczone(c, z, 0, z->ro_scratch, shift, nullptr);
+            coins(c, z, z->ro_scratch, sdm);
+        } else return fail(FM_E_ARG, "unknown phase %d", phase);
+    }
+    HIPCHK(hipEventRecord(c->e1, st));
+    HIPCHK(hipEventSynchronize(c->e1));
+    HIPCHK(hipGetLastError());
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->e0, c->e1));
+    *ms = t / reps;
+    return FM_OK;
+}

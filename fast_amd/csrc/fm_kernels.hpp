@@ -1,0 +1,48 @@
+// Kernel launch interface shared by the kernels and the C ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fm_device.hpp"
+
+namespace fm {
+
+struct Box { int i0, i1, j0, j1, k0, k1; };
+
+struct RhsArgs {
+    const double* rop;      // state feeding the RHS (rop_ssiter)
+    const double* rop_n;    // time level N   (implicit)
+    const double* rop_n1;   // time level N-1 (implicit)
+    const double* rk_in;    // explicit stage: base state
+    double* rk_out;         // explicit stage: new state
+    int zero_init;          // drodm starts at 0 (else coefH * drodm)
+    double coefH, coefW;    // RK coefficients
+    int implicit, nitcfg;
+    double c1, c2, c3;      // BDF coefficients
+};
+
+struct LuArgs {
+    int i0, i1, j0, j1, k0, k1;   // LU range (shift_lu)
+    double* dw;                   // drodm in, dW out
+    const double* rop;            // state for the Jacobians (J, I terms) and Newton base
+    const double* rop_k;          // state for the K terms of the upper sweep
+    double* out;                  // Newton-updated state
+    int newton;
+};
+
+struct BcArgs {
+    int w[6];     // ghost window (1-based cell indices)
+    int idir;     // 1..6
+    int kind;     // 0 extrapolate, 1 viscous wall mirror, 2 periodic
+    int lrhs;     // 1: applied to the RHS (zeroes extrapolations)
+};
+
+void launch_metric(const KZ& z, hipStream_t st);
+void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, int lamin, hipStream_t st);
+void launch_wiggle(const KZ& z, const double* rop, hipStream_t st);
+int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st);
+void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
+void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);
+void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st);
+void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double* out, hipStream_t st);
+void launch_tke(const KZ& z, const double* ro, double* part, int nblk, hipStream_t st);
+
+}  // namespace fm

@@ -28,6 +28,25 @@ class HostZone:
         classify_zone(p, self.x, self.y, self.z)
         self.param_int = p
         self.param_real = r
+        self._allocate()
+
+    @classmethod
+    def from_arrays(cls, param_int, param_real, x, y, z, name="zone"):
+        """Rebuild a zone from stored descriptor vectors and coordinates (fixtures, restarts)."""
+        self = cls.__new__(cls)
+        self.name = name
+        self.numerics = None
+        self.state = None
+        self.param_int = np.ascontiguousarray(param_int, dtype=np.int32).copy()
+        self.param_real = np.ascontiguousarray(param_real, dtype=np.float64).copy()
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self._allocate()
+        return self
+
+    def _allocate(self):
+        p = self.param_int
         nd = int(p[P.NDIMDX])
         self.nd = nd
         self.ro = np.zeros(5 * nd)

@@ -1,0 +1,102 @@
+/*
+ * fastmi -- MI355X-native FastS structured Navier-Stokes iteration, C ABI.
+ *
+ * The drop-in boundary of the FastS compute path.  A context owns device memory for a set
+ * of structured zones; each zone is described by the reference's own Parameter_int /
+ * Parameter_real vectors (slot numbers of Fast/FastS/FastS/param_solver.h:80-312), and its
+ * fields use the reference layout (five primitive variables rho,u,v,w,T one after the other,
+ * each NDIMDX doubles, i fastest, ghost layers included).  Host buffers belong to the caller;
+ * device mirrors belong to the context.  Every entry point returns 0 on success or a negative
+ * FM_E* code (never exits), and fm_last_error() describes the last failure.
+ *
+ * Reference interfaces replaced (all under Fast/FastS/FastS/ unless noted):
+ *   fm_zone_add          allocate_metric  (Metric/allocate_metric.cpp:34-481, PyTree.py:267)
+ *   fm_zone_metric       init_metric      (Fast/FastC/FastC/Metric/init_metric.cpp:34-366)
+ *   fm_compute           fasts._computePT (Compute/computePT.cpp:44-721, PyTree.py:42-236)
+ *   fm_zone_substep      gsdr3 for one sub-iteration (Compute/gsdr3.cpp:57-484)
+ *   fm_apply_bc          fasts._applyBC   (BC/applyBC.cpp:33-255, PyTree.py:1155)
+ *   fm_residual_norms    display_ss_iteration (DISPLAY/display_ss_iteration.cpp:29-191)
+ *   fm_enstrophy_tke     fasts._computePT_enstrophy (POST/computePT_enstrophy.cpp:127-227)
+ *   fm_zone_set_field / fm_zone_get_field   the numpy buffers the reference borrows in place
+ *                        (K_PYTREE::getValueAF, computePT.cpp:225-369): here they are copied
+ *                        to and from device memory explicitly.
+ */
+#ifndef FASTMI_H
+#define FASTMI_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_OK 0
+#define FM_E_ARG -1        /* invalid argument */
+#define FM_E_UNSUPPORTED -2 /* option outside the implemented path */
+#define FM_E_HIP -3        /* HIP runtime error */
+#define FM_E_NOMEM -4      /* device allocation failed */
+
+/* field identifiers for fm_zone_set_field / fm_zone_get_field */
+#define FM_FIELD_RO 0      /* time level N   (5*NDIMDX) */
+#define FM_FIELD_RO_M1 1   /* time level N-1 (5*NDIMDX) */
+#define FM_FIELD_RO_P1 2   /* time level N+1 (5*NDIMDX) */
+#define FM_FIELD_MUT 3     /* laminar viscosity (NDIMDX) */
+#define FM_FIELD_DRODM 4   /* right-hand side / increment (5*NDIMDX) */
+#define FM_FIELD_COE 5     /* time step and LU coefficients (NEQ_COE*NDIMDX) */
+#define FM_FIELD_METRIC 6  /* ti | tj | tk | vol (NDIMDX_MTR*(2*NEQ_IJ+NEQ_K+1)) */
+#define FM_FIELD_WIG 7     /* SENSEUR sensor (3*NDIMDX) */
+
+typedef struct fm_ctx fm_ctx;
+
+const char* fm_last_error(void);
+int fm_ctx_create(fm_ctx** out, int device);
+int fm_ctx_destroy(fm_ctx* ctx);
+
+/* Register a zone.  param_int/param_real are copied (n_int / n_real entries, BC table
+   included).  x, y, z: vertex coordinates with ghost vertices (NDIMDX_XYZ doubles each). */
+int fm_zone_add(fm_ctx* ctx, const int* param_int, int n_int, const double* param_real,
+                int n_real, const double* x, const double* y, const double* z, int* zone_id);
+/* Face vectors and volumes on the device (cp_tijk, cp_vol). */
+int fm_zone_metric(fm_ctx* ctx, int zone);
+int fm_zone_set_field(fm_ctx* ctx, int zone, int field, const double* host, long n);
+int fm_zone_get_field(fm_ctx* ctx, int zone, int field, double* host, long n);
+/* Update a zone's param_real (time step, CFL ...) without re-allocating. */
+int fm_zone_set_param_real(fm_ctx* ctx, int zone, const double* param_real, int n_real);
+
+/* One sub-iteration of one zone (gsdr3).  stop: 0 full, 1 after the RHS, 2 after the RHS
+   boundary conditions and residual norms, 3 after LU-SGS.  Synchronous. */
+int fm_zone_substep(fm_ctx* ctx, int zone, int nitcfg, int nitrun, int first_it,
+                    int lssiter_verif, int lexit_lu, int stop);
+/* Ghost-cell fill of a time level (0 N, 1 N-1, 2 N+1). */
+int fm_apply_bc(fm_ctx* ctx, int zone, int level, int nstep);
+
+/* nit_c time steps of all zones, sub-iterations nstep_deb..nstep_fin each, time levels
+   rotated after each step (computePT.cpp:534-679).  modulo_verif: residuals are computed at
+   the last sub-iteration of steps where (nitrun+it) % modulo_verif == 0.  Asynchronous on
+   the context stream; fm_sync waits. */
+int fm_compute(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int nit_c,
+               int modulo_verif, int first_it);
+int fm_sync(fm_ctx* ctx);
+
+/* Residual of the last verified sub-iteration: out[0..4] L2, out[5..9] Linf per equation,
+   already divided by DTC like display_ss_iteration.cpp:140-141. */
+int fm_residual_norms(fm_ctx* ctx, int zone, double* out);
+/* Cell sums over the interior: out[0] = sum 1/2 rho |u|^2 vol, out[1] = sum vol. */
+int fm_enstrophy_tke(fm_ctx* ctx, int zone, int level, double* out);
+
+/* Timing helper for benchmarks: events recorded on the context stream around fm_compute. */
+int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int nit_c,
+                     int modulo_verif, int first_it, float* ms);
+
+/* Per-phase timing for the roofline report: runs one phase of the first sub-iteration of a
+   zone `reps` times on the context stream between two HIP events and returns the mean
+   milliseconds per run.  phase: FM_PHASE_COE (viscosity + time step / LU coefficients),
+   FM_PHASE_RESIDUAL (fused fluxes + balance + BDF update), FM_PHASE_LU (lower + upper sweeps
+   with the Newton update), FM_PHASE_BC (ghost-cell fill).  No counterpart in the reference. */
+#define FM_PHASE_COE 0
+#define FM_PHASE_RESIDUAL 1
+#define FM_PHASE_LU 2
+#define FM_PHASE_BC 3
+int fm_time_phase(fm_ctx* ctx, int zone, int phase, int reps, float* ms_per_run);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

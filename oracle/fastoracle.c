@@ -1345,6 +1345,7 @@ static int fo_bczone(int* p, double* pr, int lrhs, int* shift_lu, double* rop, i
         if (lrhs == 1 && (t == 0 || t == 1 || t == 2 || t == 10 || t == 11 || t == 14 || t == 16 ||
                           t == 17 || t == 20 || t == 21)) t = 0;
         if (t == 0 || t == 11) bc_apply(Z, p, t, idir, lrhs, cl, rop);
+        else if (p[KFLUDOM] == 5) return -1;  /* Roe wall-flux correction (correct_flux.for) */
         else if ((t == 6 || t == 4) && p[IFLOW] > 1) bc_apply(Z, p, 6, idir, lrhs, cl, rop);
         else return -1;
     }

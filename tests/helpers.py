@@ -1,0 +1,52 @@
+"""Shared case builders for the test suite."""
+import numpy as np
+
+from fast_amd import params as P, cases, mesh as M
+from fast_amd.zone import HostZone
+
+
+def jittered_box(n, num, flow="lamin", seed=3, amp=0.05, bcs=None, perturb=0.02):
+    """TGV field on a [0, 2pi]^3 box whose interior vertices are jittered (a 3dfull zone)."""
+    st = cases.tgv_state()
+    x, y, z = M.cart_vertices(n, (0, 0, 0), (2 * np.pi,) * 3)
+    rng = np.random.default_rng(seed)
+    nv = (n[0] + 5, n[1] + 5, n[2] + 5)
+    X = x.reshape(nv, order="F"); Y = y.reshape(nv, order="F"); Z = z.reshape(nv, order="F")
+    h = 2 * np.pi / np.array(n)
+    s = (slice(1, -1),) * 3
+    X[s] += amp * h[0] * rng.uniform(-1, 1, X[s].shape)
+    Y[s] += amp * h[1] * rng.uniform(-1, 1, Y[s].shape)
+    Z[s] += amp * h[2] * rng.uniform(-1, 1, Z[s].shape)
+    x, y, z = X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")
+    bcs = cases.ALL_PERIODIC if bcs is None else bcs
+    zz = HostZone("jbox", n[0], n[1], n[2], num, st, x, y, z, bcs=bcs, flow=flow)
+    xc, yc, zc = M.cell_centers_from_vertices(x, y, z, nv)
+    w = M.tgv_primitives(xc, yc, zc, st)
+    w = w + perturb * rng.uniform(-1, 1, w.size)
+    zz.ro[:] = w; zz.ro_m1[:] = w; zz.ro_p1[:] = w
+    return zz
+
+
+WALL_J = [("BCautoperiod", "imin", None), ("BCautoperiod", "imax", None),
+          ("BCWallViscous", "jmin", None), ("BCWallViscous", "jmax", None),
+          ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+EXTRAP_I = [("BCExtrapolate", "imin", None), ("BCExtrapolate", "imax", None),
+            ("BCWallViscous", "jmin", None), ("BCWallViscous", "jmax", None),
+            ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+
+
+def numerics(scheme, mode, **kw):
+    if mode == "explicit":
+        return P.Numerics(scheme=scheme, temporal_scheme="explicit", time_step=0.01, **kw)
+    if mode == "bdf":
+        return P.Numerics(scheme=scheme, temporal_scheme="implicit", ss_iteration=1, time_step=0.01, **kw)
+    if mode == "bdf_ss2":
+        return P.Numerics(scheme=scheme, temporal_scheme="implicit", ss_iteration=2, time_step=0.01, **kw)
+    if mode == "local":
+        return P.Numerics(scheme=scheme, temporal_scheme="implicit", ss_iteration=1,
+                          time_step_nature="local", cfl=5.0, **kw)
+    raise ValueError(mode)
+
+
+def rotate(z):
+    m1 = z.ro_m1.copy(); z.ro_m1[:] = z.ro; z.ro[:] = z.ro_p1; z.ro_p1[:] = m1

@@ -1,0 +1,167 @@
+"""HIP kernels against the oracle, bit for bit (FP64, contraction off on both sides).
+
+Each case builds the same zone twice, runs one or more sub-iterations through the C ABI on
+the GPU and through the oracle's C restatement on the CPU, and compares every interior field
+(and, after a full sub-iteration, the ghost cells) with exact equality.
+"""
+import numpy as np
+import pytest
+
+from fast_amd import params as P, cases
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctxmod():
+    from fast_amd import _lib
+    return _lib
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    return oracle.Oracle()
+
+
+def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True):
+    zo = make()
+    zg = make()
+    orc.init_metric(zo)
+    orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    ctx = ctxmod.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    # same N-1 / N+1 initialisation as the oracle side
+    tmp = np.zeros_like(zg.ro)
+    ctx.get(zid, ctxmod.FIELD_RO, tmp)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, tmp)
+    ctx.set(zid, ctxmod.FIELD_RO_P1, tmp)
+    rdm = np.zeros(zo.rdm_size())
+    for step in range(nsteps):
+        fi = first_it if step == 0 else 1
+        for it in range(1, nsub + 1):
+            orc.substep(zo, it, first_it=fi, lssiter_verif=1 if verif else 0, rdm=rdm, stop=stop)
+            ctx.substep(zid, it, first_it=fi, lssiter_verif=1 if verif else 0, stop=stop)
+        if nsteps > 1:
+            H.rotate(zo)
+            # rotate the device levels the same way (N-1 <- N <- N+1)
+            a = [np.zeros_like(zo.ro) for _ in range(3)]
+            for f, buf in zip((ctxmod.FIELD_RO, ctxmod.FIELD_RO_M1, ctxmod.FIELD_RO_P1), a):
+                ctx.get(zid, f, buf)
+            ctx.set(zid, ctxmod.FIELD_RO_M1, a[0])
+            ctx.set(zid, ctxmod.FIELD_RO, a[2])
+            ctx.set(zid, ctxmod.FIELD_RO_P1, a[1])
+    ctx.pull(zid, zg)
+    norms = ctx.residual_norms(zid) if (verif and zo.param_int[P.ITYPCP] <= 1 and stop != 1) else None
+    ctx.close()
+    return zo, zg, rdm, norms
+
+
+def assert_same(zo, zg, fields=("ro", "ro_m1", "ro_p1", "drodm"), ghosts=False):
+    assert np.array_equal(zo.metric, zg.metric)
+    for f in fields:
+        a, b = getattr(zo, f), getattr(zg, f)
+        if ghosts and f != "drodm":
+            # the reference also writes -F into the drodm of the first ghost layer; those
+            # values are never used, so drodm is compared on the interior only
+            ok = np.array_equal(a, b)
+        else:
+            ok = np.array_equal(zo.interior(a), zg.interior(b))
+        if not ok:
+            d = np.abs(zo.interior(a) - zg.interior(b))
+            raise AssertionError("%s differs: max %.3e at %s" % (f, d.max(), np.unravel_index(d.argmax(), d.shape)))
+
+
+SCHEMES = ["roe_min", "roe_kap", "roe_nul", "ausmpred", "senseur"]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("flow", ["lamin", "euler"])
+@pytest.mark.parametrize("stop", [1, 3, 0])
+def test_tgv_cart_implicit_one_substep(ctxmod, orc, scheme, flow, stop):
+    mk = lambda: cases.tgv_zone((13, 11, 9), H.numerics(scheme, "bdf"), perturb=0.02, flow=flow)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, 1, stop=stop)
+    assert_same(zo, zg, ghosts=(stop == 0))
+    assert np.array_equal(zo.interior(zo.xmut, 1), zg.interior(zg.xmut, 1))
+    assert np.array_equal(zo.interior(zo.coe), zg.interior(zg.coe))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "bdf_ss2", "local"])
+def test_tgv_cart_modes_two_steps(ctxmod, orc, scheme, mode):
+    nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
+    mk = lambda: cases.tgv_zone((12, 10, 8), H.numerics(scheme, mode), perturb=0.02)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
+@pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "bdf_ss2", "local"])
+def test_3dfull_jittered(ctxmod, orc, scheme, mode):
+    nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
+    mk = lambda: H.jittered_box((11, 9, 8), H.numerics(scheme, mode))
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert zo.zone_type == P.ZONE_3DFULL
+    assert_same(zo, zg, ghosts=True)
+
+
+@pytest.mark.parametrize("bcs", ["wall_j", "extrap_i"])
+@pytest.mark.parametrize("mode", ["bdf", "local", "bdf_ss2"])
+def test_walls_lu_extension(ctxmod, orc, bcs, mode):
+    """Viscous walls extend the LU range by one ghost layer with the RHS mirrored there
+    (BCzone.cpp:115-253); extrapolated faces zero the RHS ghosts."""
+    b = H.WALL_J if bcs == "wall_j" else H.EXTRAP_I
+    nsub = 2 if mode == "bdf_ss2" else 1
+    mk = lambda: H.jittered_box((10, 12, 7), H.numerics("ausmpred", mode), bcs=b)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
+def test_norms_match_residual_blocks(ctxmod, orc):
+    """GPU L2/Linf against the oracle's per-block sums (cprdu3s1.for), relative 1e-12."""
+    mk = lambda: cases.tgv_zone((24, 20, 16), H.numerics("roe_min", "bdf"), perturb=0.02)
+    zo, zg, rdm, norms = run_pair(ctxmod, orc, mk, 1, stop=2)
+    nss = int(zo.param_int[P.NSSITER])
+    r = rdm.reshape(-1, 2, 5, nss)      # (block, sum/max, eq, it) in Fortran order reversed
+    s2 = r[:, 0, :, 0].sum(axis=0)
+    mx = r[:, 1, :, 0].max(axis=0)
+    n = 24 * 20 * 16
+    dtc = zo.param_real[P.DTC]
+    l2 = np.sqrt(s2 / n) / dtc
+    linf = mx / dtc
+    np.testing.assert_allclose(norms[:5], l2, rtol=1e-12, atol=0)
+    assert np.array_equal(norms[5:], linf)
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (2, 3, 2), (67, 5, 3)])
+def test_ragged_sizes(ctxmod, orc, n):
+    """Tiny and odd sizes (two cells per direction, sizes not multiple of the 64x4 blocks)."""
+    mk = lambda: cases.tgv_zone(n, H.numerics("roe_min", "bdf"), perturb=0.02)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, 1, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
+def test_compute_loop_matches_oracle(ctxmod, orc):
+    """fm_compute (time-step loop with level rotation) against the oracle driven the same way."""
+    num = H.numerics("roe_min", "bdf")
+    zo = cases.tgv_zone((16, 12, 10), num, perturb=0.02)
+    zg = cases.tgv_zone((16, 12, 10), num, perturb=0.02)
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro); zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    ctx = ctxmod.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+    nsteps = 4
+    ctx.compute(0, 1, 2, nit_c=nsteps, modulo_verif=2, first_it=0)
+    ctx.sync()
+    for step in range(nsteps):
+        orc.substep(zo, 1, first_it=0 if step == 0 else 1)
+        H.rotate(zo)
+    ctx.pull(zid, zg)
+    ctx.close()
+    assert np.array_equal(zo.interior(zo.ro), zg.interior(zg.ro))
+    assert np.array_equal(zo.ro, zg.ro)

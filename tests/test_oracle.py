@@ -1,0 +1,113 @@
+"""Pinning the oracle (CPU, no GPU).
+
+1. Golden fixtures written from the reference FastS Fortran kernels (tests/golden/*.npz,
+   produced by tests/golden/gen_golden.py): the oracle must reproduce them bit for bit.
+2. When oracle/_ref/libfastref.so is present (the reference kernels compiled from their own
+   sources), a wider matrix of cases is compared live, bit for bit.
+"""
+import copy
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from fast_amd import params as P, cases
+from fast_amd.zone import HostZone
+from tests import helpers as H
+import oracle
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return oracle.Oracle()
+
+
+def zone_from_fixture(d):
+    z = HostZone.from_arrays(d["param_int"], d["param_real"], d["x"], d["y"], d["z"])
+    z.ro[:] = d["in_ro"]; z.ro_m1[:] = d["in_ro_m1"]; z.ro_p1[:] = d["in_ro_p1"]
+    return z
+
+
+def interior_equal(z, a, b, nvar=5):
+    return np.array_equal(z.interior(a, nvar), z.interior(b, nvar))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_fixture(orc, path):
+    d = np.load(path, allow_pickle=False)
+    z = zone_from_fixture(d)
+    orc.init_metric(z)
+    assert np.array_equal(z.metric, d["metric"]), "metric"
+    # first sub-iteration RHS
+    zr = copy.deepcopy(z)
+    rdm = np.zeros(zr.rdm_size())
+    explicit = int(zr.param_int[P.ITYPCP]) == 2
+    orc.substep(zr, 1, first_it=0, lssiter_verif=1, rdm=rdm, stop=1 if explicit else 2)
+    assert interior_equal(zr, zr.drodm, d["rhs_drodm"]), "RHS"
+    assert interior_equal(zr, zr.xmut, d["rhs_xmut"], 1), "viscosity"
+    assert interior_equal(zr, zr.coe, d["rhs_coe"], int(zr.param_int[P.NEQ_COE])), "coefficients"
+    if not explicit:
+        assert np.array_equal(rdm, d["rdm"]), "residual blocks"
+    # full time steps
+    nsub, nsteps = int(d["nsub"]), int(d["nsteps"])
+    for step in range(nsteps):
+        for it in range(1, nsub + 1):
+            orc.substep(z, it, first_it=0 if step == 0 else 1)
+        H.rotate(z)
+    for f in ("ro", "ro_m1", "ro_p1"):
+        assert np.array_equal(getattr(z, f), d["step_" + f]), f
+    assert interior_equal(z, z.drodm, d["step_drodm"]), "increment"
+
+
+requires_ref = pytest.mark.skipif(not oracle.Reference.available(),
+                                  reason="reference kernels not built (make -C oracle ref)")
+
+
+@requires_ref
+@pytest.mark.parametrize("scheme", ["roe_min", "roe_kap", "roe_nul", "ausmpred", "senseur"])
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "bdf_ss2", "local"])
+@pytest.mark.parametrize("mesh", ["cart", "full", "wall"])
+def test_oracle_vs_reference(orc, scheme, mode, mesh):
+    if mesh == "wall" and scheme.startswith("roe"):
+        pytest.skip("Roe with walls needs the wall-flux correction (correct_flux.for), not implemented")
+    ref = oracle.Reference()
+    nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((9, 8, 7), H.numerics(scheme, mode), perturb=0.02)
+    elif mesh == "full":
+        mk = lambda: H.jittered_box((8, 7, 6), H.numerics(scheme, mode))
+    else:
+        mk = lambda: H.jittered_box((8, 9, 5), H.numerics(scheme, mode), bcs=H.WALL_J)
+    zr, zo = mk(), mk()
+    p_ref, _ = ref.classify(zr)
+    assert np.array_equal(p_ref[:P.N_PARAM_INT], zr.param_int[:P.N_PARAM_INT])
+    ref.init_metric(zr); orc.init_metric(zo)
+    assert np.array_equal(zr.metric, zo.metric)
+    ref.apply_bc(zr, zr.ro); orc.apply_bc(zo, zo.ro)
+    assert np.array_equal(zr.ro, zo.ro)
+    for z in (zr, zo):
+        z.ro_m1[:] = z.ro; z.ro_p1[:] = z.ro
+    for step in range(2):
+        for it in range(1, nsub + 1):
+            ref.substep(zr, it, first_it=0 if step == 0 else 1)
+            orc.substep(zo, it, first_it=0 if step == 0 else 1)
+        H.rotate(zr); H.rotate(zo)
+    for f in ("ro", "ro_m1", "ro_p1"):
+        assert np.array_equal(getattr(zr, f), getattr(zo, f)), f
+    assert interior_equal(zr, zr.drodm, zo.drodm)
+    assert interior_equal(zr, zr.coe, zo.coe, int(zr.param_int[P.NEQ_COE]))
+
+
+@requires_ref
+def test_classification_vs_reference():
+    """Zone type and metric sizing (nature_geom_dom) for a Cartesian box, a jittered box and
+    the channel-with-bump mesh."""
+    ref = oracle.Reference()
+    for z in (cases.tgv_zone((7, 6, 5)), H.jittered_box((7, 6, 5), H.numerics("roe_min", "bdf")),
+              cases.channel_zone(12, 10, 4)):
+        p_ref, degen = ref.classify(z)
+        assert np.array_equal(p_ref[:P.N_PARAM_INT], z.param_int[:P.N_PARAM_INT])
+        assert not degen.any()
