@@ -1,0 +1,358 @@
+"""FastS solver API over CGNS/Python trees, running on the MI355X.
+
+Drop-in for the FastS compute path of ``Fast/FastS/FastS/PyTree.py``:
+
+* ``warmup(t, tc=None, ...) -> (t, tc, metrics)``       PyTree.py:341-545
+* ``_compute(t, metrics, nitrun, tc=None, ..., NIT=1)``  PyTree.py:42-236
+* ``allocate_metric(t)`` / ``_init_metric(t, metrics, hook)``  PyTree.py:267-335
+* ``display_temporal_criteria(t, metrics, nitrun, format=None, ...)``  PyTree.py:2031-2076
+* ``_applyBC(t, metrics, hook1, nstep, var)``           PyTree.py:1155
+* ``_computeEnstrophy(t, metrics)`` (kinetic-energy part)  PyTree.py:1878
+* ``_createConvergenceHistory(t, nrecs)``               PyTree.py:2152
+* numerics setters ``_setNum2Base`` / ``_setNum2Zones``  (FastC/PyTree.py:115-137)
+
+The tree holds the fields exactly as the reference's does (FlowSolution#Centers Density,
+VelocityX/Y/Z, Temperature and their _M1/_P1 time levels, ghost cells included); the device
+copy is authoritative during ``_compute`` and the tree arrays are refreshed in place after it.
+Like the reference's module-global ``FastC.HOOK``, one solver state lives per process.
+"""
+import sys
+
+import numpy as np
+
+from . import Internal
+from . import params as P
+from .zone import HostZone
+
+VARS = ["Density", "VelocityX", "VelocityY", "VelocityZ", "Temperature"]
+HOOK = {}
+_MODELS = {"NSLaminar": "lamin", "Euler": "euler"}
+_STATE_KEYS = ("gamma", "cv", "ro", "p", "u", "T", "mach", "reynolds", "prandtl", "mus", "ts",
+               "cs", "vx", "vy", "vz")
+
+
+# ---------------------------------------------------------------------------------------
+# numerics (FastC._setNum2Base / _setNum2Zones)
+def _set_define(node, num):
+    d = Internal.createUniqueChild(node, ".Solver#define", "UserDefinedData_t")
+    for k, v in num.items():
+        Internal.createUniqueChild(d, k, "DataArray_t", v)
+
+
+def _setNum2Base(t, numb):
+    for b in Internal.getBases(t):
+        _set_define(b, numb)
+
+
+def _setNum2Zones(t, numz):
+    for z in Internal.getZones(t):
+        _set_define(z, numz)
+
+
+def _get_define(node):
+    out = {}
+    d = Internal.getNodeFromName1(node, ".Solver#define")
+    if d is not None:
+        for c in d[2]:
+            out[c[0]] = Internal.getValue(c)
+    return out
+
+
+def _numerics(base, zone):
+    n = _get_define(base)
+    n.update(_get_define(zone))
+    n.setdefault("temporal_scheme", "implicit")
+    if n.get("scheme") in ("roe_kap", "roe_min", "roe_nul"):
+        n["slope"] = {"roe_kap": "o3", "roe_min": "minmod", "roe_nul": "o1"}[n["scheme"]]
+    keys = set(P.Numerics.DEFAULTS)
+    return P.Numerics(**{k: v for k, v in n.items() if k in keys})
+
+
+# ---------------------------------------------------------------------------------------
+# reference state (Cassiopee's C.addState is not available: stored as DataArray_t children)
+def _addState(t, state, model="NSLaminar"):
+    """ReferenceState and FlowEquationSet/GoverningEquations of every base (the nodes
+    FastC._buildOwnData reads, Fast/FastC/FastC/PyTree.py:1452-1490)."""
+    if model not in _MODELS:
+        raise NotImplementedError("governing equations %r are not supported" % model)
+    for b in Internal.getBases(t):
+        rs = Internal.createUniqueChild(b, "ReferenceState", "ReferenceState_t")
+        for k in _STATE_KEYS:
+            Internal.createUniqueChild(rs, k, "DataArray_t", float(state[k]))
+        fe = Internal.createUniqueChild(b, "FlowEquationSet", "FlowEquationSet_t")
+        Internal.createUniqueChild(fe, "GoverningEquations", "GoverningEquations_t", model)
+
+
+def _get_state(base, zone):
+    rs = Internal.getNodeFromName1(zone, "ReferenceState") or Internal.getNodeFromName1(base, "ReferenceState")
+    if rs is None:
+        raise ValueError("ReferenceState missing (use fast_amd.PyTree._addState)")
+    return {k: Internal.getValue(Internal.getNodeFromName1(rs, k)) for k in _STATE_KEYS}
+
+
+# ---------------------------------------------------------------------------------------
+# zones <-> tree
+def newZoneFromHost(hz, base, bcs=None):
+    """Add a zone built from a HostZone to a base: coordinates, centre fields (N level),
+    ZoneBC with PointRange windows in the ghost-inclusive vertex numbering."""
+    p = hz.param_int
+    nv = (int(p[P.NIJK_XYZ]), int(p[P.NIJK_XYZ + 1]), int(p[P.NIJK_XYZ + 2]))
+    nc = (int(p[P.NIJK]), int(p[P.NIJK + 1]), int(p[P.NIJK + 2]))
+    z = Internal.newZone(hz.name, nv[0], nv[1], nv[2], parent=base)
+    g = Internal.createChild(z, "GridCoordinates", "GridCoordinates_t")
+    for n, a in (("CoordinateX", hz.x), ("CoordinateY", hz.y), ("CoordinateZ", hz.z)):
+        Internal.createChild(g, n, "DataArray_t", a.reshape(nv, order="F"))
+    fs = Internal.createChild(z, "FlowSolution#Centers", "FlowSolution_t")
+    Internal.createChild(fs, "GridLocation", "GridLocation_t", "CellCenter")
+    nd = hz.nd
+    for i, v in enumerate(VARS):
+        Internal.createChild(fs, v, "DataArray_t", hz.ro[i * nd:(i + 1) * nd].reshape(nc, order="F").copy(order="F"))
+    zbc = Internal.createChild(z, "ZoneBC", "ZoneBC_t")
+    pt = int(p[P.PT_BC])
+    nb = int(p[pt]) if pt > 0 else 0
+    inv = {v: k for k, v in P.BC_TAGS.items()}
+    for n in range(nb):
+        q = int(p[pt + 1 + n])
+        typ, idir = int(p[q]), int(p[q + 1])
+        fen = [int(v) for v in p[q + 2:q + 8]]
+        rng = _fen_to_pointrange(p, idir, fen)
+        Internal.newBC("BC%d_%s" % (n, ["imin", "imax", "jmin", "jmax", "kmin", "kmax"][idir - 1]),
+                       inv[typ], rng, zbc)
+    return z
+
+
+def _fen_to_pointrange(p, idir, fen):
+    """Inverse of getRange (Fast/FastC/FastC/getRange.cpp:57-110) for full-face windows."""
+    nf, nfk = int(p[P.NIJK + 3]), int(p[P.NIJK + 4])
+    nv = [int(p[P.NIJK_XYZ]), int(p[P.NIJK_XYZ + 1]), int(p[P.NIJK_XYZ + 2])]
+    rng = []
+    for d in range(3):
+        g = nf if d < 2 else nfk
+        lo, hi = fen[2 * d] + g, fen[2 * d + 1] + g + 1
+        rng.append([lo, hi])
+    d = (idir - 1) // 2
+    v = 1 if idir % 2 == 1 else nv[d]
+    rng[d] = [v, v]
+    return rng
+
+
+def _zone_bcs(z, p):
+    bcs = []
+    zbc = Internal.getNodeFromName1(z, "ZoneBC")
+    if zbc is None:
+        return bcs
+    for bc in Internal.getNodesFromType1(zbc, "BC_t"):
+        btype = Internal.getValue(bc)
+        rng = Internal.getValue(Internal.getNodeFromType1(bc, "IndexRange_t"))
+        in_bc = [int(rng[0][0]), int(rng[0][1]), int(rng[1][0]), int(rng[1][1]), int(rng[2][0]), int(rng[2][1])]
+        idir, fen = P.get_range(in_bc, p)
+        bcs.append((btype, (idir, fen), None))
+    return bcs
+
+
+def _host_zone(base, z):
+    dims = z[1]
+    nv = [int(dims[0][0]), int(dims[1][0]), int(dims[2][0])]
+    ni, nj, nk = nv[0] - 5, nv[1] - 5, nv[2] - 5
+    num = _numerics(base, z)
+    st = _get_state(base, z)
+    g = Internal.getNodeFromName1(z, "GridCoordinates")
+    x, y, zz = (np.asarray(Internal.getNodeFromName1(g, n)[1]).ravel(order="F").astype(np.float64)
+                for n in ("CoordinateX", "CoordinateY", "CoordinateZ"))
+    p0 = P.build_param_int(ni, nj, nk, num)
+    bcs = _zone_bcs(z, p0)
+    fe = Internal.getNodeFromName1(z, "FlowEquationSet") or Internal.getNodeFromName1(base, "FlowEquationSet")
+    ge = Internal.getNodeFromName1(fe, "GoverningEquations") if fe is not None else None
+    model = Internal.getValue(ge) if ge is not None else "NSLaminar"
+    if model not in _MODELS:
+        raise NotImplementedError("governing equations %r are not supported" % model)
+    hz = HostZone(z[0], ni, nj, nk, num, st, x, y, zz, bcs=bcs, flow=_MODELS[model])
+    return hz
+
+
+def _fields(z, suffix=""):
+    fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+    return [Internal.getNodeFromName1(fs, v + suffix) for v in VARS]
+
+
+def _tree_to_level(z, hz, suffix, out):
+    nd = hz.nd
+    for i, n in enumerate(_fields(z, suffix)):
+        out[i * nd:(i + 1) * nd] = np.asarray(n[1]).ravel(order="F")
+
+
+def _level_to_tree(z, hz, suffix, a):
+    nd = hz.nd
+    for i, n in enumerate(_fields(z, suffix)):
+        np.copyto(n[1], a[i * nd:(i + 1) * nd].reshape(n[1].shape, order="F"))
+
+
+# ---------------------------------------------------------------------------------------
+def allocate_metric(t, verbose=0):
+    """Classify every zone, build its descriptor vectors, register it on the GPU and return
+    the list of metric arrays (ti|tj|tk|vol per zone)."""
+    from . import _lib
+    ctx = HOOK.get("ctx")
+    if ctx is None:
+        ctx = _lib.Context(0)
+        HOOK.clear()
+        HOOK.update(ctx=ctx, zones=[], ids=[])
+    metrics = []
+    for b in Internal.getBases(t):
+        for z in Internal.getNodesFromType1(b, "Zone_t"):
+            hz = _host_zone(b, z)
+            zid = ctx.add_zone(hz)
+            HOOK["zones"].append((z, hz))
+            HOOK["ids"].append(zid)
+            o = Internal.createUniqueChild(z, ".Solver#ownData", "UserDefinedData_t")
+            Internal.createUniqueChild(o, "Parameter_int", "DataArray_t", hz.param_int)
+            Internal.createUniqueChild(o, "Parameter_real", "DataArray_t", hz.param_real)
+            if verbose:
+                print("typezone: %s, %s %s" % (["3D curvilinear", "3D homogeneous k", "3D Cartesian", "2D"][hz.zone_type],
+                                              z[0], hz.dims))
+            metrics.append([hz.metric])
+    return metrics
+
+
+def _init_metric(t, metrics, hook=None):
+    """Face vectors and volumes (computed on the GPU when the zone was registered)."""
+    from . import _lib
+    ctx = HOOK["ctx"]
+    for (z, hz), zid, m in zip(HOOK["zones"], HOOK["ids"], metrics):
+        ctx.get(zid, _lib.FIELD_METRIC, m[0])
+
+
+def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list_graph=None,
+           Padding=None, verbose=0):
+    """Build the solver state: metrics, time levels _M1/_P1, ghost cells.  -> (t, tc, metrics)"""
+    from . import _lib
+    if tc is not None:
+        raise NotImplementedError("zone connectivity trees (tc) are not supported yet")
+    HOOK.clear()
+    metrics = allocate_metric(t, verbose)
+    _init_metric(t, metrics, HOOK)
+    ctx = HOOK["ctx"]
+    first_it = 1
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        _tree_to_level(z, hz, "", hz.ro)
+        has_m1 = Internal.getNodeFromName1(fs, "Density_M1") is not None
+        if has_m1:
+            _tree_to_level(z, hz, "_M1", hz.ro_m1)
+        else:
+            first_it = 0       # createPrimVars sets FIRST_IT = 0 when _M1 had to be created
+            hz.ro_m1[:] = hz.ro
+        ctx.set(zid, _lib.FIELD_RO, hz.ro)
+        ctx.apply_bc(zid, 0)
+        ctx.get(zid, _lib.FIELD_RO, hz.ro)
+        if not has_m1:
+            hz.ro_m1[:] = hz.ro
+        hz.ro_p1[:] = hz.ro
+        ctx.set(zid, _lib.FIELD_RO_M1, hz.ro_m1)
+        ctx.set(zid, _lib.FIELD_RO_P1, hz.ro_p1)
+        for sfx, a in (("", hz.ro), ("_M1", hz.ro_m1), ("_P1", hz.ro_p1)):
+            nd = hz.nd
+            nc = z[1][:, 1]
+            for i, v in enumerate(VARS):
+                arr = a[i * nd:(i + 1) * nd].reshape(tuple(int(c) for c in nc), order="F").copy(order="F")
+                Internal.createUniqueChild(fs, v + sfx, "DataArray_t", arr)
+    HOOK["first_it"] = first_it
+    HOOK["nitrun_done"] = 0
+    return t, tc, metrics
+
+
+def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, layer="c", NIT=1,
+             ucData=None, vtune=None):
+    """NIT time steps (all sub-iterations) of every zone on the GPU; tree fields updated in place."""
+    import time
+    from . import _lib
+    if tc is not None:
+        raise NotImplementedError("zone connectivity trees (tc) are not supported yet")
+    ctx = HOOK["ctx"]
+    base = Internal.getBases(t)[0]
+    bdef = _get_define(base)
+    modulo_verif = int(bdef.get("modulo_verif", 200))
+    hz0 = HOOK["zones"][0][1]
+    nssiter = int(hz0.param_int[P.NSSITER])
+    t0 = time.time()
+    ctx.compute(nitrun, 1, nssiter, nit_c=NIT, modulo_verif=modulo_verif, first_it=HOOK["first_it"])
+    ctx.sync()
+    t1 = time.time()
+    HOOK["first_it"] = 1
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        for sfx, f, a in (("", _lib.FIELD_RO, hz.ro), ("_M1", _lib.FIELD_RO_M1, hz.ro_m1),
+                          ("_P1", _lib.FIELD_RO_P1, hz.ro_p1)):
+            ctx.get(zid, f, a)
+            _level_to_tree(z, hz, sfx, a)
+    if vtune is not None:
+        return t1 - t0, 0.0
+    return None
+
+
+def _applyBC(t, metrics, hook1=None, nstep=1, var="Density"):
+    from . import _lib
+    ctx = HOOK["ctx"]
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        _tree_to_level(z, hz, "", hz.ro)
+        ctx.set(zid, _lib.FIELD_RO, hz.ro)
+        ctx.apply_bc(zid, 0, nstep)
+        ctx.get(zid, _lib.FIELD_RO, hz.ro)
+        _level_to_tree(z, hz, "", hz.ro)
+
+
+def display_temporal_criteria(t, metrics, nitrun, format=None, gmres=None, verbose="firstlast",
+                              stopAtNan=True):
+    """Residual L2 / Linf per zone and equation of the last verified sub-iteration
+    (display_ss_iteration.cpp:112-145).  format "store" records them in the tree's
+    ZoneConvergenceHistory; returns {"L2": ..., "Loo": ...} arrays [zone, equation]."""
+    ctx = HOOK["ctx"]
+    l2, loo = [], []
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        try:
+            r = ctx.residual_norms(zid)
+        except Exception:
+            r = np.full(10, np.nan)
+        l2.append(r[:5]); loo.append(r[5:])
+        if format == "store":
+            h = Internal.getNodeFromName1(z, "ZoneConvergenceHistory")
+            if h is not None:
+                it = Internal.getNodeFromName1(h, "IterationValues")[1]
+                rl2 = Internal.getNodeFromName1(h, "RSD_L2")[1]
+                rlo = Internal.getNodeFromName1(h, "RSD_oo")[1]
+                k = int(Internal.getNodeFromName1(h, "LastRecordedIteration")[1][0])
+                if k < it.size:
+                    it[k] = nitrun; rl2[k * 5:(k + 1) * 5] = r[:5]; rlo[k * 5:(k + 1) * 5] = r[5:]
+                    Internal.getNodeFromName1(h, "LastRecordedIteration")[1][0] = k + 1
+        elif format is not None or verbose != "none":
+            fmt = "%.16e" if format == "double" else "%.6e"
+            print("iteration %d zone %s: L2 %s Loo %s" % (nitrun, z[0], " ".join(fmt % v for v in r[:5]),
+                                                         " ".join(fmt % v for v in r[5:])))
+    if stopAtNan:
+        for z, hz in HOOK["zones"]:
+            if not np.isfinite(hz.ro).all():
+                sys.exit(1)
+    out = {"L2": np.array(l2), "Loo": np.array(loo)}
+    return out if gmres is not None else out
+
+
+displayTemporalCriteria = display_temporal_criteria
+
+
+def _createConvergenceHistory(t, nrecs):
+    for z in Internal.getZones(t):
+        h = Internal.createUniqueChild(z, "ZoneConvergenceHistory", "ConvergenceHistory_t")
+        Internal.createUniqueChild(h, "LastRecordedIteration", "DataArray_t", np.zeros(1, np.int32))
+        Internal.createUniqueChild(h, "IterationValues", "DataArray_t", np.zeros(nrecs, np.int32))
+        Internal.createUniqueChild(h, "RSD_L2", "DataArray_t", np.zeros(5 * nrecs))
+        Internal.createUniqueChild(h, "RSD_oo", "DataArray_t", np.zeros(5 * nrecs))
+
+
+def _computeEnstrophy(t, metrics):
+    """Returns (enstrophy, tke) summed over the zones; the enstrophy term is not computed
+    (None); tke = sum 1/2 rho |u|^2 vol / sum vol over interior cells."""
+    ctx = HOOK["ctx"]
+    k = v = 0.0
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        s = ctx.tke(zid, 0)
+        k += s[0]; v += s[1]
+    return None, k / v

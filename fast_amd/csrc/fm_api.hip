@@ -18,8 +18,9 @@ enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 
        ITYPCP = 29, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
        NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
        EXPLOC = 54, NSSITER = 58, SLOPE = 64, PT_BC = 70, NB_RELAX = 71, IMPLICITSOLVER = 74,
-       LU_MATCH = 75, SENSORTYPE = 85, BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2 };
+       LU_MATCH = 75, SENSORTYPE = 85, BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2, BC_NBDATA = 8 };
 enum { R_DTC = 0, R_GAMMA = 1, R_CVINF = 2, R_ROINF = 3, R_PINF = 4, R_VINF = 5, R_TINF = 6,
+       R_VXINF = 19, R_VYINF = 20, R_VZINF = 21,
        R_PRANDT = 10, R_XMUL0 = 11, R_TEMP0 = 12, R_CS = 13, R_CFL = 15, R_PSIROE = 34,
        R_WIG_AMP = 72 };
 
@@ -32,7 +33,13 @@ static int fail(int code, const char* fmt, ...) {
 }
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail(FM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
 
-struct BcRec { int type, idir, fen[6]; };
+struct BcRec {
+    int type, idir, fen[6];
+    int kind;              // kernel kind (BcArgs::kind) when applied to a state
+    double state[5];       // conservative reference state (BCzone.cpp DEFAULT_STATE)
+    double* pext;          // outpres: device copy of the per-face-point pressure
+    int inc_bc[3];         // outpres: indbci strides (INCREMENT_BC.h)
+};
 
 struct Zone {
     std::vector<int> p;
@@ -73,6 +80,7 @@ static void free_zone(Zone* z) {
                       z->k.coe, z->k.wig, z->k.ti, (double*)z->k.x, (double*)z->k.y,
                       (double*)z->k.z, z->norm_part, z->norm_out};
     for (double* b : bufs) if (b) (void)hipFree(b);
+    for (BcRec& b : z->bcs) if (b.pext) (void)hipFree(b.pext);
     delete z;
 }
 
@@ -147,16 +155,62 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     int nb = pt_bcs > 0 ? p[pt_bcs] : 0;
     for (int n = 0; n < nb; n++) {
         int pt = p[pt_bcs + 1 + n];
-        BcRec b; b.type = p[pt + BC_TYPE]; b.idir = p[pt + BC_IDIR];
+        BcRec b; b.type = p[pt + BC_TYPE]; b.idir = p[pt + BC_IDIR]; b.pext = nullptr;
         for (int q = 0; q < 6; q++) b.fen[q] = p[pt + BC_FEN + q];
-        if (!(b.type == 0 || b.type == 11 || ((b.type == 4 || b.type == 6) && p[IFLOW] > 1))) {
-            free_zone(z);
-            return fail(FM_E_UNSUPPORTED, "boundary condition type %d not implemented", b.type);
-        }
+        int t = b.type, nbdata = p[pt + BC_NBDATA];
+        const int* size_data = p + pt + BC_NBDATA + 1;
+        const double* data = nbdata ? pr + p[pt_bcs + 1 + n + nb] : nullptr;
         if (b.idir < 1 || b.idir > 6) { free_zone(z); return fail(FM_E_ARG, "BC direction %d", b.idir); }
-        if ((b.type == 4 || b.type == 6) && kf == 5) {
+        bool wall = t == 3 || t == 4 || t == 5 || t == 6 || t == 7 || t == 12;
+        if (wall && kf == 5) {
             free_zone(z);
             return fail(FM_E_UNSUPPORTED, "Roe flux with wall boundaries (wall-flux correction) not implemented");
+        }
+        if (t == 0) b.kind = 0;
+        else if (t == 11) b.kind = 2;
+        else if ((t == 4 || t == 6) && p[IFLOW] > 1) b.kind = 1;
+        else if (t == 3 || t == 5 || (t == 4 && p[IFLOW] == 1)) b.kind = 3;
+        else if (t == 6) b.kind = -1;    // viscous wall in an Euler zone: BCzone.cpp does nothing
+        else if (t == 1 || (t == 13 && nbdata == 0)) b.kind = 4;
+        else if (t == 10) b.kind = 5;
+        else if (t == 16 && (nbdata == 1 || nbdata == 2)) b.kind = 6;
+        else {
+            free_zone(z);
+            return fail(FM_E_UNSUPPORTED, "boundary condition type %d (with %d data arrays) not implemented", t, nbdata);
+        }
+        // reference state: free stream unless the BC carries its own data
+        if (nbdata == 0) {
+            b.state[0] = pr[R_ROINF]; b.state[1] = pr[R_VXINF] * pr[R_ROINF];
+            b.state[2] = pr[R_VYINF] * pr[R_ROINF]; b.state[3] = pr[R_VZINF] * pr[R_ROINF];
+            b.state[4] = pr[R_PINF] / (pr[R_GAMMA] - 1.) +
+                         0.5 * (pr[R_VXINF] * b.state[1] + pr[R_VYINF] * b.state[2] + pr[R_VZINF] * b.state[3]);
+        } else {
+            for (int q = 0; q < 5; q++) b.state[q] = 0.;
+        }
+        if (b.kind == 6) {
+            const int* fen = b.fen;
+            if (b.idir <= 2) { b.inc_bc[0] = fen[3] - fen[2] + 1; b.inc_bc[1] = fen[2]; b.inc_bc[2] = fen[4]; }
+            else if (b.idir <= 4) { b.inc_bc[0] = fen[1] - fen[0] + 1; b.inc_bc[1] = fen[0]; b.inc_bc[2] = fen[4]; }
+            else { b.inc_bc[0] = fen[1] - fen[0] + 1; b.inc_bc[1] = fen[0]; b.inc_bc[2] = fen[2]; }
+            // one exit pressure per face point (a leading 1-value array is the mesh-motion
+            // coefficient and is skipped, as BCzone.cpp does)
+            const double* pe = size_data[0] == 1 ? data + 1 : data;
+            long npe = size_data[0] == 1 ? size_data[1] : size_data[0];
+            long need = (long)(b.fen[1] - b.fen[0] + 1) * (b.fen[3] - b.fen[2] + 1) * (b.fen[5] - b.fen[4] + 1);
+            if (npe < need || (pe - pr) + npe > n_real) {
+                free_zone(z);
+                return fail(FM_E_ARG, "BCOutpres: %ld pressure values for %ld face points", npe, need);
+            }
+            double* dp = nullptr;
+            int rc2 = dmalloc(&dp, npe);
+            if (rc2 != FM_OK) { free_zone(z); return rc2; }
+            b.pext = dp;
+            z->bcs.push_back(b);
+            if (hipMemcpy(dp, pe, sizeof(double) * npe, hipMemcpyHostToDevice) != hipSuccess) {
+                free_zone(z);
+                return fail(FM_E_HIP, "BCOutpres data upload");
+            }
+            continue;
         }
         z->bcs.push_back(b);
     }
@@ -307,7 +361,13 @@ static void bczone(fm_ctx* c, Zone* z, int lrhs, double* rop, int shift_lu[6], i
         if (lrhs == 1 && (t == 0 || t == 1 || t == 2 || t == 10 || t == 11 || t == 14 || t == 16 ||
                           t == 17 || t == 20 || t == 21)) t = 0;
         BcArgs a; a.idir = b.idir; a.lrhs = lrhs;
-        a.kind = (t == 0) ? 0 : (t == 11) ? 2 : 1;
+        a.kind = t == 0 ? 0 : b.kind;
+        if (a.kind < 0) continue;
+        a.tijk = b.idir <= 2 ? k.ti : b.idir <= 4 ? k.tj : k.tk;
+        a.neq_mtr = b.idir <= 4 ? z->p[NEQ_IJ] : z->p[NEQ_K];
+        for (int q = 0; q < 5; q++) a.state[q] = b.state[q];
+        a.pext = b.pext;
+        for (int q = 0; q < 3; q++) a.inc_bc[q] = b.inc_bc[q];
         if (cl_window(z, b.idir, b.fen, a.w)) launch_bc_face(k, rop, a, c->stream);
     }
     if (lcorner_out) *lcorner_out = lcorner;
@@ -387,8 +447,9 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
             if (la.out == z->ro_scratch) {
-                // copy the updated cells back (whole LU box: ghost layers the range extends into)
-                HIPCHK(hipMemcpyAsync(ro_p1, z->ro_scratch, sizeof(double) * 5 * k.nd, hipMemcpyDeviceToDevice, st));
+                // copy the updated cells back (the LU box, ghost layers it extends into included)
+                Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+                launch_copy_box(k, z->ro_scratch, ro_p1, lb, st);
             }
         }
     }

@@ -286,176 +286,28 @@ __global__ void __launch_bounds__(256) k_residual(KZ z, RhsArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
-// LU-SGS on i+j+k hyperplanes (invlu_l.for / invlu_u.for).  Cells of one hyperplane are
-// independent; per cell the neighbour terms are added in the reference order.
-struct LUC { double gam1, gamm1, cp, gam2; };
-
-template <bool CART, int D, bool LOWER>
-__device__ __forceinline__ void lu_term(const KZ& z, const LUC& c, const double* __restrict__ rop,
-                                        const double* __restrict__ dw, long l1, long lt,
-                                        double signe, double b[5]) {
-    long nd = z.nd;
-    double u = rop[l1 + nd], v = rop[l1 + 2 * nd], w = rop[l1 + 3 * nd], t = rop[l1 + 4 * nd];
-    double q2 = .5 * (u * u + v * v + w * w);
-    double h = c.cp * t + q2;
-    double ph2 = c.gamm1 * q2;
-    double cs = z.coe[l1 + (1 + D) * nd] * signe;
-    double gam1 = c.gam1, gamm1 = c.gamm1, gam2 = c.gam2;
-    double d1 = dw[l1], d2 = dw[l1 + nd], d3 = dw[l1 + 2 * nd], d4 = dw[l1 + 3 * nd], d5 = dw[l1 + 4 * nd];
-    if (CART) {
-        double tc = D == 0 ? z.ti[0] : D == 1 ? z.tj[0] : z.tk[0];
-        if (D == 0) {
-            double qn = tc * u;
-            double b21 = (tc * ph2 - u * qn), b22 = (qn - tc * gam2 * u) + cs, b23 = (-tc * gamm1 * v);
-            double b24 = (-tc * gamm1 * w), b25 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = (tc * h - gamm1 * u * qn), b53 = (-gamm1 * v * qn);
-            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-            b[0] = cs * d1 + tc * d2;
-            b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
-            b[2] = (-v * qn) * d1 + (tc * v) * d2 + ((qn) + cs) * d3;
-            b[3] = (-w * qn) * d1 + (tc * w) * d2 + ((qn) + cs) * d4;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
-        } else if (D == 1) {
-            double qn = tc * v;
-            double b31 = (tc * ph2 - v * qn), b32 = (-tc * gamm1 * u), b33 = (qn - tc * gam2 * v) + cs;
-            double b34 = (-tc * gamm1 * w), b35 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = (-gamm1 * u * qn), b53 = (tc * h - gamm1 * v * qn);
-            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-            b[0] = cs * d1 + tc * d3;
-            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d3;
-            b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
-            b[3] = (-w * qn) * d1 + (tc * w) * d3 + ((qn) + cs) * d4;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
-        } else {
-            double qn = tc * w;
-            double b41 = tc * ph2 - w * qn, b42 = -tc * gamm1 * u, b43 = -tc * gamm1 * v;
-            double b44 = (qn - tc * gam2 * w) + cs, b45 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = -gamm1 * u * qn, b53 = -gamm1 * v * qn;
-            double b54 = tc * h - gamm1 * w * qn, b55 = gam1 * qn + cs;
-            b[0] = cs * d1 + tc * d4;
-            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d4;
-            b[2] = (-v * qn) * d1 + (qn + cs) * d3 + (tc * v) * d4;
-            b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
-        }
-        return;
-    }
-    const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
-    long inc = D == 0 ? z.mi : D == 1 ? z.mj : z.mk;
-    long ndm = z.ndm, lt1, lt2;
-    if (LOWER) { lt1 = lt - inc; lt2 = lt; }
-    else { lt1 = lt + inc; lt2 = lt + 2 * inc; }
-    double tcx = 0.5 * (tm[lt2] + tm[lt1]);
-    double tcy = 0.5 * (tm[lt2 + ndm] + tm[lt1 + ndm]);
-    double tcz = 0.5 * (tm[lt2 + 2 * ndm] + tm[lt1 + 2 * ndm]);
-    double qn = tcx * u + tcy * v + tcz * w;
-    double b21 = (tcx * ph2 - u * qn), b22 = (qn - tcx * gam2 * u) + cs;
-    double b23 = (tcy * u - tcx * gamm1 * v), b24 = (tcz * u - tcx * gamm1 * w), b25 = tcx * gamm1;
-    double b31 = (tcy * ph2 - v * qn), b32 = (tcx * v - tcy * gamm1 * u);
-    double b33 = (qn - tcy * gam2 * v) + cs, b34 = (tcz * v - tcy * gamm1 * w), b35 = tcy * gamm1;
-    double b41 = (tcz * ph2 - w * qn), b42 = (tcx * w - tcz * gamm1 * u);
-    double b43 = (tcy * w - tcz * gamm1 * v), b44 = (qn - tcz * gam2 * w) + cs, b45 = tcz * gamm1;
-    double b51 = qn * (ph2 - h), b52 = (tcx * h - gamm1 * u * qn), b53 = (tcy * h - gamm1 * v * qn);
-    double b54 = (tcz * h - gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-    b[0] = cs * d1 + tcx * d2 + tcy * d3 + tcz * d4;
-    b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
-    b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
-    b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
-    b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
-}
-
-template <bool CART>
-__global__ void k_lu_lower_plane(KZ z, LuArgs a, int s) {
-    int jj = blockIdx.x * blockDim.x + threadIdx.x;
-    int kk = blockIdx.y;
-    int nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, ni = a.i1 - a.i0 + 1;
-    if (jj >= nj || kk >= nk) return;
-    int ii = s - jj - kk;
-    if (ii < 0 || ii >= ni) return;
-    int i = a.i0 + ii, j = a.j0 + jj, k = a.k0 + kk;
-    long nd = z.nd, l = IDX(z, i, j, k);
-    long lt = CART ? 0 : MTR(z, i, j, k);
-    LUC c; c.gam1 = z.gam; c.gamm1 = c.gam1 - 1.; c.cp = z.gam * z.cv; c.gam2 = c.gamm1 - 1.;
-    double* dw = a.dw;
-    const double* rop = a.rop;
-    double xal = z.coe[l] * 0.5;
-    double x[5], b[5];
-#pragma unroll
-    for (int e = 0; e < 5; e++) x[e] = dw[l + e * nd];
-    if (CART) {
-        if (jj > 0) { lu_term<CART, 1, true>(z, c, rop, dw, l - z.incj, lt, 0.5, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-        if (kk > 0) { lu_term<CART, 2, true>(z, c, rop, dw, l - z.inck, lt, 0.5, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    } else {
-        if (kk > 0) { lu_term<CART, 2, true>(z, c, rop, dw, l - z.inck, lt, 0.5, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-        if (jj > 0) { lu_term<CART, 1, true>(z, c, rop, dw, l - z.incj, lt, 0.5, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    }
-    if (ii > 0) { lu_term<CART, 0, true>(z, c, rop, dw, l - 1, lt, 0.5, b);
-#pragma unroll
-        for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    double diag = 1. / z.coe[l + 4 * nd];
-#pragma unroll
-    for (int e = 0; e < 5; e++) dw[l + e * nd] = x[e] * diag;
-}
-
-// Upper sweep + Newton update (mjr_newton.for).  J and I terms and the Newton base state use
-// rop; the K term uses rop_k (the already updated state of plane k+1 when the sweep works in
-// place, as the reference does for sub-iterations >= 2); the new state goes to out.
-template <bool CART>
-__global__ void k_lu_upper_plane(KZ z, LuArgs a, int s) {
-    int jj = blockIdx.x * blockDim.x + threadIdx.x;
-    int kk = blockIdx.y;
-    int nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, ni = a.i1 - a.i0 + 1;
-    if (jj >= nj || kk >= nk) return;
-    int ii = s - jj - kk;
-    if (ii < 0 || ii >= ni) return;
-    // reversed coordinates: plane s counts from the top corner
-    int i = a.i1 - ii, j = a.j1 - jj, k = a.k1 - kk;
-    long nd = z.nd, l = IDX(z, i, j, k);
-    long lt = CART ? 0 : MTR(z, i, j, k);
-    LUC c; c.gam1 = z.gam; c.gamm1 = c.gam1 - 1.; c.cp = z.gam * z.cv; c.gam2 = c.gamm1 - 1.;
-    double* dw = a.dw;
-    double xal = z.coe[l] * -0.5;
-    double dg = z.coe[l + 4 * nd];
-    double x[5], b[5];
-#pragma unroll
-    for (int e = 0; e < 5; e++) x[e] = dw[l + e * nd] * dg;
-    if (kk > 0) { lu_term<CART, 2, false>(z, c, a.rop_k, dw, l + z.inck, lt, -0.5, b);
-#pragma unroll
-        for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    if (jj > 0) { lu_term<CART, 1, false>(z, c, a.rop, dw, l + z.incj, lt, -0.5, b);
-#pragma unroll
-        for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    if (ii > 0) { lu_term<CART, 0, false>(z, c, a.rop, dw, l + 1, lt, -0.5, b);
-#pragma unroll
-        for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal; }
-    double diag = 1. / z.coe[l + 4 * nd];
-#pragma unroll
-    for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; }
-    if (a.newton) {
-        const double* R = a.rop;
-        double* O = a.out;
-        double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
-        double ro_old = R[l], u_old = R[l + nd], v_old = R[l + 2 * nd], w_old = R[l + 3 * nd], t_old = R[l + 4 * nd];
-        double r1 = R[l] + x[0];
-        double r_1 = 1. / r1;
-        double u1 = (ro_old * u_old + x[1]) * r_1, v1 = (ro_old * v_old + x[2]) * r_1, w1 = (ro_old * w_old + x[3]) * r_1;
-        double roe_old = ro_old * (z.cv * t_old + 0.5 * (u_old * u_old + v_old * v_old + w_old * w_old));
-        O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1;
-        O[l + 4 * nd] = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
-    }
-}
-
-// ---------------------------------------------------------------------------------------
 // Ghost cells.  Face BC over a window (BCExtrapolate.for, BCPeriodic.for,
 // BCWallViscous_{i,j,k}.for), then correct_coins.for edges and corners.
+// shape_tab_mtr.for: which metric components a face BC reads
+__device__ __forceinline__ void shape_tab(int neq_mtr, int idir, int& ic, int& jc, int& kc,
+                                          double& cim, double& cjm, double& ckm) {
+    if (neq_mtr == 3) { ic = 1; jc = 2; kc = 3; cim = 1.; cjm = 1.; ckm = 1.; }
+    else if (neq_mtr == 2) { ic = 1; jc = 2; kc = 2; cim = 1.; cjm = 1.; ckm = 0.; }
+    else if (neq_mtr == 1) {
+        ic = 1; jc = 1; kc = 1;
+        cim = idir <= 2 ? 1. : 0.; cjm = (idir == 3 || idir == 4) ? 1. : 0.; ckm = idir >= 5 ? 1. : 0.;
+    } else { ic = 1; jc = 1; kc = 1; cim = 0.; cjm = 0.; ckm = 0.; }
+}
+
+// metric index of the boundary face of the ghost column through (c0, c1, c2)
+__device__ __forceinline__ long bc_face_mtr(const KZ& z, const BcArgs& a, int c0, int c1, int c2) {
+    int d = (a.idir - 1) / 2;
+    bool up = (a.idir % 2) == 0;
+    int c[3] = {c0, c1, c2};
+    c[d] = up ? a.w[2 * d] : a.w[2 * d + 1] + 1;
+    return MTR(z, c[0], c[1], c[2]);
+}
+
 __global__ void k_bc_face(KZ z, double* __restrict__ rop, BcArgs a) {
     int n0 = a.w[1] - a.w[0] + 1, n1 = a.w[3] - a.w[2] + 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -468,8 +320,8 @@ __global__ void k_bc_face(KZ z, double* __restrict__ rop, BcArgs a) {
     int c[3] = {i, j, k};
     int n = d == 0 ? z.ni : d == 1 ? z.nj : z.nk;
     if (a.kind == 0) c[d] = up ? a.w[2 * d] - 1 : a.w[2 * d + 1] + 1;
-    else if (a.kind == 1) c[d] = up ? 2 * a.w[2 * d] - 1 - c[d] : 2 * a.w[2 * d + 1] + 1 - c[d];
-    else c[d] = up ? c[d] - n : c[d] + n;
+    else if (a.kind == 2) c[d] = up ? c[d] - n : c[d] + n;
+    else c[d] = up ? 2 * a.w[2 * d] - 1 - c[d] : 2 * a.w[2 * d + 1] + 1 - c[d];
     long l1 = IDX(z, c[0], c[1], c[2]);
     double c1 = a.lrhs ? 0. : 1.;
     if (a.kind == 0) {
@@ -478,11 +330,176 @@ __global__ void k_bc_face(KZ z, double* __restrict__ rop, BcArgs a) {
     } else if (a.kind == 2) {
 #pragma unroll
         for (int e = 0; e < 5; e++) rop[l + e * nd] = rop[l1 + e * nd] * c1;
-    } else {
+    } else if (a.kind == 1) {
         double u = rop[l1 + nd], v = rop[l1 + 2 * nd], w = rop[l1 + 3 * nd];
         rop[l] = rop[l1];
         rop[l + nd] = 0. - u; rop[l + 2 * nd] = 0. - v; rop[l + 3 * nd] = 0. - w;
         rop[l + 4 * nd] = rop[l1 + 4 * nd];
+    } else {
+        // BCWallInviscid.for: mirror state with the normal velocity reversed (no mesh motion,
+        // so the entrainment terms are +0)
+        int ic, jc, kc; double cim, cjm, ckm;
+        shape_tab(a.neq_mtr, a.idir, ic, jc, kc, cim, cjm, ckm);
+        long lm = bc_face_mtr(z, a, i, j, k), ndm = z.ndm;
+        double tcx = a.tijk[lm + (ic - 1) * ndm] * cim, tcy = a.tijk[lm + (jc - 1) * ndm] * cjm;
+        double tcz = a.tijk[lm + (kc - 1) * ndm] * ckm;
+        double surf = dmax(sqrt(tcx * tcx + tcy * tcy + tcz * tcz), 1e-30);
+        double s_1 = 1. / surf;
+        tcx = tcx * s_1; tcy = tcy * s_1; tcz = tcz * s_1;
+        double u = rop[l1 + nd], v = rop[l1 + 2 * nd], w = rop[l1 + 3 * nd];
+        double qn = (u * tcx + v * tcy + w * tcz);
+        double ut = u - qn * tcx, vt = v - qn * tcy, wt = w - qn * tcz;
+        double ua = 2. * ut - u, va = 2. * vt - v, wa = 2. * wt - w;
+        rop[l] = rop[l1];
+        rop[l + nd] = 0. + ua; rop[l + 2 * nd] = 0. + va; rop[l + 3 * nd] = 0. + wa;
+        rop[l + 4 * nd] = rop[l1 + 4 * nd];
+    }
+}
+
+// Characteristic ghost state (FastS BC/non_reflection.for): exterior state (roe..ete),
+// interior cell l1, boundary normal (tcx, tcy, tcz) pointing out of the zone.
+struct NonRefl { double svar1, svar2, svar3, svar4, qvar5, r, u, v, w, qn, sn; };
+
+__device__ __forceinline__ NonRefl non_reflection(const double* __restrict__ rop, long nd, long l1,
+                                                  double roe, double rue, double rve, double rwe,
+                                                  double ete, double tcx, double tcy, double tcz,
+                                                  double gam, double gam1, double gam1_1, double cvinf) {
+    NonRefl o;
+    double sn = sqrt(tcx * tcx + tcy * tcy + tcz * tcz);
+    double roi = rop[l1];
+    double rui = rop[l1 + nd] * roi, rvi = rop[l1 + 2 * nd] * roi, rwi = rop[l1 + 3 * nd] * roi;
+    double eti = rop[l1 + 4 * nd] * cvinf * roi + .5 * (rui * rui + rvi * rvi + rwi * rwi) / roi;
+    double qref1 = .5 * (roe + roi), qref2 = .5 * (rue + rui), qref3 = .5 * (rve + rvi);
+    double qref4 = .5 * (rwe + rwi), qref5 = .5 * (ete + eti);
+    double r = qref1, ri = 1. / r;
+    double u = qref2 * ri, v = qref3 * ri, w = qref4 * ri;
+    double p = gam1 * (qref5 - .5 * r * (u * u + v * v + w * w));
+    double c = sqrt(gam * p * ri), ci = 1. / c, s_1 = 1. / sn;
+    double qn = tcx * u + tcy * v + tcz * w - 0.;
+    double tnx, tny;
+    if (sn == 0.) { tnx = 1.; tny = 0.; } else { tnx = tcx / sn; tny = tcy / sn; }
+    double eig4 = qn + sn * c, eig5 = qn - sn * c;
+    double q[2][5] = {{roe, rue, rve, rwe, ete}, {roi, rui, rvi, rwi, eti}};
+    double ch[2][5];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        double q1 = q[s][0], q2 = q[s][1], q3 = q[s][2], q4 = q[s][3], q5 = q[s][4];
+        double s1 = q1, s2 = (q2 - u * q1) * ri, s3 = (q3 - v * q1) * ri, s4 = (q4 - w * q1) * ri;
+        double s5 = gam1 * (.5 * (u * u + v * v + w * w) * q1 - u * q2 - v * q3 - w * q4 + q5);
+        ch[s][0] = s1 - ci * ci * s5;
+        ch[s][1] = tcz * s_1 * (tnx * s2 + tny * s3) - s4;
+        ch[s][2] = -tny * s2 + tnx * s3;
+        ch[s][3] = .5 * s_1 * (tcx * s2 + tcy * s3 + tcz * s4) + .5 * ri * ci * s5;
+        ch[s][4] = -.5 * s_1 * (tcx * s2 + tcy * s3 + tcz * s4) + .5 * ri * ci * s5;
+    }
+    // characteristic variables from outside where the eigenvalue is negative
+    double rv1 = qn < 0. ? ch[0][0] : ch[1][0], rv2 = qn < 0. ? ch[0][1] : ch[1][1];
+    double rv3 = qn < 0. ? ch[0][2] : ch[1][2];
+    double rv4 = eig4 < 0. ? ch[0][3] : ch[1][3], rv5 = eig5 < 0. ? ch[0][4] : ch[1][4];
+    o.svar1 = rv1 + r * ci * (rv4 + rv5);
+    o.svar2 = tcz * s_1 * tnx * rv2 - tny * rv3 + tcx * s_1 * (rv4 - rv5);
+    o.svar3 = tcz * s_1 * tny * rv2 + tnx * rv3 + tcy * s_1 * (rv4 - rv5);
+    o.svar4 = -rv2 + tcz * s_1 * (rv4 - rv5);
+    double svar5 = r * c * (rv4 + rv5);
+    o.qvar5 = .5 * (u * u + v * v + w * w) * o.svar1 + r * (u * o.svar2 + v * o.svar3 + w * o.svar4)
+              + gam1_1 * svar5;
+    o.r = r; o.u = u; o.v = v; o.w = w; o.qn = qn; o.sn = sn;
+    return o;
+}
+
+// Face BCs computed one ghost column at a time: the first rank from the adjacent interior
+// cell (bvbs_farfield / bvbs_inflow: BCFarfield.for; bvbs_outflow: BCOutflow.for;
+// bvbs_outpres: BCOutpres + linearization_state.for), then the outer ranks in the reference
+// loop order (BC_nextrank.for for farfield / inflow, BC_extrap.for otherwise).
+__global__ void k_bc_column(KZ z, double* __restrict__ rop, BcArgs a) {
+    const int d = (a.idir - 1) / 2;
+    const bool up = (a.idir % 2) == 0;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int na = a.w[2 * ta + 1] - a.w[2 * ta] + 1, nb = a.w[2 * tb + 1] - a.w[2 * tb] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int ca = a.w[2 * ta] + t % na, cb = a.w[2 * tb] + t / na;
+    const long nd = z.nd, ndm = z.ndm;
+    int ic, jc, kc; double cim, cjm, ckm;
+    shape_tab(a.neq_mtr, a.idir, ic, jc, kc, cim, cjm, ckm);
+    const double snorm = up ? 1. : -1.;
+    const double gam = z.gam, gam1 = z.gam - 1., gam1_1 = 1. / gam1;
+    const double rgp = z.cv * gam1, cvinv = 1. / z.cv;
+    const double c4 = 5. / 6., c5 = 2. / 6., c6 = -1. / 6.;
+    const double c0 = 1. / c6, c1 = -(c4 + c5) * c0, c2 = -c6 * c0, c3 = (2. - c5 - c4) * c0;
+    const long inc = d == 0 ? 1 : d == 1 ? z.incj : z.inck;
+    const long sinc = up ? -inc : inc;
+    const int g0 = up ? a.w[2 * d] : a.w[2 * d + 1];
+    int c[3];
+    c[ta] = ca; c[tb] = cb; c[d] = g0;
+    const long lm = bc_face_mtr(z, a, c[0], c[1], c[2]);
+    const double tcx = a.tijk[lm + (ic - 1) * ndm] * cim * snorm;
+    const double tcy = a.tijk[lm + (jc - 1) * ndm] * cjm * snorm;
+    const double tcz = a.tijk[lm + (kc - 1) * ndm] * ckm * snorm;
+    const long l = IDX(z, c[0], c[1], c[2]);
+    const long l1 = l + sinc;
+    if (a.kind == 4) {
+        NonRefl o = non_reflection(rop, nd, l1, a.state[0], a.state[1], a.state[2], a.state[3],
+                                   a.state[4], tcx, tcy, tcz, gam, gam1, gam1_1, z.cv);
+        double roinv = 1. / o.svar1;
+        double u = o.u + o.r * o.svar2 * roinv, v = o.v + o.r * o.svar3 * roinv, w = o.w + o.r * o.svar4 * roinv;
+        rop[l] = o.svar1; rop[l + nd] = u; rop[l + 2 * nd] = v; rop[l + 3 * nd] = w;
+        rop[l + 4 * nd] = (o.qvar5 * roinv - .5 * (u * u + v * v + w * w)) * cvinv;
+    } else if (a.kind == 5) {
+        double roe = rop[l1], rue = rop[l1 + nd] * roe, rve = rop[l1 + 2 * nd] * roe, rwe = rop[l1 + 3 * nd] * roe;
+        double roe_inv = 1. / a.state[0];
+        double ete = a.state[4] - 0.5 * (a.state[1] * a.state[1] + a.state[2] * a.state[2] + a.state[3] * a.state[3]) * roe_inv
+                     + 0.5 * (rue * rop[l1 + nd] + rve * rop[l1 + 2 * nd] + rwe * rop[l1 + 3 * nd]);
+        NonRefl o = non_reflection(rop, nd, l1, roe, rue, rve, rwe, ete, tcx, tcy, tcz, gam, gam1, gam1_1, z.cv);
+        rop[l] = rop[l1];
+        double roinv = 1. / rop[l];
+        double u2, u3, u4;
+        if (o.qn < 0) {
+            u2 = rop[l1 + nd] - o.qn / (o.sn * o.sn) * tcx;
+            u3 = rop[l1 + 2 * nd] - o.qn / (o.sn * o.sn) * tcy;
+            u4 = rop[l1 + 3 * nd] - o.qn / (o.sn * o.sn) * tcz;
+        } else {
+            u2 = rop[l1 + nd]; u3 = rop[l1 + 2 * nd]; u4 = rop[l1 + 3 * nd];
+        }
+        rop[l + nd] = u2; rop[l + 2 * nd] = u3; rop[l + 3 * nd] = u4;
+        rop[l + 4 * nd] = (o.qvar5 * roinv - .5 * (u2 * u2 + u3 * u3 + u4 * u4)) * cvinv;
+    } else {
+        long li = (long)(ca - a.inc_bc[1]) + (long)(cb - a.inc_bc[2]) * a.inc_bc[0];
+        double sn = sqrt(tcx * tcx + tcy * tcy + tcz * tcz);
+        double s_1 = 1. / sn, tnx, tny, tnz;
+        if (sn == 0.) { tnx = 1.; tny = 0.; tnz = 0.; }
+        else { tnx = tcx * s_1; tny = tcy * s_1; tnz = tcz * s_1; }
+        double roi = rop[l1], ui = rop[l1 + nd], vi = rop[l1 + 2 * nd], wi = rop[l1 + 3 * nd];
+        double Ti = rop[l1 + 4 * nd], pi = roi * rgp * Ti;
+        double ro0 = roi, p0 = ro0 * rgp * Ti;
+        double roc0 = sqrt(ro0 * gam * p0);
+        double pe = a.pext[li];
+        double dqn = (pe - pi) / roc0;
+        double rog = roi + ro0 * ro0 * dqn / roc0;
+        rop[l] = rog;
+        rop[l + nd] = ui - dqn * tnx; rop[l + 2 * nd] = vi - dqn * tny; rop[l + 3 * nd] = wi - dqn * tnz;
+        rop[l + 4 * nd] = pe / (rog * rgp);
+    }
+    // outer ranks in the reference loop order (ascending index)
+    const int glo = up ? g0 + 1 : a.w[2 * d], ghi = up ? a.w[2 * d + 1] : g0 - 1;
+    for (int g = glo; g <= ghi; g++) {
+        c[d] = g;
+        const long lg = IDX(z, c[0], c[1], c[2]);
+        const long q0 = lg + sinc, q1 = lg + 2 * sinc, q2 = lg + 3 * sinc;
+        if (a.kind == 4) {
+            rop[lg] = rop[q0] * c3 + rop[q1] * c1 + rop[q2] * c2;
+            if (fabs(rop[lg] / rop[q0] - 1.) >= 0.7) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) rop[lg + e * nd] = rop[q0 + e * nd];
+            } else {
+#pragma unroll
+                for (int e = 1; e < 5; e++)
+                    rop[lg + e * nd] = rop[q0 + e * nd] * c3 + rop[q1 + e * nd] * c1 + rop[q2 + e * nd] * c2;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 5; e++) rop[lg + e * nd] = rop[l + e * nd];
+        }
     }
 }
 
@@ -676,23 +693,15 @@ int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st) {
     return 0;
 }
 
-void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
-    int nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, ni = a.i1 - a.i0 + 1;
-    int nplanes = ni + nj + nk - 2;
-    dim3 g((nj + 63) / 64, nk), b(64);
-    for (int s = 0; s < nplanes; s++) {
-        if (z.typ == 2) hipLaunchKernelGGL(k_lu_lower_plane<true>, g, b, 0, st, z, a, s);
-        else hipLaunchKernelGGL(k_lu_lower_plane<false>, g, b, 0, st, z, a, s);
-    }
-    for (int s = 0; s < nplanes; s++) {
-        if (z.typ == 2) hipLaunchKernelGGL(k_lu_upper_plane<true>, g, b, 0, st, z, a, s);
-        else hipLaunchKernelGGL(k_lu_upper_plane<false>, g, b, 0, st, z, a, s);
-    }
-}
-
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st) {
     int n0 = a.w[1] - a.w[0] + 1, n1 = a.w[3] - a.w[2] + 1, n2 = a.w[5] - a.w[4] + 1;
     if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+    if (a.kind >= 4) {
+        int d = (a.idir - 1) / 2;
+        int ncol = (d == 0 ? n1 * n2 : d == 1 ? n0 * n2 : n0 * n1);
+        hipLaunchKernelGGL(k_bc_column, dim3((ncol + 127) / 128), dim3(128), 0, st, z, rop, a);
+        return;
+    }
     dim3 g((n0 * n1 + 255) / 256, n2);
     hipLaunchKernelGGL(k_bc_face, g, dim3(256), 0, st, z, rop, a);
 }

@@ -31,8 +31,15 @@ struct LuArgs {
 struct BcArgs {
     int w[6];     // ghost window (1-based cell indices)
     int idir;     // 1..6
-    int kind;     // 0 extrapolate, 1 viscous wall mirror, 2 periodic
+    int kind;     // per ghost cell: 0 extrapolate, 1 viscous wall mirror, 2 periodic,
+                  // 3 inviscid wall; per ghost column: 4 farfield / inflow, 5 outflow, 6 outpres
     int lrhs;     // 1: applied to the RHS (zeroes extrapolations)
+    // face BCs that read the boundary metric and a reference state
+    const double* tijk;   // ti, tj or tk for the face direction
+    int neq_mtr;          // NEQ_IJ or NEQ_K
+    double state[5];      // conservative reference state (BCzone.cpp DEFAULT_STATE)
+    const double* pext;   // outpres: one pressure per face point
+    int inc_bc[3];        // outpres: indbci strides of the BC window
 };
 
 void launch_metric(const KZ& z, hipStream_t st);
@@ -40,6 +47,7 @@ void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, i
 void launch_wiggle(const KZ& z, const double* rop, hipStream_t st);
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st);
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
+void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st);
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);
 void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st);
 void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double* out, hipStream_t st);

@@ -1,0 +1,346 @@
+// LU-SGS sweeps (invlu_l.for / invlu_u.for) and the Newton update (mjr_newton.for).
+//
+// The lower sweep solves (D + L) dW* = R cell by cell in increasing i, j, k; every cell needs
+// the final dW* of its three lower neighbours.  The upper sweep is the mirror image.  Results
+// must be bit-identical to the reference's lexicographic loops, so every cell evaluates the
+// same expressions in the same order; only the schedule differs:
+//
+//  * the LU box is cut into blocks of CI x 8 x 8 cells (i x j x k).  Blocks with the same
+//    bi + bj + bk are independent: one launch per block diagonal, in sweep order;
+//  * a block is first staged into LDS with coalesced loads (the latency of global memory is
+//    paid once per block, not once per cell), together with the products of its halo cells,
+//    which earlier launches finished;
+//  * one wave then sweeps the block: lane (jj, kk) owns the i-line segment of (j, k) and at
+//    step s handles cell ii = s - jj - kk (a skewed wavefront).  The lane that needs cell
+//    (i, j-1, k) is the one just "below" in j, which produced it one step earlier: the
+//    neighbour products B(W) dW of a finished cell travel through LDS (j, k) or stay in
+//    registers (i).
+//
+// The off-diagonal products b = B(W_nb) dW_nb are formed by the cell that owns W_nb, exactly
+// as the reference forms them for the cell that consumes them (same operands, same order).
+#include "fm_kernels.hpp"
+
+namespace fm {
+
+struct LUC { double gam1, gamm1, cp, gam2; };
+
+__device__ __forceinline__ LUC lu_const(const KZ& z) {
+    LUC c;
+    c.gam1 = z.gam; c.gamm1 = c.gam1 - 1.; c.cp = z.gam * z.cv; c.gam2 = c.gamm1 - 1.;
+    return c;
+}
+
+// b = B_D(u, v, w, t; cs, face normal) d   (invlu_l.for / invlu_u.for neighbour term)
+template <bool CART, int D>
+__device__ __forceinline__ void lu_prod(const LUC& c, double u, double v, double w, double t,
+                                        double cs, double tcx, double tcy, double tcz,
+                                        const double d[5], double b[5]) {
+    double q2 = .5 * (u * u + v * v + w * w);
+    double h = c.cp * t + q2;
+    double ph2 = c.gamm1 * q2;
+    double gam1 = c.gam1, gamm1 = c.gamm1, gam2 = c.gam2;
+    double d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3], d5 = d[4];
+    if (CART) {
+        double tc = tcx;
+        if (D == 0) {
+            double qn = tc * u;
+            double b21 = (tc * ph2 - u * qn), b22 = (qn - tc * gam2 * u) + cs, b23 = (-tc * gamm1 * v);
+            double b24 = (-tc * gamm1 * w), b25 = tc * gamm1;
+            double b51 = qn * (ph2 - h), b52 = (tc * h - gamm1 * u * qn), b53 = (-gamm1 * v * qn);
+            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+            b[0] = cs * d1 + tc * d2;
+            b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
+            b[2] = (-v * qn) * d1 + (tc * v) * d2 + ((qn) + cs) * d3;
+            b[3] = (-w * qn) * d1 + (tc * w) * d2 + ((qn) + cs) * d4;
+            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+        } else if (D == 1) {
+            double qn = tc * v;
+            double b31 = (tc * ph2 - v * qn), b32 = (-tc * gamm1 * u), b33 = (qn - tc * gam2 * v) + cs;
+            double b34 = (-tc * gamm1 * w), b35 = tc * gamm1;
+            double b51 = qn * (ph2 - h), b52 = (-gamm1 * u * qn), b53 = (tc * h - gamm1 * v * qn);
+            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+            b[0] = cs * d1 + tc * d3;
+            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d3;
+            b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
+            b[3] = (-w * qn) * d1 + (tc * w) * d3 + ((qn) + cs) * d4;
+            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+        } else {
+            double qn = tc * w;
+            double b41 = tc * ph2 - w * qn, b42 = -tc * gamm1 * u, b43 = -tc * gamm1 * v;
+            double b44 = (qn - tc * gam2 * w) + cs, b45 = tc * gamm1;
+            double b51 = qn * (ph2 - h), b52 = -gamm1 * u * qn, b53 = -gamm1 * v * qn;
+            double b54 = tc * h - gamm1 * w * qn, b55 = gam1 * qn + cs;
+            b[0] = cs * d1 + tc * d4;
+            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d4;
+            b[2] = (-v * qn) * d1 + (qn + cs) * d3 + (tc * v) * d4;
+            b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
+            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+        }
+        return;
+    }
+    double qn = tcx * u + tcy * v + tcz * w;
+    double b21 = (tcx * ph2 - u * qn), b22 = (qn - tcx * gam2 * u) + cs;
+    double b23 = (tcy * u - tcx * gamm1 * v), b24 = (tcz * u - tcx * gamm1 * w), b25 = tcx * gamm1;
+    double b31 = (tcy * ph2 - v * qn), b32 = (tcx * v - tcy * gamm1 * u);
+    double b33 = (qn - tcy * gam2 * v) + cs, b34 = (tcz * v - tcy * gamm1 * w), b35 = tcy * gamm1;
+    double b41 = (tcz * ph2 - w * qn), b42 = (tcx * w - tcz * gamm1 * u);
+    double b43 = (tcy * w - tcz * gamm1 * v), b44 = (qn - tcz * gam2 * w) + cs, b45 = tcz * gamm1;
+    double b51 = qn * (ph2 - h), b52 = (tcx * h - gamm1 * u * qn), b53 = (tcy * h - gamm1 * v * qn);
+    double b54 = (tcz * h - gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+    b[0] = cs * d1 + tcx * d2 + tcy * d3 + tcz * d4;
+    b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
+    b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
+    b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
+    b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+}
+
+// Face normal of direction D used for the product of the cell at metric index lt: the mean of
+// its two faces (3dfull); the single Cartesian value otherwise.
+template <bool CART, int D>
+__device__ __forceinline__ void lu_normal(const KZ& z, long lt, double& tcx, double& tcy, double& tcz) {
+    if (CART) {
+        tcx = D == 0 ? z.ti[0] : D == 1 ? z.tj[0] : z.tk[0];
+        tcy = tcz = 0.;
+        return;
+    }
+    const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
+    long inc = D == 0 ? z.mi : D == 1 ? z.mj : z.mk;
+    long ndm = z.ndm, l2 = lt + inc;
+    tcx = 0.5 * (tm[l2] + tm[lt]);
+    tcy = 0.5 * (tm[l2 + ndm] + tm[lt + ndm]);
+    tcz = 0.5 * (tm[l2 + 2 * ndm] + tm[lt + 2 * ndm]);
+}
+
+// Product of a neighbour finished by an earlier launch, read back from global memory.
+template <bool CART, int D>
+__device__ __forceinline__ void lu_prod_global(const KZ& z, const LUC& c, const double* __restrict__ rop,
+                                               const double* __restrict__ dw, int i, int j, int k,
+                                               double signe, double b[5]) {
+    long nd = z.nd, l1 = IDX(z, i, j, k);
+    double tcx, tcy, tcz;
+    lu_normal<CART, D>(z, CART ? 0 : MTR(z, i, j, k), tcx, tcy, tcz);
+    double d[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) d[e] = dw[l1 + e * nd];
+    double cs = z.coe[l1 + (1 + D) * nd] * signe;
+    lu_prod<CART, D>(c, rop[l1 + nd], rop[l1 + 2 * nd], rop[l1 + 3 * nd], rop[l1 + 4 * nd], cs,
+                     tcx, tcy, tcz, d, b);
+}
+
+// LDS image of one block: the block's own fields (state, coefficients, right-hand side and,
+// for 3dfull, the three mean face normals), the products of the halo cells finished by earlier
+// launches, and the double-buffered j/k product exchange.  Rows are indexed by lane so that
+// the sweep reads (one cell per lane, lanes on different i) hit 16 distinct 8-byte banks.
+template <bool CART, int CI>
+struct LuLds {
+    static constexpr int NV = CART ? 15 : 24;
+    double st[NV][CI][64];
+    double hI[5][64];
+    double hJ[5][CI][8];
+    double hK[5][CI][8];
+    double ex[2][2][5][64];
+};
+
+// One block of CI x 8 x 8 cells (i x j x k, in sweep order) per launch diagonal.
+//  1. the 4 waves stage the block into LDS with coalesced loads (16 consecutive i per line)
+//     and form the products of the halo cells;
+//  2. wave 0 runs the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles
+//     cell ii = s - jj - kk at step s; results go straight to global memory.
+template <bool CART, bool LOWER, int CI>
+__global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int bk_lo, int nbi, int nbj) {
+    const int bj = blockIdx.x, bk = bk_lo + blockIdx.y, bi = diag - bj - bk;
+    if (bi < 0 || bi >= nbi || bj >= nbj) return;
+    __shared__ LuLds<CART, CI> S;
+    constexpr int NV = LuLds<CART, CI>::NV;
+    const int tid = threadIdx.x;
+    const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    const int oi0 = bi * CI;
+    const int ci = ni - oi0 < CI ? ni - oi0 : CI;
+    const int oj0 = bj * 8, ok0 = bk * 8;
+    const long nd = z.nd;
+    const LUC c = lu_const(z);
+    const double signe = LOWER ? 0.5 : -0.5;
+    const double* __restrict__ rop = a.rop;
+    const double* __restrict__ ropk = LOWER ? a.rop : a.rop_k;
+    double* __restrict__ dw = a.dw;
+    // sweep-order offsets -> zone indices
+    auto I = [&](int o) { return LOWER ? a.i0 + o : a.i1 - o; };
+    auto J = [&](int o) { return LOWER ? a.j0 + o : a.j1 - o; };
+    auto K = [&](int o) { return LOWER ? a.k0 + o : a.k1 - o; };
+
+    // 1a. stage the block
+    for (int e = tid; e < 64 * CI; e += 256) {
+        const int L = e / CI, ii = e % CI;
+        const int oj = oj0 + (L & 7), ok = ok0 + (L >> 3);
+        if (oj >= nj || ok >= nk || ii >= ci) continue;
+        const int i = I(oi0 + ii), j = J(oj), k = K(ok);
+        const long l = IDX(z, i, j, k);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            S.st[v][ii][L] = rop[l + v * nd];
+            S.st[5 + v][ii][L] = z.coe[l + v * nd];
+            S.st[10 + v][ii][L] = dw[l + v * nd];
+        }
+        if (!CART) {
+            const long lt = MTR(z, i, j, k);
+            double tx, ty, tz;
+            lu_normal<CART, 0>(z, lt, tx, ty, tz);
+            S.st[15][ii][L] = tx; S.st[16][ii][L] = ty; S.st[17][ii][L] = tz;
+            lu_normal<CART, 1>(z, lt, tx, ty, tz);
+            S.st[18][ii][L] = tx; S.st[19][ii][L] = ty; S.st[20][ii][L] = tz;
+            lu_normal<CART, 2>(z, lt, tx, ty, tz);
+            S.st[21][ii][L] = tx; S.st[22][ii][L] = ty; S.st[23][ii][L] = tz;
+        }
+    }
+    // 1b. products of the halo cells (finished by earlier launches)
+    for (int e = tid; e < 64 + 16 * CI; e += 256) {
+        double b[5];
+        if (e < 64) {
+            const int L = e, oj = oj0 + (L & 7), ok = ok0 + (L >> 3);
+            if (oi0 == 0 || oj >= nj || ok >= nk) continue;
+            lu_prod_global<CART, 0>(z, c, rop, dw, I(oi0 - 1), J(oj), K(ok), signe, b);
+#pragma unroll
+            for (int v = 0; v < 5; v++) S.hI[v][L] = b[v];
+        } else if (e < 64 + 8 * CI) {
+            const int q = e - 64, ii = q / 8, kk = q % 8, ok = ok0 + kk;
+            if (oj0 == 0 || ii >= ci || ok >= nk) continue;
+            lu_prod_global<CART, 1>(z, c, rop, dw, I(oi0 + ii), J(oj0 - 1), K(ok), signe, b);
+#pragma unroll
+            for (int v = 0; v < 5; v++) S.hJ[v][ii][kk] = b[v];
+        } else {
+            const int q = e - 64 - 8 * CI, ii = q / 8, jj = q % 8, oj = oj0 + jj;
+            if (ok0 == 0 || ii >= ci || oj >= nj) continue;
+            lu_prod_global<CART, 2>(z, c, ropk, dw, I(oi0 + ii), J(oj), K(ok0 - 1), signe, b);
+#pragma unroll
+            for (int v = 0; v < 5; v++) S.hK[v][ii][jj] = b[v];
+        }
+    }
+    __syncthreads();
+    if (tid >= 64) return;
+
+    // 2. skewed wavefront, one wave
+    const int L = tid, jj = L & 7, kk = L >> 3;
+    const int oj = oj0 + jj, ok = ok0 + kk;
+    const bool lane_ok = oj < nj && ok < nk;
+    const int j = J(oj), k = K(ok);
+    const bool kalias = !LOWER && a.rop_k != a.rop;
+    // the Cartesian face normals are read once: inside the loop the compiler could not keep
+    // them in registers across the stores to dw
+    const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
+    double pi[5] = {0., 0., 0., 0., 0.};
+    const int nsteps = ci + 14;
+    for (int s = 0; s < nsteps; s++) {
+        const int ii = s - jj - kk;
+        const int rd = (s + 1) & 1, wr = s & 1;
+        if (lane_ok && ii >= 0 && ii < ci) {
+            const int oi = oi0 + ii;
+            const long l = IDX(z, I(oi), j, k);
+            const double r0 = S.st[0][ii][L], u0 = S.st[1][ii][L], v0 = S.st[2][ii][L],
+                         w0 = S.st[3][ii][L], t0 = S.st[4][ii][L];
+            const double cf0 = S.st[5][ii][L], cf4 = S.st[9][ii][L];
+            const double xal = cf0 * signe;
+            double x[5], b[5];
+#pragma unroll
+            for (int e = 0; e < 5; e++) x[e] = LOWER ? S.st[10 + e][ii][L] : S.st[10 + e][ii][L] * cf4;
+            // neighbour terms in the reference order: lower 3dcart J,K,I; lower 3dfull and
+            // upper K,J,I
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int D = (LOWER && CART) ? (q == 0 ? 1 : q == 1 ? 2 : 0) : (q == 0 ? 2 : q == 1 ? 1 : 0);
+                const int o = D == 0 ? oi : D == 1 ? oj : ok;
+                if (o == 0) continue;
+                if (D == 0) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) b[e] = ii == 0 ? S.hI[e][L] : pi[e];
+                } else if (D == 1) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) b[e] = jj == 0 ? S.hJ[e][ii][kk] : S.ex[rd][0][e][L - 1];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) b[e] = kk == 0 ? S.hK[e][ii][jj] : S.ex[rd][1][e][L - 8];
+                }
+#pragma unroll
+                for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
+            }
+            const double diag = 1. / cf4;
+#pragma unroll
+            for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; }
+            double uk = u0, vk = v0, wk = w0, tk = t0;
+            if (!LOWER && a.newton) {
+                double* __restrict__ O = a.out;
+                const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
+                const double r1 = r0 + x[0];
+                const double r_1 = 1. / r1;
+                const double u1 = (r0 * u0 + x[1]) * r_1, v1 = (r0 * v0 + x[2]) * r_1, w1 = (r0 * w0 + x[3]) * r_1;
+                const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
+                const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+                O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
+                if (kalias) { uk = u1; vk = v1; wk = w1; tk = t1; }
+            }
+            // products of this cell for its three consumers
+            double tx, ty, tz;
+            if (CART) { tx = tci; ty = tz = 0.; }
+            else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
+            lu_prod<CART, 0>(c, u0, v0, w0, t0, S.st[6][ii][L] * signe, tx, ty, tz, x, pi);
+            if (CART) tx = tcj;
+            else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
+            lu_prod<CART, 1>(c, u0, v0, w0, t0, S.st[7][ii][L] * signe, tx, ty, tz, x, b);
+#pragma unroll
+            for (int e = 0; e < 5; e++) S.ex[wr][0][e][L] = b[e];
+            if (CART) tx = tck;
+            else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
+            lu_prod<CART, 2>(c, uk, vk, wk, tk, S.st[8][ii][L] * signe, tx, ty, tz, x, b);
+#pragma unroll
+            for (int e = 0; e < 5; e++) S.ex[wr][1][e][L] = b[e];
+        }
+        // the next step reads what the neighbouring lanes of this wave wrote; LDS accesses of
+        // one wave complete in issue order, so only the compiler must not reorder them
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    (void)NV;
+}
+
+// Copy the Newton-updated LU box back (in-place sub-iterations).
+__global__ void k_copy_box(KZ z, const double* __restrict__ src, double* __restrict__ dst, Box b) {
+    int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = b.k0 + blockIdx.y;
+    if (t >= ni * nj) return;
+    int i = b.i0 + t % ni, j = b.j0 + t / ni;
+    long nd = z.nd, l = IDX(z, i, j, k);
+#pragma unroll
+    for (int e = 0; e < 5; e++) dst[l + e * nd] = src[l + e * nd];
+}
+
+namespace {
+template <bool CART, bool LOWER>
+void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
+    // 8 cells per line keeps a block under 80 KB of LDS: two blocks per CU
+    constexpr int CI = 8;
+    int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    int nbi = (ni + CI - 1) / CI, nbj = (nj + 7) / 8, nbk = (nk + 7) / 8;
+    int ndiag = nbi + nbj + nbk - 2;
+    for (int d = 0; d < ndiag; d++) {
+        int lo = d - (nbi - 1) - (nbj - 1);
+        if (lo < 0) lo = 0;
+        int hi = d < nbk - 1 ? d : nbk - 1;
+        int jhi = d < nbj - 1 ? d : nbj - 1;
+        dim3 g(jhi + 1, hi - lo + 1);
+        hipLaunchKernelGGL((k_lu_block<CART, LOWER, CI>), g, dim3(256), 0, st, z, a, d, lo, nbi, nbj);
+    }
+}
+}  // namespace
+
+void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
+    if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
+    if (z.typ == 2) { sweep<true, true>(z, a, st); sweep<true, false>(z, a, st); }
+    else { sweep<false, true>(z, a, st); sweep<false, false>(z, a, st); }
+}
+
+void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st) {
+    int n0 = b.i1 - b.i0 + 1, n1 = b.j1 - b.j0 + 1, n2 = b.k1 - b.k0 + 1;
+    if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+    hipLaunchKernelGGL(k_copy_box, dim3((n0 * n1 + 255) / 256, n2), dim3(256), 0, st, z, src, dst, b);
+}
+
+}  // namespace fm

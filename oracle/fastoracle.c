@@ -1193,6 +1193,202 @@ static void bc_apply(const Ix* Z, const int* p, int bctype, int idir, int lrhs, 
             }
 }
 
+/* shape_tab_mtr.for: which metric components a face BC reads */
+static void shape_tab(int neq_mtr, int idir, int* ic, int* jc, int* kc, double* cim, double* cjm,
+                      double* ckm)
+{
+    if (neq_mtr == 3) { *ic = 1; *jc = 2; *kc = 3; *cim = 1.; *cjm = 1.; *ckm = 1.; }
+    else if (neq_mtr == 2) { *ic = 1; *jc = 2; *kc = 2; *cim = 1.; *cjm = 1.; *ckm = 0.; }
+    else if (neq_mtr == 1) {
+        *ic = 1; *jc = 1; *kc = 1;
+        *cim = idir <= 2 ? 1. : 0.;
+        *cjm = (idir == 3 || idir == 4) ? 1. : 0.;
+        *ckm = idir >= 5 ? 1. : 0.;
+    } else { *ic = 1; *jc = 1; *kc = 1; *cim = 0.; *cjm = 0.; *ckm = 0.; }
+}
+
+/* Characteristic (non-reflecting) state of the first ghost rank: FastS BC/non_reflection.for
+   with the exterior state (roe..ete) and the interior cell l1; returns the primitive ghost
+   state (BCFarfield.for / BCInflow_i.for epilogue) or, for BCOutflow, qn, sn and qvar5.      */
+typedef struct { double svar1, svar2, svar3, svar4, qvar5, r, u, v, w, qn, sn; } NonRefl;
+
+static NonRefl non_reflection(const double* rop, long nd, long l1, double roe, double rue,
+                              double rve, double rwe, double ete, double tcx, double tcy,
+                              double tcz, double gam, double gam1, double gam1_1, double cvinf)
+{
+    NonRefl o;
+    double sn = sqrt(tcx * tcx + tcy * tcy + tcz * tcz);
+    double roi = rop[l1];
+    double rui = rop[l1 + nd] * roi, rvi = rop[l1 + 2 * nd] * roi, rwi = rop[l1 + 3 * nd] * roi;
+    double eti = rop[l1 + 4 * nd] * cvinf * roi + .5 * (rui * rui + rvi * rvi + rwi * rwi) / roi;
+    double qen = 0.;   /* entrainment velocity: no mesh motion */
+    double qref1 = .5 * (roe + roi), qref2 = .5 * (rue + rui), qref3 = .5 * (rve + rvi);
+    double qref4 = .5 * (rwe + rwi), qref5 = .5 * (ete + eti);
+    double r = qref1, ri = 1. / r;
+    double u = qref2 * ri, v = qref3 * ri, w = qref4 * ri;
+    double p = gam1 * (qref5 - .5 * r * (u * u + v * v + w * w));
+    double c = sqrt(gam * p * ri), ci = 1. / c, s_1 = 1. / sn;
+    double qn = tcx * u + tcy * v + tcz * w - qen;
+    double tnx, tny;
+    if (sn == 0.) { tnx = 1.; tny = 0.; } else { tnx = tcx / sn; tny = tcy / sn; }
+    double eigv1 = qn, eigv2 = qn, eigv3 = qn, eigv4 = qn + sn * c, eigv5 = qn - sn * c;
+    double q[2][5] = {{roe, rue, rve, rwe, ete}, {roi, rui, rvi, rwi, eti}};
+    double ch[2][5];
+    for (int s = 0; s < 2; s++) {
+        double q1 = q[s][0], q2 = q[s][1], q3 = q[s][2], q4 = q[s][3], q5 = q[s][4];
+        double s1 = q1, s2 = (q2 - u * q1) * ri, s3 = (q3 - v * q1) * ri, s4 = (q4 - w * q1) * ri;
+        double s5 = gam1 * (.5 * (u * u + v * v + w * w) * q1 - u * q2 - v * q3 - w * q4 + q5);
+        ch[s][0] = s1 - ci * ci * s5;
+        ch[s][1] = tcz * s_1 * (tnx * s2 + tny * s3) - s4;
+        ch[s][2] = -tny * s2 + tnx * s3;
+        ch[s][3] = .5 * s_1 * (tcx * s2 + tcy * s3 + tcz * s4) + .5 * ri * ci * s5;
+        ch[s][4] = -.5 * s_1 * (tcx * s2 + tcy * s3 + tcz * s4) + .5 * ri * ci * s5;
+    }
+    double rv1 = ch[1][0], rv2 = ch[1][1], rv3 = ch[1][2], rv4 = ch[1][3], rv5 = ch[1][4];
+    if (eigv1 < 0.) rv1 = ch[0][0];
+    if (eigv2 < 0.) rv2 = ch[0][1];
+    if (eigv3 < 0.) rv3 = ch[0][2];
+    if (eigv4 < 0.) rv4 = ch[0][3];
+    if (eigv5 < 0.) rv5 = ch[0][4];
+    o.svar1 = rv1 + r * ci * (rv4 + rv5);
+    o.svar2 = tcz * s_1 * tnx * rv2 - tny * rv3 + tcx * s_1 * (rv4 - rv5);
+    o.svar3 = tcz * s_1 * tny * rv2 + tnx * rv3 + tcy * s_1 * (rv4 - rv5);
+    o.svar4 = -rv2 + tcz * s_1 * (rv4 - rv5);
+    double svar5 = r * c * (rv4 + rv5);
+    o.qvar5 = .5 * (u * u + v * v + w * w) * o.svar1 + r * (u * o.svar2 + v * o.svar3 + w * o.svar4)
+              + gam1_1 * svar5;
+    o.r = r; o.u = u; o.v = v; o.w = w; o.qn = qn; o.sn = sn;
+    return o;
+}
+
+/* Face BCs with a first-rank formula and an outer-rank rule, one ghost column at a time:
+   bvbs_farfield.for / bvbs_inflow.for (BCFarfield.for, then BC_nextrank.for),
+   bvbs_outflow.for (BCOutflow.for, then BC_extrap.for), bvbs_outpres.for
+   (BCOutpres_{i,j,k}.for + linearization_state.for, then BC_extrap.for),
+   bvbs_wall_inviscid.for (BCWallInviscid.for on every rank).
+   state: the BC's conservative state (BCzone.cpp DEFAULT_STATE); pext: per-face-point
+   pressure (outpres) indexed like indbci with inc_bc.                                      */
+static void bc_column(const Ix* Z, const int* p, const double* pr, int t, int idir,
+                      const int* ind, const double* tijk, int neq_mtr, const double* state,
+                      const double* pext, const int* inc_bc, double* rop)
+{
+    long nd = Z->nd, ndm = Z->ndm;
+    int d = (idir - 1) / 2, up = idir % 2 == 0;
+    int ic, jc, kc;
+    double cim, cjm, ckm;
+    shape_tab(neq_mtr, idir, &ic, &jc, &kc, &cim, &cjm, &ckm);
+    double snorm = up ? 1. : -1.;
+    double gam = pr[GAMMA], gam1 = pr[GAMMA] - 1., gam1_1 = 1. / gam1;
+    double rgp = pr[CVINF] * gam1, cv = pr[CVINF], cvinv = 1. / cv;
+    double c4 = 5. / 6., c5 = 2. / 6., c6 = -1. / 6.;
+    double c0 = 1. / c6, c1 = -(c4 + c5) * c0, c2 = -c6 * c0, c3 = (2. - c5 - c4) * c0;
+    long inc = d == 0 ? 1 : d == 1 ? Z->incj : Z->inck;
+    long sinc = up ? -inc : inc;                 /* towards the interior */
+    int g0 = up ? ind[2 * d] : ind[2 * d + 1];   /* first ghost rank */
+    int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    for (long b = ind[2 * tb]; b <= ind[2 * tb + 1]; b++)
+        for (long a = ind[2 * ta]; a <= ind[2 * ta + 1]; a++) {
+            long c[3];
+            c[ta] = a; c[tb] = b;
+            c[d] = up ? g0 : g0 + 1;
+            long lmtr = MTR(Z, c[0], c[1], c[2]);
+            double tx = tijk[lmtr + (ic - 1) * ndm], ty = tijk[lmtr + (jc - 1) * ndm];
+            double tz = tijk[lmtr + (kc - 1) * ndm];
+            if (t == 3) {
+                /* BCWallInviscid.for: mirror of the interior, normal velocity reversed */
+                double tcx = tx * cim, tcy = ty * cjm, tcz = tz * ckm;
+                double surf = sqrt(tcx * tcx + tcy * tcy + tcz * tcz);
+                surf = fmax2(surf, 1e-30);
+                double s_1 = 1. / surf;
+                tcx = tcx * s_1; tcy = tcy * s_1; tcz = tcz * s_1;
+                for (long g = ind[2 * d]; g <= ind[2 * d + 1]; g++) {
+                    c[d] = g;
+                    long l = IDX(Z, c[0], c[1], c[2]);
+                    long cm[3] = {c[0], c[1], c[2]};
+                    cm[d] = up ? 2 * ind[2 * d] - 1 - g : 2 * ind[2 * d + 1] + 1 - g;
+                    long ldjr = IDX(Z, cm[0], cm[1], cm[2]);
+                    /* the mesh velocity term (vent x c_ale) is +0 without mesh motion */
+                    double u = rop[ldjr + nd], v = rop[ldjr + 2 * nd], w = rop[ldjr + 3 * nd];
+                    double qn = (u * tcx + v * tcy + w * tcz);
+                    double ut = u - qn * tcx, vt = v - qn * tcy, wt = w - qn * tcz;
+                    double ua = 2. * ut - u, va = 2. * vt - v, wa = 2. * wt - w;
+                    rop[l] = rop[ldjr];
+                    rop[l + nd] = 0. + ua;
+                    rop[l + 2 * nd] = 0. + va;
+                    rop[l + 3 * nd] = 0. + wa;
+                    rop[l + 4 * nd] = rop[ldjr + 4 * nd];
+                }
+                continue;
+            }
+            double tcx = tx * cim * snorm, tcy = ty * cjm * snorm, tcz = tz * ckm * snorm;
+            c[d] = g0;
+            long l = IDX(Z, c[0], c[1], c[2]);
+            long l1 = l + sinc;
+            if (t == 1 || t == 13) {
+                NonRefl o = non_reflection(rop, nd, l1, state[0], state[1], state[2], state[3],
+                                           state[4], tcx, tcy, tcz, gam, gam1, gam1_1, pr[CVINF]);
+                double roinv = 1. / o.svar1;
+                double u = o.u + o.r * o.svar2 * roinv, v = o.v + o.r * o.svar3 * roinv;
+                double w = o.w + o.r * o.svar4 * roinv;
+                rop[l] = o.svar1; rop[l + nd] = u; rop[l + 2 * nd] = v; rop[l + 3 * nd] = w;
+                rop[l + 4 * nd] = (o.qvar5 * roinv - .5 * (u * u + v * v + w * w)) * cvinv;
+            } else if (t == 10) {
+                double roe = rop[l1], rue = rop[l1 + nd] * roe, rve = rop[l1 + 2 * nd] * roe;
+                double rwe = rop[l1 + 3 * nd] * roe;
+                double roe_inv = 1. / state[0];
+                double ete = state[4] - 0.5 * (state[1] * state[1] + state[2] * state[2] + state[3] * state[3]) * roe_inv
+                             + 0.5 * (rue * rop[l1 + nd] + rve * rop[l1 + 2 * nd] + rwe * rop[l1 + 3 * nd]);
+                NonRefl o = non_reflection(rop, nd, l1, roe, rue, rve, rwe, ete, tcx, tcy, tcz, gam,
+                                           gam1, gam1_1, pr[CVINF]);
+                rop[l] = rop[l1];
+                double roinv = 1. / rop[l];
+                if (o.qn < 0) {
+                    rop[l + nd] = rop[l1 + nd] - o.qn / (o.sn * o.sn) * tcx;
+                    rop[l + 2 * nd] = rop[l1 + 2 * nd] - o.qn / (o.sn * o.sn) * tcy;
+                    rop[l + 3 * nd] = rop[l1 + 3 * nd] - o.qn / (o.sn * o.sn) * tcz;
+                } else {
+                    rop[l + nd] = rop[l1 + nd]; rop[l + 2 * nd] = rop[l1 + 2 * nd];
+                    rop[l + 3 * nd] = rop[l1 + 3 * nd];
+                }
+                rop[l + 4 * nd] = (o.qvar5 * roinv - .5 * (rop[l + nd] * rop[l + nd] + rop[l + 2 * nd] * rop[l + 2 * nd]
+                                                          + rop[l + 3 * nd] * rop[l + 3 * nd])) * cvinv;
+            } else { /* 16: BCOutpres_{i,j,k}.for with linearization_state.for */
+                long li = (a - inc_bc[1]) + (b - inc_bc[2]) * inc_bc[0];
+                double sn = sqrt(tcx * tcx + tcy * tcy + tcz * tcz);
+                double s_1 = 1. / sn, tnx, tny, tnz;
+                if (sn == 0.) { tnx = 1.; tny = 0.; tnz = 0.; }
+                else { tnx = tcx * s_1; tny = tcy * s_1; tnz = tcz * s_1; }
+                double roi = rop[l1], ui = rop[l1 + nd], vi = rop[l1 + 2 * nd], wi = rop[l1 + 3 * nd];
+                double Ti = rop[l1 + 4 * nd], pi = roi * rgp * Ti;
+                double ro0 = rop[l1], T0 = rop[l1 + 4 * nd], p0 = ro0 * rgp * T0;
+                double roc0 = sqrt(ro0 * gam * p0);
+                double dqn = (pext[li] - pi) / roc0;
+                double rog = roi + ro0 * ro0 * dqn / roc0;
+                rop[l] = rog;
+                rop[l + nd] = ui - dqn * tnx; rop[l + 2 * nd] = vi - dqn * tny; rop[l + 3 * nd] = wi - dqn * tnz;
+                rop[l + 4 * nd] = pext[li] / (rog * rgp);
+            }
+            /* outer ranks in the reference loop order (ascending index) */
+            int glo = up ? g0 + 1 : ind[2 * d], ghi = up ? ind[2 * d + 1] : g0 - 1;
+            for (int g = glo; g <= ghi; g++) {
+                c[d] = g;
+                long lg = IDX(Z, c[0], c[1], c[2]);
+                long q0 = lg + sinc, q1 = lg + 2 * sinc, q2 = lg + 3 * sinc;
+                if (t == 1 || t == 13) {   /* BC_nextrank.for */
+                    rop[lg] = rop[q0] * c3 + rop[q1] * c1 + rop[q2] * c2;
+                    if (fabs(rop[lg] / rop[q0] - 1.) >= 0.7) {
+                        for (int e = 0; e < 5; e++) rop[lg + e * nd] = rop[q0 + e * nd];
+                    } else {
+                        for (int e = 1; e < 5; e++)
+                            rop[lg + e * nd] = rop[q0 + e * nd] * c3 + rop[q1 + e * nd] * c1 + rop[q2 + e * nd] * c2;
+                    }
+                } else {                   /* BC_extrap.for: copy of the first rank */
+                    for (int e = 0; e < 5; e++) rop[lg + e * nd] = rop[l + e * nd];
+                }
+            }
+        }
+}
+
 /* correct_coins.for */
 static void fo_correct_coins(const int* p, const int* ind_loop, double* rop)
 {
@@ -1303,8 +1499,13 @@ static void correct_coins_all(const int* p, const int* ind_loop, double* rop)
 }
 
 /* BCzone.cpp:60-865 for one thread owning the zone (npass = 0).  Supported types:
-   0 extrapolate, 11 autoperiod, 4/6 viscous adiabatic wall.  Returns -1 otherwise.      */
-static int fo_bczone(int* p, double* pr, int lrhs, int* shift_lu, double* rop, int* lcorner_out)
+   0 extrapolate, 1 farfield, 3/5 (and 4 in Euler) inviscid wall, 4/6 viscous adiabatic
+   wall, 10 outflow, 11 autoperiod, 13 inflow (free-stream state), 16 outpres (pressure data).
+   Returns -1 otherwise, and for Roe with walls (correct_flux.for is not restated).        */
+static int is_wall_type(int t) { return t == 3 || t == 4 || t == 5 || t == 6 || t == 7 || t == 12; }
+
+static int fo_bczone(int* p, double* pr, const double* ti, const double* tj, const double* tk,
+                     int lrhs, int* shift_lu, double* rop, int* lcorner_out)
 {
     Ix zz = mkix(p); Ix* Z = &zz;
     (void)pr;
@@ -1344,10 +1545,41 @@ static int fo_bczone(int* p, double* pr, int lrhs, int* shift_lu, double* rop, i
         if (cl_window(p, idir, p + pt + BC_FEN, cl)) continue;
         if (lrhs == 1 && (t == 0 || t == 1 || t == 2 || t == 10 || t == 11 || t == 14 || t == 16 ||
                           t == 17 || t == 20 || t == 21)) t = 0;
+        if (p[KFLUDOM] == 5 && is_wall_type(t)) return -1;  /* Roe wall-flux correction (correct_flux.for) */
+        const double* tijk = idir <= 2 ? ti : idir <= 4 ? tj : tk;
+        int neq_mtr = idir <= 4 ? p[NEQ_IJ] : p[NEQ_K];
+        int nbdata = p[pt + BC_NBDATA];
+        const double* data = nbdata ? pr + p[pt_bcs + 1 + n + nb_bc] : NULL;
+        /* BCzone.cpp DEFAULT_STATE: free stream unless the BC carries data */
+        double st[5];
+        const double* state = st;
+        if (nbdata == 0) {
+            st[0] = pr[ROINF]; st[1] = pr[VXINF] * pr[ROINF]; st[2] = pr[VYINF] * pr[ROINF];
+            st[3] = pr[VZINF] * pr[ROINF];
+            st[4] = pr[PINF] / (pr[GAMMA] - 1.) + 0.5 * (pr[VXINF] * st[1] + pr[VYINF] * st[2] + pr[VZINF] * st[3]);
+        } else state = data;
         if (t == 0 || t == 11) bc_apply(Z, p, t, idir, lrhs, cl, rop);
-        else if (p[KFLUDOM] == 5) return -1;  /* Roe wall-flux correction (correct_flux.for) */
+        else if (t == 1 || t == 10) bc_column(Z, p, pr, t, idir, cl, tijk, neq_mtr, state, NULL, NULL, rop);
+        else if (t == 13) {
+            if (nbdata != 0) return -1;           /* BCInflow from file data (bvbs_inflow_fich) */
+            bc_column(Z, p, pr, 13, idir, cl, tijk, neq_mtr, state, NULL, NULL, rop);
+        } else if (t == 3 || (t == 4 && p[IFLOW] == 1) || t == 5)
+            bc_column(Z, p, pr, 3, idir, cl, tijk, neq_mtr, state, NULL, NULL, rop);
         else if ((t == 6 || t == 4) && p[IFLOW] > 1) bc_apply(Z, p, 6, idir, lrhs, cl, rop);
-        else return -1;
+        else if (t == 6) { /* viscous wall in an Euler zone: BCzone.cpp has no branch for it */ }
+        else if (t == 16) {
+            /* BCOutpres: one pressure per face point (nbdata 1; a leading 1-value array is the
+               mesh-motion coefficient and is skipped) */
+            const int* fen = p + pt + BC_FEN;
+            const int* size_data = p + pt + BC_NBDATA + 1;
+            int inc_bc[3];
+            if (idir <= 2) { inc_bc[0] = fen[3] - fen[2] + 1; inc_bc[1] = fen[2]; inc_bc[2] = fen[4]; }
+            else if (idir <= 4) { inc_bc[0] = fen[1] - fen[0] + 1; inc_bc[1] = fen[0]; inc_bc[2] = fen[4]; }
+            else { inc_bc[0] = fen[1] - fen[0] + 1; inc_bc[1] = fen[0]; inc_bc[2] = fen[2]; }
+            if (nbdata == 0 || nbdata > 2) return -1;
+            const double* pext = size_data[0] == 1 ? data + 1 : data;
+            bc_column(Z, p, pr, 16, idir, cl, tijk, neq_mtr, state, pext, inc_bc, rop);
+        } else return -1;
     }
     if (lcorner_out) *lcorner_out = lcorner;
     return 0;
@@ -1358,8 +1590,8 @@ int fo_apply_bc(int* p, double* pr, int nstep, double* rop, double* ti, double* 
 {
     int shift[6];
     int ind_dm[6] = {1, p[IJKV], 1, p[IJKV + 1], 1, p[IJKV + 2]};
-    (void)nstep; (void)ti; (void)tj; (void)tk; (void)x; (void)y; (void)z; (void)xmut;
-    int rc = fo_bczone(p, pr, 0, shift, rop, NULL);
+    (void)nstep; (void)x; (void)y; (void)z; (void)xmut;
+    int rc = fo_bczone(p, pr, ti, tj, tk, 0, shift, rop, NULL);
     if (rc) return rc;
     correct_coins_all(p, ind_dm, rop);
     return 0;
@@ -1405,7 +1637,7 @@ int fo_substep(int* p, double* pr, int nitcfg, int nitrun, int first_it, int lss
     if (stop == 1) return 0;
     if (implicit) {
         int shift_lu[6], lcorner = 1;
-        if (fo_bczone(p, pr, 1, shift_lu, drodm, &lcorner)) return -1;
+        if (fo_bczone(p, pr, ti, tj, tk, 1, shift_lu, drodm, &lcorner)) return -1;
         if (lcorner == 0) correct_coins_all(p, shift_lu, drodm);
         if (lssiter_verif == 1 && rdm) fo_residual_blocks(p, nitcfg, p[NSSITER], drodm, rdm);
         if (stop == 2) return 0;
@@ -1414,7 +1646,7 @@ int fo_substep(int* p, double* pr, int nitcfg, int nitrun, int first_it, int lss
     if (stop == 3) return 0;
     if (lexit_lu == 0) {
         int shift[6];
-        if (fo_bczone(p, pr, 0, shift, ro_cl, NULL)) return -1;
+        if (fo_bczone(p, pr, ti, tj, tk, 0, shift, ro_cl, NULL)) return -1;
         correct_coins_all(p, sdm, ro_cl);
     }
     return 0;

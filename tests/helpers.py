@@ -50,3 +50,24 @@ def numerics(scheme, mode, **kw):
 
 def rotate(z):
     m1 = z.ro_m1.copy(); z.ro_m1[:] = z.ro; z.ro[:] = z.ro_p1; z.ro_p1[:] = m1
+
+
+INVISCID_J = [("BCautoperiod", "imin", None), ("BCautoperiod", "imax", None),
+              ("BCWallInviscid", "jmin", None), ("BCWallInviscid", "jmax", None),
+              ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+FARFIELD_OUTFLOW = [("BCFarfield", "imin", None), ("BCOutflow", "imax", None),
+                    ("BCWallViscous", "jmin", None), ("BCFarfield", "jmax", None),
+                    ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+
+
+def inflow_outpres(n, p_exit=None, seed=5):
+    """Inflow (free stream) at imin, outpres at imax with one exit pressure per face cell,
+    viscous wall at jmin, farfield at jmax, periodic in k."""
+    nj, nk = n[1], n[2]
+    if p_exit is None:
+        st = cases.tgv_state()
+        p0 = st["p"]
+        p_exit = p0 * (1.0 + 0.01 * np.random.default_rng(seed).uniform(-1, 1, nj * nk))
+    return [("BCInflow", "imin", None), ("BCOutpres", "imax", [p_exit]),
+            ("BCWallViscous", "jmin", None), ("BCFarfield", "jmax", None),
+            ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]

@@ -120,6 +120,30 @@ def test_walls_lu_extension(ctxmod, orc, bcs, mode):
     assert_same(zo, zg, ghosts=True)
 
 
+BC_CASES = {
+    # channel with a bump: farfield in/out, viscous walls, periodic k (3dfull)
+    "channel": lambda sch, mode: cases.channel_zone(18, 12, 6, H.numerics(sch, mode)),
+    # Euler zone with inviscid (slip) walls
+    "inviscid": lambda sch, mode: H.jittered_box((9, 10, 6), H.numerics(sch, mode), bcs=H.INVISCID_J, flow="euler"),
+    # farfield inlet, outflow exit
+    "outflow": lambda sch, mode: H.jittered_box((11, 7, 5), H.numerics(sch, mode), bcs=H.FARFIELD_OUTFLOW),
+    # free-stream inflow, exit pressure per face point
+    "outpres": lambda sch, mode: H.jittered_box((9, 6, 5), H.numerics(sch, mode), bcs=H.inflow_outpres((9, 6, 5))),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BC_CASES))
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "local", "bdf_ss2"])
+@pytest.mark.parametrize("scheme", ["ausmpred", "senseur"])
+def test_boundary_conditions(ctxmod, orc, case, mode, scheme):
+    """Farfield / inflow (characteristic, BC_nextrank), outflow and outpres (BC_extrap), and
+    inviscid walls against the oracle, ghost cells included, over two time steps."""
+    nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
+    mk = lambda: BC_CASES[case](scheme, mode)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
 def test_norms_match_residual_blocks(ctxmod, orc):
     """GPU L2/Linf against the oracle's per-block sums (cprdu3s1.for), relative 1e-12."""
     mk = lambda: cases.tgv_zone((24, 20, 16), H.numerics("roe_min", "bdf"), perturb=0.02)

@@ -69,9 +69,9 @@ requires_ref = pytest.mark.skipif(not oracle.Reference.available(),
 @requires_ref
 @pytest.mark.parametrize("scheme", ["roe_min", "roe_kap", "roe_nul", "ausmpred", "senseur"])
 @pytest.mark.parametrize("mode", ["explicit", "bdf", "bdf_ss2", "local"])
-@pytest.mark.parametrize("mesh", ["cart", "full", "wall"])
+@pytest.mark.parametrize("mesh", ["cart", "full", "wall", "channel", "inviscid", "outflow", "outpres"])
 def test_oracle_vs_reference(orc, scheme, mode, mesh):
-    if mesh == "wall" and scheme.startswith("roe"):
+    if mesh not in ("cart", "full") and scheme.startswith("roe"):
         pytest.skip("Roe with walls needs the wall-flux correction (correct_flux.for), not implemented")
     ref = oracle.Reference()
     nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
@@ -79,8 +79,16 @@ def test_oracle_vs_reference(orc, scheme, mode, mesh):
         mk = lambda: cases.tgv_zone((9, 8, 7), H.numerics(scheme, mode), perturb=0.02)
     elif mesh == "full":
         mk = lambda: H.jittered_box((8, 7, 6), H.numerics(scheme, mode))
-    else:
+    elif mesh == "wall":
         mk = lambda: H.jittered_box((8, 9, 5), H.numerics(scheme, mode), bcs=H.WALL_J)
+    elif mesh == "channel":
+        mk = lambda: cases.channel_zone(10, 8, 4, H.numerics(scheme, mode))
+    elif mesh == "inviscid":
+        mk = lambda: H.jittered_box((7, 8, 5), H.numerics(scheme, mode), bcs=H.INVISCID_J, flow="euler")
+    elif mesh == "outflow":
+        mk = lambda: H.jittered_box((8, 7, 5), H.numerics(scheme, mode), bcs=H.FARFIELD_OUTFLOW)
+    else:
+        mk = lambda: H.jittered_box((8, 6, 5), H.numerics(scheme, mode), bcs=H.inflow_outpres((8, 6, 5)))
     zr, zo = mk(), mk()
     p_ref, _ = ref.classify(zr)
     assert np.array_equal(p_ref[:P.N_PARAM_INT], zr.param_int[:P.N_PARAM_INT])
