@@ -22,9 +22,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# compulsory bytes per interior cell-iteration on a Cartesian zone (FP64, 5 equations):
-# residual 80 (read W, write R), LU lower 120, LU upper + update 120
-BYTES_PER_CELL = {"residual": 80.0, "lu": 240.0, "iteration": 320.0}
+# Algorithmic (compulsory) HBM bytes per interior cell on a Cartesian zone, FP64, 5 equations,
+# for the implicit BDF iteration with one Newton sub-iteration:
+#   residual: read W^n 40 + W^{n-1} 40, write R 40                       = 120
+#   LU-SGS:   lower (read R 40 + W 40, write dW* 40) + upper and Newton
+#             update (read dW* 40 + W 40, write W^{n+1} 40)               = 240
+#   iteration (320 of the LU-SGS iteration + 80 for the two BDF levels)  = 400
+#   coefficients (read W 40, write mu 8 + coe 40), counted separately     =  88
+BYTES_PER_CELL = {"coe": 88.0, "residual": 120.0, "lu": 240.0, "bc": 0.0, "iteration": 400.0}
+KERNELS = {"coe": "k_coe", "residual": "k_residual", "lu": "k_lu_pipe (lower + upper sweeps)",
+           "bc": "k_bc_face"}
 
 
 def parse():
@@ -37,6 +44,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phase-reps", type=int, default=5)
+    ap.add_argument("--lu", choices=["pipeline", "blocks"], default="pipeline")
     return ap.parse_args()
 
 
@@ -49,14 +57,14 @@ def make_zone(n, scheme):
     return z
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
+def traffic_from_profiles(phase):
+    """HBM bytes per launch of a phase's kernels from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_tgv256.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
-        return d.get(kernel, {}).get("hbm_bytes_per_launch")
+        return d.get(phase, {}).get("hbm_bytes_per_launch")
     except Exception:
         return None
 
@@ -98,6 +106,7 @@ def main():
     from fast_amd import _lib
     z = make_zone(a.n, a.scheme)
     ctx = _lib.Context(0)
+    ctx.set_option(_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE if a.lu == "pipeline" else _lib.LU_BLOCKS)
     zid = ctx.add_zone(z)
     ctx.apply_bc(zid, 0)
     buf = np.zeros_like(z.ro)
@@ -126,20 +135,24 @@ def main():
         phases[name] = ctx.time_phase(zid, ph, reps=a.phase_reps)
     ctx.close()
     step_ms = wall / a.steps * 1e3
-    res_ms = phases["residual"]
-    achieved = BYTES_PER_CELL["residual"] * ncell / (res_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_residual", "achieved": round(achieved, 1),
+    dom = max(("residual", "lu"), key=lambda k: phases[k])
+    dom_ms = phases[dom]
+    achieved = BYTES_PER_CELL[dom] * ncell / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles("k_residual"),
-                "bytes_per_cell": BYTES_PER_CELL["residual"],
-                "launch_ms": round(res_ms, 4)}
+                "traffic": traffic_from_profiles(dom),
+                "bytes_per_cell": BYTES_PER_CELL[dom],
+                "launch_ms": round(dom_ms, 4),
+                "phase_gbs": {k: round(BYTES_PER_CELL[k] * ncell / (v * 1e-3) / 1e9, 1)
+                              for k, v in phases.items() if BYTES_PER_CELL[k] > 0}}
     it_gbs = value * 1e6 * BYTES_PER_CELL["iteration"] / 1e9
     out = {"metric": "Mcell-iter/s NS residual+LU-SGS at 256^3", "value": round(value, 2),
            "unit": "Mcell-iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(step_ms, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
            "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
-                      % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic"},
+                      % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic",
+                      "lu_schedule": a.lu},
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "event_ms_per_step": round(ms_ev / a.steps, 3),

@@ -284,6 +284,14 @@ def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, lay
                           ("_P1", _lib.FIELD_RO_P1, hz.ro_p1)):
             ctx.get(zid, f, a)
             _level_to_tree(z, hz, sfx, a)
+        # CFL max / min / mean of the last verification step (computePT.cpp:629-645)
+        try:
+            cfl = ctx.cfl(zid)
+        except _lib.FastMIError:
+            cfl = None
+        if cfl is not None:
+            o = Internal.createUniqueChild(z, ".Solver#ownData", "UserDefinedData_t")
+            Internal.createUniqueChild(o, "CFL_minmaxmoy", "DataArray_t", cfl)
     if vtune is not None:
         return t1 - t0, 0.0
     return None
@@ -348,11 +356,12 @@ def _createConvergenceHistory(t, nrecs):
 
 
 def _computeEnstrophy(t, metrics):
-    """Returns (enstrophy, tke) summed over the zones; the enstrophy term is not computed
-    (None); tke = sum 1/2 rho |u|^2 vol / sum vol over interior cells."""
+    """(enstrophy, tke) volume averages over the interior cells of all zones
+    (computePT_enstrophy.cpp:206-227): sum 1/2 rho |curl u|^2 vol / sum vol and
+    sum 1/2 rho |u|^2 vol / sum vol, from the N level."""
     ctx = HOOK["ctx"]
-    k = v = 0.0
+    k = e = v = 0.0
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
         s = ctx.tke(zid, 0)
-        k += s[0]; v += s[1]
-    return None, k / v
+        k += s[0]; v += s[1]; e += s[2]
+    return e / v, k / v

@@ -38,8 +38,14 @@ SIGNATURES = {
     "fm_enstrophy_tke": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
     "fm_time_phase": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float)]),
+    "fm_ctx_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "fm_cfl": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
+OPT_LU_SCHEDULE = 1
+LU_BLOCKS, LU_PIPELINE = 0, 1
+OPT_RESIDUAL_SCHEDULE = 2
+RES_CELLS, RES_FACES = 0, 1
 
 
 class FastMIError(RuntimeError):
@@ -90,6 +96,9 @@ class Context:
         check(self.lib.fm_ctx_create(ctypes.byref(h), device))
         self.h = h
         self.zones = []
+
+    def set_option(self, option, value):
+        check(self.lib.fm_ctx_set_option(self.h, option, value))
 
     def close(self):
         if self.h:
@@ -151,8 +160,14 @@ class Context:
         check(self.lib.fm_residual_norms(self.h, zone, dptr(out)))
         return out
 
+    def cfl(self, zone):
+        out = np.zeros(3)
+        check(self.lib.fm_cfl(self.h, zone, dptr(out)))
+        return out
+
     def tke(self, zone, level=0):
-        out = np.zeros(2)
+        """(sum 1/2 rho|u|^2 vol, sum vol, sum 1/2 rho|curl u|^2 vol) of a time level."""
+        out = np.zeros(3)
         check(self.lib.fm_enstrophy_tke(self.h, zone, level, dptr(out)))
         return out
 

@@ -48,13 +48,17 @@ struct Zone {
     KZ k;
     double* ro[3];           // current N, N-1, N+1
     double* ro_scratch;      // Newton target when the LU sweep works in place
+    double* lu_dws;          // dW* between the pipelined LU sweeps
+    int* lu_sync;            // tile counter and error word of the pipelined LU sweeps
     double* norm_part; double* norm_out; int norm_blocks;
-    double h_norms[10]; int have_norms;
+    double h_norms[10]; int have_norms, have_cfl;
     long n_metric;
 };
 
 struct fm_ctx {
     int device;
+    int lu_schedule;         // LU_PIPELINE or LU_BLOCKS
+    int res_schedule;        // RES_FACES or RES_CELLS
     hipStream_t stream;
     std::vector<Zone*> zones;
     hipEvent_t e0, e1;
@@ -67,6 +71,8 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     HIPCHK(hipSetDevice(device));
     fm_ctx* c = new fm_ctx();
     c->device = device;
+    c->lu_schedule = LU_PIPELINE;
+    c->res_schedule = RES_FACES;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
     HIPCHK(hipEventCreate(&c->e0));
@@ -81,6 +87,8 @@ static void free_zone(Zone* z) {
                       (double*)z->k.z, z->norm_part, z->norm_out};
     for (double* b : bufs) if (b) (void)hipFree(b);
     for (BcRec& b : z->bcs) if (b.pext) (void)hipFree(b.pext);
+    if (z->lu_dws) (void)hipFree(z->lu_dws);
+    if (z->lu_sync) (void)hipFree(z->lu_sync);
     delete z;
 }
 
@@ -139,7 +147,8 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->p.assign(p, p + n_int);
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
-    z->norm_part = z->norm_out = nullptr; z->have_norms = 0;
+    z->lu_dws = nullptr; z->lu_sync = nullptr;
+    z->norm_part = z->norm_out = nullptr; z->have_norms = 0; z->have_cfl = 0;
     KZ& k = z->k;
     k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
     k.nit = p[NIJK]; k.njt = p[NIJK + 1]; k.nkt = p[NIJK + 2];
@@ -221,12 +230,18 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     double *xd, *yd, *zd, *mtr;
     if ((rc = dmalloc(&z->ro[0], 5 * nd)) || (rc = dmalloc(&z->ro[1], 5 * nd)) ||
         (rc = dmalloc(&z->ro[2], 5 * nd)) || (rc = dmalloc(&z->ro_scratch, 5 * nd)) ||
+        (p[ITYPCP] <= 1 && (rc = dmalloc(&z->lu_dws, 5 * nd))) ||
         (rc = dmalloc(&k.xmut, nd)) || (rc = dmalloc(&k.drodm, 5 * nd)) ||
         (rc = dmalloc(&k.coe, (long)p[NEQ_COE] * nd)) || (rc = dmalloc(&k.wig, 3 * nd)) ||
         (rc = dmalloc(&mtr, nmtr)) || (rc = dmalloc(&xd, nxyz)) || (rc = dmalloc(&yd, nxyz)) ||
         (rc = dmalloc(&zd, nxyz))) {
         free_zone(z);
         return rc;
+    }
+    if (hipMalloc((void**)&z->lu_sync, 4 * sizeof(int)) != hipSuccess ||
+        hipMemset(z->lu_sync, 0, 4 * sizeof(int)) != hipSuccess) {
+        free_zone(z);
+        return fail(FM_E_NOMEM, "hipMalloc of the LU counters failed");
     }
     k.ti = mtr; k.tj = mtr + ndm * p[NEQ_IJ]; k.tk = mtr + 2 * ndm * p[NEQ_IJ];
     k.vol = mtr + ndm * (2 * p[NEQ_IJ] + p[NEQ_K]);
@@ -403,9 +418,15 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             launch_coe(k, ro_ssiter, first_it, 0, b, lamin, st);
         }
     }
+    // cptst3.for:114-119: CFL monitor at the first sub-iteration of a verification step
+    if (nitcfg == 1 && lssiter_verif) {
+        launch_cfl(k, ro_ssiter, z->norm_part, z->norm_blocks, z->norm_out + 10, st);
+        z->have_cfl = 1;
+    }
     if (k.kflu == 2 && nitcfg == 1) launch_wiggle(k, ro_ssiter, st);
     RhsArgs a;
     memset(&a, 0, sizeof a);
+    a.schedule = c->res_schedule;
     a.rop = ro_ssiter; a.rop_n = ro; a.rop_n1 = ro_m1;
     a.implicit = implicit; a.nitcfg = nitcfg;
     a.zero_init = (implicit || nitcfg == 1);
@@ -436,6 +457,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.i0 = shift_lu[0]; la.i1 = shift_lu[1]; la.j0 = shift_lu[2]; la.j1 = shift_lu[3];
             la.k0 = shift_lu[4]; la.k1 = shift_lu[5];
             la.dw = k.drodm; la.newton = 1;
+            la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
             la.rop = ro_ssiter;
             if (ro_ssiter == ro_p1) {
                 // in place (sub-iterations >= 2): the K terms of the upper sweep see the state
@@ -463,6 +485,19 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
     return FM_OK;
 }
 
+// A pipelined LU sweep whose bounded waits gave up has produced garbage: report it.
+static int check_lu(fm_ctx* c) {
+    for (Zone* z : c->zones) {
+        LuArgs la;
+        memset(&la, 0, sizeof la);
+        la.schedule = LU_PIPELINE; la.sync = z->lu_sync;
+        int f = lu_pipeline_failed(la, c->stream);
+        if (f < 0) return fail(FM_E_HIP, "reading the LU error word failed");
+        if (f > 0) return fail(FM_E_HIP, "pipelined LU-SGS sweep timed out waiting for a neighbour tile");
+    }
+    return FM_OK;
+}
+
 extern "C" int fm_zone_substep(fm_ctx* c, int zone, int nitcfg, int nitrun, int first_it,
                                int lssiter_verif, int lexit_lu, int stop) {
     (void)nitrun;
@@ -473,7 +508,7 @@ extern "C" int fm_zone_substep(fm_ctx* c, int zone, int nitcfg, int nitrun, int 
     int rc = substep(c, z, nitcfg, first_it, lssiter_verif, lexit_lu, stop);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
-    return FM_OK;
+    return check_lu(c);
 }
 
 extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
@@ -500,10 +535,12 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
             for (Zone* z : c->zones) {
                 int implicit = z->k.itypcp <= 1;
                 int nssiter = z->k.nssiter;
-                int lssiter_verif = 0, lexit_lu = 0;
+                // souszones_list.cpp:44-84: a verification step computes residuals (and the
+                // CFL monitor) at every sub-iteration and skips LU-SGS at the last one
+                int lssiter_verif = verif, lexit_lu = 0;
                 if (implicit && nstep == nssiter) {
                     if (!verif) continue;           // skip (computePT.cpp:581-582)
-                    lssiter_verif = 1; lexit_lu = 1;
+                    lexit_lu = 1;
                 }
                 int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 0);
                 if (rc) return rc;
@@ -540,7 +577,36 @@ extern "C" int fm_compute_timed(fm_ctx* c, int nitrun, int nstep_deb, int nstep_
 extern "C" int fm_sync(fm_ctx* c) {
     if (!c) return fail(FM_E_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
+    return check_lu(c);
+}
+
+extern "C" int fm_cfl(fm_ctx* c, int zone, double* out) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !out) return fail(FM_E_ARG, "fm_cfl: bad arguments");
+    if (!z->have_cfl) return fail(FM_E_ARG, "no CFL computed yet (first sub-iteration of a verification step)");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double h[3];
+    HIPCHK(hipMemcpy(h, z->norm_out + 10, sizeof h, hipMemcpyDeviceToHost));
+    // computePT.cpp:629-645: max, min and the mean over the interior cells
+    out[0] = h[0]; out[1] = h[1];
+    out[2] = h[2] / ((double)z->k.ni * z->k.nj * z->k.nk);
     return FM_OK;
+}
+
+extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
+    if (!c) return fail(FM_E_ARG, "null context");
+    if (option == FM_OPT_LU_SCHEDULE) {
+        if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE) return fail(FM_E_ARG, "LU schedule %d", value);
+        c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : LU_BLOCKS;
+        return FM_OK;
+    }
+    if (option == FM_OPT_RESIDUAL_SCHEDULE) {
+        if (value != FM_RES_CELLS && value != FM_RES_FACES) return fail(FM_E_ARG, "residual schedule %d", value);
+        c->res_schedule = value == FM_RES_FACES ? RES_FACES : RES_CELLS;
+        return FM_OK;
+    }
+    return fail(FM_E_ARG, "unknown option %d", option);
 }
 
 extern "C" int fm_residual_norms(fm_ctx* c, int zone, double* out) {
@@ -565,12 +631,12 @@ extern "C" int fm_enstrophy_tke(fm_ctx* c, int zone, int level, double* out) {
     int nblk = z->norm_blocks;
     launch_tke(z->k, z->ro[level], z->norm_part, nblk, c->stream);
     HIPCHK(hipGetLastError());
-    std::vector<double> h(2 * nblk);
+    std::vector<double> h(3 * nblk);
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(h.data(), z->norm_part, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost));
-    double sk = 0., sv = 0.;
-    for (int b = 0; b < nblk; b++) { sk += h[2 * b]; sv += h[2 * b + 1]; }
-    out[0] = sk; out[1] = sv;
+    HIPCHK(hipMemcpy(h.data(), z->norm_part, sizeof(double) * 3 * nblk, hipMemcpyDeviceToHost));
+    double sk = 0., se = 0., sv = 0.;
+    for (int b = 0; b < nblk; b++) { sk += h[3 * b]; se += h[3 * b + 1]; sv += h[3 * b + 2]; }
+    out[0] = sk; out[1] = sv; out[2] = se;
     return FM_OK;
 }
 
@@ -586,12 +652,14 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     double* ro = z->ro[0];
     RhsArgs a;
     memset(&a, 0, sizeof a);
+    a.schedule = c->res_schedule;
     a.rop = ro; a.rop_n = ro; a.rop_n1 = z->ro[1]; a.implicit = implicit; a.nitcfg = 1;
     a.zero_init = 1; a.coefW = 0.5; a.rk_in = ro; a.rk_out = z->ro_scratch;
     a.c1 = -1.5; a.c2 = 2.; a.c3 = -0.5;
     LuArgs la;
     la.i0 = 1; la.i1 = k.ni; la.j0 = 1; la.j1 = k.nj; la.k0 = 1; la.k1 = k.nk;
     la.dw = k.drodm; la.rop = ro; la.rop_k = ro; la.out = z->ro_scratch; la.newton = 1;
+    la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipEventRecord(c->e0, st));

@@ -199,6 +199,55 @@ __global__ void k_wiggle(KZ z, const double* __restrict__ rop) {
     if (jj && kk) wiggle_one(z, rop, l, 1, 0);
 }
 
+// Update after the face balance (acc): core3as2.for (implicit, writes the Newton residual
+// into drodm) or core3ark3.for (explicit, keeps drodm and writes the new stage state).
+__device__ __forceinline__ void res_update(const KZ& z, const RhsArgs& a, const double* __restrict__ rop,
+                                           long l, const double acc[5]) {
+    long nd = z.nd;
+    double cf = z.coe[l];
+    if (a.implicit) {
+        const double* A = rop; const double* B = a.rop_n; const double* C = a.rop_n1;
+        double cv = z.cv, c1 = a.c1, c2 = a.c2, c3 = a.c3;
+        double al = A[l], cl = C[l];
+        if (a.nitcfg == 1) {
+            double c4 = c1 + c2;
+            z.drodm[l] = cf * acc[0] + c4 * al + c3 * cl;
+#pragma unroll
+            for (int m = 1; m < 4; m++)
+                z.drodm[l + m * nd] = cf * acc[m] + c4 * A[l + m * nd] * al + c3 * C[l + m * nd] * cl;
+            double roe = al * (cv * A[l + 4 * nd] + 0.5 * (A[l + nd] * A[l + nd] + A[l + 2 * nd] * A[l + 2 * nd] + A[l + 3 * nd] * A[l + 3 * nd]));
+            double roe_n1 = cl * (cv * C[l + 4 * nd] + 0.5 * (C[l + nd] * C[l + nd] + C[l + 2 * nd] * C[l + 2 * nd] + C[l + 3 * nd] * C[l + 3 * nd]));
+            z.drodm[l + 4 * nd] = cf * acc[4] + c4 * roe + c3 * roe_n1;
+        } else {
+            double bl = B[l];
+            z.drodm[l] = cf * acc[0] + c1 * al + c2 * bl + c3 * cl;
+#pragma unroll
+            for (int m = 1; m < 4; m++)
+                z.drodm[l + m * nd] = cf * acc[m] + c1 * A[l + m * nd] * al + c2 * B[l + m * nd] * bl + c3 * C[l + m * nd] * cl;
+            double roe = al * (cv * A[l + 4 * nd] + 0.5 * (A[l + nd] * A[l + nd] + A[l + 2 * nd] * A[l + 2 * nd] + A[l + 3 * nd] * A[l + 3 * nd]));
+            double roe_n = bl * (cv * B[l + 4 * nd] + 0.5 * (B[l + nd] * B[l + nd] + B[l + 2 * nd] * B[l + 2 * nd] + B[l + 3 * nd] * B[l + 3 * nd]));
+            double roe_n1 = cl * (cv * C[l + 4 * nd] + 0.5 * (C[l + nd] * C[l + nd] + C[l + 2 * nd] * C[l + 2 * nd] + C[l + 3 * nd] * C[l + 3 * nd]));
+            z.drodm[l + 4 * nd] = cf * acc[4] + c1 * roe + c2 * roe_n + c3 * roe_n1;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 5; e++) z.drodm[l + e * nd] = acc[e];
+        double cv = z.cv, cvinv = 1. / cv, cvinv2 = 0.5 * cvinv;
+        double r_1 = a.coefW * cf;
+        double t1 = acc[0] * r_1, t2 = acc[1] * r_1, t3 = acc[2] * r_1, t4 = acc[3] * r_1, t5 = acc[4] * r_1;
+        const double* R = a.rk_in;
+        double* O = a.rk_out;
+        double ro_old = R[l], u_old = R[l + nd], v_old = R[l + 2 * nd], w_old = R[l + 3 * nd], t_old = R[l + 4 * nd];
+        double r1 = ro_old + t1;
+        O[l] = r1;
+        r_1 = 1. / r1;
+        double u1 = (ro_old * u_old + t2) * r_1, v1 = (ro_old * v_old + t3) * r_1, w1 = (ro_old * w_old + t4) * r_1;
+        O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1;
+        double roe_old = ro_old * (cv * t_old + 0.5 * (u_old * u_old + v_old * v_old + w_old * w_old));
+        O[l + 4 * nd] = (roe_old + t5) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Residual: fused init_rhs + 6 face fluxes + balance + update, one thread per interior cell.
 // Accumulation order per cell as the reference sweep (template_FluxAndBalance.for:192-271):
@@ -241,47 +290,75 @@ __global__ void __launch_bounds__(256) k_residual(KZ z, RhsArgs a) {
 #pragma unroll
     for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
 
-    double cf = z.coe[l];
-    if (a.implicit) {
-        const double* A = rop; const double* B = a.rop_n; const double* C = a.rop_n1;
-        double cv = z.cv, c1 = a.c1, c2 = a.c2, c3 = a.c3;
-        double al = A[l], cl = C[l];
-        if (a.nitcfg == 1) {
-            double c4 = c1 + c2;
-            z.drodm[l] = cf * acc[0] + c4 * al + c3 * cl;
+    res_update(z, a, rop, l, acc);
+}
+
+// Residual with every face flux evaluated once.  Block: 64 i-lanes x 5 j-rows of threads
+// marching RES_KC planes in k.  Lane tx of rows 0..3 evaluates the i-face at i0 + tx (its cell
+// i0 + tx takes face i + 1 from lane tx + 1, so a block covers 63 cells); row r evaluates the
+// j-face at j0 + r (row 4 only provides the face above the last cell row); each cell carries
+// the k-face under it from the previous plane.  Per cell the faces are summed in the reference
+// order (template_FluxAndBalance.for:192-271), so the result is the one of k_residual.
+constexpr int RES_KC = 32;
+
+template <int KFLU, int SLOPE, bool LAMIN, bool CART>
+__global__ void __launch_bounds__(320) k_residual_faces(KZ z, RhsArgs a) {
+    __shared__ double fj[2][5][5][64];        // [plane parity][row][eq][lane]
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * 63 + tx + 1;
+    const int j = blockIdx.y * 4 + ty + 1;
+    const int k0 = blockIdx.z * RES_KC + 1;
+    const int k1 = k0 + RES_KC - 1 < z.nk ? k0 + RES_KC - 1 : z.nk;
+    const bool cell = tx < 63 && ty < 4 && i <= z.ni && j <= z.nj;
+    const bool iface = ty < 4 && j <= z.nj && i <= z.ni + 1;
+    const bool jface = tx < 63 && i <= z.ni && j <= z.nj + 1;
+    const double* __restrict__ rop = a.rop;
+    const long nd = z.nd;
+    FluxConst f = flux_const(z);
+    double fk[5] = {0., 0., 0., 0., 0.};
+    if (cell) face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k0, fk);
+    for (int k = k0; k <= k1; k++) {
+        const int par = k & 1;
+        double fl[5];
+        if (jface) {
+            face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j, k, fl);
 #pragma unroll
-            for (int m = 1; m < 4; m++)
-                z.drodm[l + m * nd] = cf * acc[m] + c4 * A[l + m * nd] * al + c3 * C[l + m * nd] * cl;
-            double roe = al * (cv * A[l + 4 * nd] + 0.5 * (A[l + nd] * A[l + nd] + A[l + 2 * nd] * A[l + 2 * nd] + A[l + 3 * nd] * A[l + 3 * nd]));
-            double roe_n1 = cl * (cv * C[l + 4 * nd] + 0.5 * (C[l + nd] * C[l + nd] + C[l + 2 * nd] * C[l + 2 * nd] + C[l + 3 * nd] * C[l + 3 * nd]));
-            z.drodm[l + 4 * nd] = cf * acc[4] + c4 * roe + c3 * roe_n1;
-        } else {
-            double bl = B[l];
-            z.drodm[l] = cf * acc[0] + c1 * al + c2 * bl + c3 * cl;
-#pragma unroll
-            for (int m = 1; m < 4; m++)
-                z.drodm[l + m * nd] = cf * acc[m] + c1 * A[l + m * nd] * al + c2 * B[l + m * nd] * bl + c3 * C[l + m * nd] * cl;
-            double roe = al * (cv * A[l + 4 * nd] + 0.5 * (A[l + nd] * A[l + nd] + A[l + 2 * nd] * A[l + 2 * nd] + A[l + 3 * nd] * A[l + 3 * nd]));
-            double roe_n = bl * (cv * B[l + 4 * nd] + 0.5 * (B[l + nd] * B[l + nd] + B[l + 2 * nd] * B[l + 2 * nd] + B[l + 3 * nd] * B[l + 3 * nd]));
-            double roe_n1 = cl * (cv * C[l + 4 * nd] + 0.5 * (C[l + nd] * C[l + nd] + C[l + 2 * nd] * C[l + 2 * nd] + C[l + 3 * nd] * C[l + 3 * nd]));
-            z.drodm[l + 4 * nd] = cf * acc[4] + c1 * roe + c2 * roe_n + c3 * roe_n1;
+            for (int e = 0; e < 5; e++) fj[par][ty][e][tx] = fl[e];
         }
-    } else {
+        double fi[5] = {0., 0., 0., 0., 0.};
+        if (iface) face_flux<KFLU, SLOPE, LAMIN, CART, 0>(f, z, rop, i, j, k, fi);
+        double fi1[5];
 #pragma unroll
-        for (int e = 0; e < 5; e++) z.drodm[l + e * nd] = acc[e];
-        double cv = z.cv, cvinv = 1. / cv, cvinv2 = 0.5 * cvinv;
-        double r_1 = a.coefW * cf;
-        double t1 = acc[0] * r_1, t2 = acc[1] * r_1, t3 = acc[2] * r_1, t4 = acc[3] * r_1, t5 = acc[4] * r_1;
-        const double* R = a.rk_in;
-        double* O = a.rk_out;
-        double ro_old = R[l], u_old = R[l + nd], v_old = R[l + 2 * nd], w_old = R[l + 3 * nd], t_old = R[l + 4 * nd];
-        double r1 = ro_old + t1;
-        O[l] = r1;
-        r_1 = 1. / r1;
-        double u1 = (ro_old * u_old + t2) * r_1, v1 = (ro_old * v_old + t3) * r_1, w1 = (ro_old * w_old + t4) * r_1;
-        O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1;
-        double roe_old = ro_old * (cv * t_old + 0.5 * (u_old * u_old + v_old * v_old + w_old * w_old));
-        O[l + 4 * nd] = (roe_old + t5) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+        for (int e = 0; e < 5; e++) fi1[e] = __shfl_down(fi[e], 1, 64);
+        double fk1[5];
+        if (cell) face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k + 1, fk1);
+        __syncthreads();
+        if (cell) {
+            const long l = IDX(z, i, j, k);
+            double acc[5];
+            if (a.zero_init) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) acc[e] = 0.;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 5; e++) acc[e] = z.drodm[l + e * nd] * a.coefH;
+            }
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] + fk[e];
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] + fj[par][ty][e][tx];
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] + fi[e];
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fi1[e];
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj[par][ty + 1][e][tx];
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk1[e];
+            res_update(z, a, rop, l, acc);
+#pragma unroll
+            for (int e = 0; e < 5; e++) fk[e] = fk1[e];
+        }
     }
 }
 
@@ -610,6 +687,72 @@ __global__ void k_norm_partial(KZ z, const double* __restrict__ dr, double* __re
     }
 }
 
+// cpcfl3.for without mesh motion: local CFL dt / min(viscous, convective time) per interior
+// cell; partial max / min / sum per block (the sum order differs from the reference loop).
+__global__ void k_cfl_partial(KZ z, const double* __restrict__ rop, double* __restrict__ part) {
+    __shared__ double smax[256], smin[256], ssum[256];
+    int tid = threadIdx.x;
+    long ncell = (long)z.ni * z.nj * z.nk;
+    double rgp = z.cv * (z.gam - 1.), gam1 = z.gam / z.prandt, gam2 = z.gam * rgp, dt = z.dtc;
+    bool lamin = z.iflow >= 2, cart = z.typ == 2;
+    double surf_c = 0.;
+    if (cart) {
+        double ddi = 2. * sqrt(z.ti[0] * z.ti[0]), ddj = 2. * sqrt(z.tj[0] * z.tj[0]), ddk = 2. * sqrt(z.tk[0] * z.tk[0]);
+        surf_c = 1. / dmax(dmax(ddi, ddj), ddk);
+    }
+    double cmax = 0., cmin = 1e8, csum = 0.;
+    long ndm = z.ndm;
+    for (long cidx = blockIdx.x * (long)blockDim.x + tid; cidx < ncell; cidx += (long)gridDim.x * blockDim.x) {
+        int i = 1 + (int)(cidx % z.ni); long r = cidx / z.ni; int j = 1 + (int)(r % z.nj); int k = 1 + (int)(r / z.nj);
+        long l = IDX(z, i, j, k);
+        double surf, vo;
+        if (cart) { surf = surf_c; vo = z.vol[0]; }
+        else {
+            const double* ti = z.ti; const double* tj = z.tj; const double* tk = z.tk;
+            long lt = MTR(z, i, j, k), li = lt + z.mi, lj = lt + z.mj, lk = lt + z.mk;
+            double ddi = sqrt(ti[li] * ti[li] + ti[li + ndm] * ti[li + ndm] + ti[li + 2 * ndm] * ti[li + 2 * ndm]);
+            ddi = ddi + sqrt(ti[lt] * ti[lt] + ti[lt + ndm] * ti[lt + ndm] + ti[lt + 2 * ndm] * ti[lt + 2 * ndm]);
+            double ddj = sqrt(tj[lj] * tj[lj] + tj[lj + ndm] * tj[lj + ndm] + tj[lj + 2 * ndm] * tj[lj + 2 * ndm]);
+            ddj = ddj + sqrt(tj[lt] * tj[lt] + tj[lt + ndm] * tj[lt + ndm] + tj[lt + 2 * ndm] * tj[lt + 2 * ndm]);
+            double ddk = sqrt(tk[lk] * tk[lk] + tk[lk + ndm] * tk[lk + ndm] + tk[lk + 2 * ndm] * tk[lk + 2 * ndm]);
+            ddk = ddk + sqrt(tk[lt] * tk[lt] + tk[lt + ndm] * tk[lt + ndm] + tk[lt + 2 * ndm] * tk[lt + 2 * ndm]);
+            surf = 1. / dmax(dmax(ddi, ddj), ddk);
+            vo = z.vol[lt];
+        }
+        long nd = z.nd;
+        double u = rop[l + nd], v = rop[l + 2 * nd], w = rop[l + 3 * nd], rr = rop[l];
+        double c = sqrt(gam2 * rop[l + 4 * nd]);
+        double sp = sqrt(u * u + v * v + w * w);
+        double xinvspc = 1. / (sp + c);
+        double conv = 2. * vo * surf * xinvspc;
+        double cfloc;
+        if (lamin) {
+            double xinvkt = gam1 / z.xmut[l];
+            double vis = 2. * rr * vo * vo * surf * surf * xinvkt;
+            cfloc = dt / dmin(vis, conv);
+        } else cfloc = dt / conv;
+        cmax = dmax(cmax, cfloc); cmin = dmin(cmin, cfloc); csum = csum + cfloc;
+    }
+    smax[tid] = cmax; smin[tid] = cmin; ssum[tid] = csum;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            smax[tid] = dmax(smax[tid], smax[tid + w]);
+            smin[tid] = dmin(smin[tid], smin[tid + w]);
+            ssum[tid] += ssum[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) { part[blockIdx.x * 3] = smax[0]; part[blockIdx.x * 3 + 1] = smin[0]; part[blockIdx.x * 3 + 2] = ssum[0]; }
+}
+
+__global__ void k_cfl_final(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double m = 0., n = 1e8, s = 0.;
+    for (int b = 0; b < nblk; b++) { m = dmax(m, part[3 * b]); n = dmin(n, part[3 * b + 1]); s += part[3 * b + 2]; }
+    out[0] = m; out[1] = n; out[2] = s;
+}
+
 __global__ void k_norm_final(const double* __restrict__ part, int nblk, double* __restrict__ out) {
     if (threadIdx.x != 0) return;
     double s[5] = {0, 0, 0, 0, 0}, m[5] = {0, 0, 0, 0, 0};
@@ -622,27 +765,72 @@ __global__ void k_norm_final(const double* __restrict__ part, int nblk, double* 
 }
 
 // Kinetic energy and volume sums over the interior (POST/computePT_enstrophy.cpp, tke part)
-__global__ void k_tke_partial(KZ z, const double* __restrict__ ro, double* __restrict__ part) {
-    __shared__ double sk[256];
-    __shared__ double sv[256];
+// Kinetic energy and enstrophy sums (POST/post.for): Green-Gauss velocity gradients from
+// 4-point face values (grad_{i,j,k}_3d{cart,full}.for, c1 = 7/12, c2 = 1/12) accumulated per
+// component in the reference loop order, divided by the volume (cp_gradu.for), then
+// 1/2 rho |u|^2 vol, 1/2 rho |curl u|^2 vol and vol summed (cptaylor.for; the sums are reduced
+// in another order than the reference loop).  On 3dfull zones the i-face above the last cell
+// carries the metric index of the face below it, as in cp_gradu_3dfull.for.
+__device__ __forceinline__ double post_face(const double* __restrict__ q, long l, long inc) {
+    const double c1 = 7. / 12, c2 = 1. / 12;
+    return (q[l] + q[l - inc]) * c1 - (q[l + inc] + q[l - 2 * inc]) * c2;
+}
+
+__global__ void k_post_partial(KZ z, const double* __restrict__ ro, double* __restrict__ part) {
+    __shared__ double sk[256], se[256], sv[256];
     int tid = threadIdx.x;
     long ncell = (long)z.ni * z.nj * z.nk;
-    double ak = 0., av = 0.;
+    double ak = 0., ae = 0., av = 0.;
+    const long nd = z.nd, ndm = z.ndm;
+    const long inc[3] = {1, z.incj, z.inck};
+    const long minc[3] = {z.mi, z.mj, z.mk};
+    const double* tm[3] = {z.ti, z.tj, z.tk};
+    const bool cart = z.typ == 2;
     for (long c = blockIdx.x * (long)blockDim.x + tid; c < ncell; c += (long)gridDim.x * blockDim.x) {
         int i = 1 + (int)(c % z.ni); long r = c / z.ni; int j = 1 + (int)(r % z.nj); int k = 1 + (int)(r / z.nj);
         long l = IDX(z, i, j, k);
-        double vol = z.typ == 2 ? z.vol[0] : z.vol[MTR(z, i, j, k)];
-        double u = ro[l + z.nd], v = ro[l + 2 * z.nd], w = ro[l + 3 * z.nd];
-        ak += 0.5 * ro[l] * (u * u + v * v + w * w) * vol;
-        av += vol;
+        long lt = cart ? 0 : MTR(z, i, j, k);
+        double vo = cart ? z.vol[0] : z.vol[lt];
+        double volinv = 1. / vo;
+        double g[3][3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            const double* q = ro + (1 + v) * nd;
+            double flo[3], fhi[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) { flo[d] = post_face(q, l, inc[d]); fhi[d] = post_face(q, l + inc[d], inc[d]); }
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                double a;
+                if (cart) {
+                    double t = tm[x][0];
+                    a = (0. - flo[x] * t) + fhi[x] * t;
+                } else {
+                    long lhi = (i == z.ni) ? lt : lt + minc[0];
+                    a = 0.;
+                    a = a - flo[2] * tm[2][lt + x * ndm];
+                    a = a - flo[1] * tm[1][lt + x * ndm];
+                    a = a - flo[0] * tm[0][lt + x * ndm];
+                    a = a + fhi[0] * tm[0][lhi + x * ndm];
+                    a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
+                    a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                }
+                g[v][x] = a * volinv;
+            }
+        }
+        double rho = ro[l], u = ro[l + nd], v = ro[l + 2 * nd], w = ro[l + 3 * nd];
+        ak += 0.5 * rho * (u * u + v * v + w * w) * vo;
+        double w1 = (g[2][1] - g[1][2]), w2 = (g[0][2] - g[2][0]), w3 = (g[1][0] - g[0][1]);
+        ae += 0.5 * rho * (w1 * w1 + w2 * w2 + w3 * w3) * vo;
+        av += vo;
     }
-    sk[tid] = ak; sv[tid] = av;
+    sk[tid] = ak; se[tid] = ae; sv[tid] = av;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (tid < w) { sk[tid] += sk[tid + w]; sv[tid] += sv[tid + w]; }
+        if (tid < w) { sk[tid] += sk[tid + w]; se[tid] += se[tid + w]; sv[tid] += sv[tid + w]; }
         __syncthreads();
     }
-    if (tid == 0) { part[blockIdx.x * 2] = sk[0]; part[blockIdx.x * 2 + 1] = sv[0]; }
+    if (tid == 0) { part[blockIdx.x * 3] = sk[0]; part[blockIdx.x * 3 + 1] = se[0]; part[blockIdx.x * 3 + 2] = sv[0]; }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -679,9 +867,32 @@ static void res_lc(const KZ& z, const RhsArgs& a, bool lamin, dim3 g, hipStream_
     }
 }
 
+template <int KFLU, int SLOPE>
+static void res_faces(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st) {
+    dim3 g((z.ni + 62) / 63, (z.nj + 3) / 4, (z.nk + RES_KC - 1) / RES_KC), b(64, 5);
+    if (lamin) {
+        if (z.typ == 2) hipLaunchKernelGGL((k_residual_faces<KFLU, SLOPE, true, true>), g, b, 0, st, z, a);
+        else hipLaunchKernelGGL((k_residual_faces<KFLU, SLOPE, true, false>), g, b, 0, st, z, a);
+    } else {
+        if (z.typ == 2) hipLaunchKernelGGL((k_residual_faces<KFLU, SLOPE, false, true>), g, b, 0, st, z, a);
+        else hipLaunchKernelGGL((k_residual_faces<KFLU, SLOPE, false, false>), g, b, 0, st, z, a);
+    }
+}
+
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st) {
-    dim3 g((z.ni + 63) / 64, (z.nj + 3) / 4, z.nk);
     bool lamin = z.iflow == 2;
+    if (a.schedule == RES_FACES) {
+        if (z.kflu == 5) {
+            if (z.slope == 1) res_faces<5, 1>(z, a, lamin, st);
+            else if (z.slope == 2) res_faces<5, 2>(z, a, lamin, st);
+            else if (z.slope == 3) res_faces<5, 3>(z, a, lamin, st);
+            else return -1;
+        } else if (z.kflu == 1 && z.slope == 2) res_faces<1, 2>(z, a, lamin, st);
+        else if (z.kflu == 2 && z.slope == 2) res_faces<2, 2>(z, a, lamin, st);
+        else return -1;
+        return 0;
+    }
+    dim3 g((z.ni + 63) / 64, (z.nj + 3) / 4, z.nk);
     if (z.kflu == 5) {
         if (z.slope == 1) res_lc<5, 1>(z, a, lamin, g, st);
         else if (z.slope == 2) res_lc<5, 2>(z, a, lamin, g, st);
@@ -720,8 +931,13 @@ void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double*
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, st, part, nblk, out);
 }
 
+void launch_cfl(const KZ& z, const double* rop, double* part, int nblk, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl_partial, dim3(nblk), dim3(256), 0, st, z, rop, part);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(64), 0, st, part, nblk, out);
+}
+
 void launch_tke(const KZ& z, const double* ro, double* part, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_tke_partial, dim3(nblk), dim3(256), 0, st, z, ro, part);
+    hipLaunchKernelGGL(k_post_partial, dim3(nblk), dim3(256), 0, st, z, ro, part);
 }
 
 }  // namespace fm

@@ -17,7 +17,9 @@ struct RhsArgs {
     double coefH, coefW;    // RK coefficients
     int implicit, nitcfg;
     double c1, c2, c3;      // BDF coefficients
+    int schedule;           // RES_FACES (each face once) or RES_CELLS (six faces per cell)
 };
+enum { RES_CELLS = 0, RES_FACES = 1 };
 
 struct LuArgs {
     int i0, i1, j0, j1, k0, k1;   // LU range (shift_lu)
@@ -26,7 +28,13 @@ struct LuArgs {
     const double* rop_k;          // state for the K terms of the upper sweep
     double* out;                  // Newton-updated state
     int newton;
+    // pipelined schedule (LU_PIPELINE): dW* buffer between the sweeps and the work counter /
+    // error word of the persistent tiles
+    double* dws;
+    int* sync;
+    int schedule;                 // LU_BLOCKS or LU_PIPELINE
 };
+enum { LU_BLOCKS = 0, LU_PIPELINE = 1 };
 
 struct BcArgs {
     int w[6];     // ghost window (1-based cell indices)
@@ -47,10 +55,12 @@ void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, i
 void launch_wiggle(const KZ& z, const double* rop, hipStream_t st);
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st);
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
+int lu_pipeline_failed(const LuArgs& a, hipStream_t st);
 void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st);
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);
 void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st);
 void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double* out, hipStream_t st);
+void launch_cfl(const KZ& z, const double* rop, double* part, int nblk, double* out, hipStream_t st);
 void launch_tke(const KZ& z, const double* ro, double* part, int nblk, hipStream_t st);
 
 }  // namespace fm

@@ -300,6 +300,267 @@ __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int 
     (void)NV;
 }
 
+// ---------------------------------------------------------------------------------------
+// Pipelined schedule: one persistent wave per 8 x 8 tile of (j, k) lines, each wave sweeping
+// its full i-lines in the skewed order above.  A tile needs, at its step s, cells that the
+// tiles before it in j and k finish at their step s + 7, so each tile simply trails its two
+// producers by about eight steps and the whole sweep takes about ni + nj + nk steps instead
+// of one launch per block diagonal.
+//
+// Cross-tile values travel through global memory and validate themselves: before a sweep the
+// output box is filled with a signalling-NaN pattern that arithmetic never produces; results
+// are stored with agent-scope (L2-coherent) stores, and a consumer re-reads a neighbour value
+// with agent-scope loads until it is no longer the pattern.  Every wait is bounded: past the
+// bound, or once any wave has given up, the error word is set and the kernel runs to its end
+// with whatever it read, so the grid always drains; the host then reports the failure.
+// Tiles are taken from a counter in dependency order (by j-tile + k-tile), so a tile only
+// waits on tiles that started before it: no deadlock whatever the dispatch order.
+constexpr long long kLuPending = 0x7FF4C0DEC0DE0001ll;
+constexpr int kLuSpinMax = 1 << 22;
+
+__device__ __forceinline__ double lu_ldc(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lu_stc(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// ticket -> tile, enumerating the tiles by diagonal jt + kt
+__device__ __forceinline__ void lu_tile_of(int t, int ntj, int ntk, int& jt, int& kt) {
+    for (int d = 0; d < ntj + ntk - 1; d++) {
+        int lo = d - (ntj - 1) > 0 ? d - (ntj - 1) : 0, hi = d < ntk - 1 ? d : ntk - 1;
+        int cnt = hi - lo + 1;
+        if (t < cnt) { kt = lo + t; jt = d - kt; return; }
+        t -= cnt;
+    }
+    jt = kt = 0;
+}
+
+// Everything one step of a lane reads from global memory, fetched P steps ahead so that the
+// memory latency overlaps the arithmetic of the steps in between.  The neighbour values of
+// another tile may still be the pending pattern when fetched; they are re-read at use.
+template <bool CART>
+struct LuSlot {
+    double w[5], cf[5], in[5];            // own state, coefficients, right-hand side
+    double rj[5], wj[4], cj;              // j neighbour in the previous tile: result, state, coe
+    double rk[5], wk[4], ck;              // k neighbour in the previous tile
+    double mo[CART ? 1 : 18];             // 3dfull: own faces (D, component, low/high)
+    double mj[CART ? 1 : 6], mk[CART ? 1 : 6];
+};
+
+__device__ __forceinline__ bool lu_any_pending(const double* v, int n) {
+    bool p = false;
+#pragma unroll
+    for (int e = 0; e < 5; e++)
+        if (e < n) p = p || __double_as_longlong(v[e]) == kLuPending;
+    return p;
+}
+
+// re-read n values (stride nd) until none is the pending pattern; bounded
+__device__ __forceinline__ void lu_settle(double* v, int n, const double* base, long nd, int* err) {
+    int spins = 0;
+    while (lu_any_pending(v, n)) {
+        if (++spins > kLuSpinMax || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int e = 0; e < 5; e++)
+            if (e < n) v[e] = lu_ldc(base + e * nd);
+    }
+}
+
+template <bool CART, bool LOWER, int P>
+__global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __restrict__ in,
+                                                double* res, int ntj, int ntk) {
+    __shared__ double ex[2][2][5][64];
+    __shared__ int s_tile;
+    int* ticket = a.sync;
+    int* err = a.sync + 1;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1);
+    __syncthreads();
+    int jt, kt;
+    lu_tile_of(s_tile, ntj, ntk, jt, kt);
+    const int L = threadIdx.x, jj = L & 7, kk = L >> 3;
+    const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    const int oj = jt * 8 + jj, ok = kt * 8 + kk;
+    const bool lane_ok = oj < nj && ok < nk;
+    const int j = LOWER ? a.j0 + oj : a.j1 - oj, k = LOWER ? a.k0 + ok : a.k1 - ok;
+    const int sgn = LOWER ? -1 : 1;
+    const long nd = z.nd, ndm = z.ndm;
+    const LUC c = lu_const(z);
+    const double signe = LOWER ? 0.5 : -0.5;
+    const double* __restrict__ rop = a.rop;
+    const bool kalias = !LOWER && a.rop_k != a.rop;
+    const double* kstate = kalias ? a.out : (LOWER ? rop : a.rop_k);
+    const bool has_j = jj == 0 && oj > 0, has_k = kk == 0 && ok > 0;
+    const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
+    const double* tm[3] = {z.ti, z.tj, z.tk};
+    const long minc[3] = {z.mi, z.mj, z.mk};
+
+    auto fetch = [&](LuSlot<CART>& q, int s) {
+        const int ii = s - jj - kk;
+        if (!(lane_ok && ii >= 0 && ii < ni)) return;
+        const int i = LOWER ? a.i0 + ii : a.i1 - ii;
+        const long l = IDX(z, i, j, k);
+#pragma unroll
+        for (int e = 0; e < 5; e++) { q.w[e] = rop[l + e * nd]; q.cf[e] = z.coe[l + e * nd]; q.in[e] = in[l + e * nd]; }
+        if (!CART) {
+            const long lt = MTR(z, i, j, k);
+#pragma unroll
+            for (int D = 0; D < 3; D++)
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) {
+                    q.mo[6 * D + 2 * cc] = tm[D][lt + cc * ndm];
+                    q.mo[6 * D + 2 * cc + 1] = tm[D][lt + minc[D] + cc * ndm];
+                }
+        }
+        if (has_j) {
+            const long l1 = IDX(z, i, j + sgn, k);
+#pragma unroll
+            for (int e = 0; e < 5; e++) q.rj[e] = lu_ldc(res + l1 + e * nd);
+#pragma unroll
+            for (int e = 0; e < 4; e++) q.wj[e] = rop[l1 + (e + 1) * nd];
+            q.cj = z.coe[l1 + 2 * nd];
+            if (!CART) {
+                const long lt1 = MTR(z, i, j + sgn, k);
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) { q.mj[2 * cc] = z.tj[lt1 + cc * ndm]; q.mj[2 * cc + 1] = z.tj[lt1 + z.mj + cc * ndm]; }
+            }
+        }
+        if (has_k) {
+            const long l1 = IDX(z, i, j, k + sgn);
+#pragma unroll
+            for (int e = 0; e < 5; e++) q.rk[e] = lu_ldc(res + l1 + e * nd);
+#pragma unroll
+            for (int e = 0; e < 4; e++) q.wk[e] = kalias ? lu_ldc(kstate + l1 + (e + 1) * nd) : kstate[l1 + (e + 1) * nd];
+            q.ck = z.coe[l1 + 3 * nd];
+            if (!CART) {
+                const long lt1 = MTR(z, i, j, k + sgn);
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) { q.mk[2 * cc] = z.tk[lt1 + cc * ndm]; q.mk[2 * cc + 1] = z.tk[lt1 + z.mk + cc * ndm]; }
+            }
+        }
+    };
+
+    double pi[5] = {0., 0., 0., 0., 0.};
+    auto body = [&](LuSlot<CART>& q, int s) {
+        const int ii = s - jj - kk;
+        const int rd = (s + 1) & 1, wr = s & 1;
+        if (!(lane_ok && ii >= 0 && ii < ni)) return;
+        const int i = LOWER ? a.i0 + ii : a.i1 - ii;
+        const long l = IDX(z, i, j, k);
+        const double r0 = q.w[0], u0 = q.w[1], v0 = q.w[2], w0 = q.w[3], t0 = q.w[4];
+        const double cf4 = q.cf[4];
+        const double xal = q.cf[0] * signe;
+        double x[5], b[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) x[e] = LOWER ? q.in[e] : q.in[e] * cf4;
+#pragma unroll
+        for (int qq = 0; qq < 3; qq++) {
+            const int D = (LOWER && CART) ? (qq == 0 ? 1 : qq == 1 ? 2 : 0) : (qq == 0 ? 2 : qq == 1 ? 1 : 0);
+            const int o = D == 0 ? ii : D == 1 ? oj : ok;
+            if (o == 0) continue;
+            if (D == 0) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) b[e] = pi[e];
+            } else if (D == 1) {
+                if (jj == 0) {
+                    const long l1 = IDX(z, i, j + sgn, k);
+                    lu_settle(q.rj, 5, res + l1, nd, err);
+                    double tx = 0., ty = 0., tz = 0.;
+                    if (CART) tx = tcj;
+                    else { tx = 0.5 * (q.mj[1] + q.mj[0]); ty = 0.5 * (q.mj[3] + q.mj[2]); tz = 0.5 * (q.mj[5] + q.mj[4]); }
+                    lu_prod<CART, 1>(c, q.wj[0], q.wj[1], q.wj[2], q.wj[3], q.cj * signe, tx, ty, tz, q.rj, b);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) b[e] = ex[rd][0][e][L - 1];
+                }
+            } else {
+                if (kk == 0) {
+                    const long l1 = IDX(z, i, j, k + sgn);
+                    lu_settle(q.rk, 5, res + l1, nd, err);
+                    if (kalias) lu_settle(q.wk, 4, kstate + l1 + nd, nd, err);
+                    double tx = 0., ty = 0., tz = 0.;
+                    if (CART) tx = tck;
+                    else { tx = 0.5 * (q.mk[1] + q.mk[0]); ty = 0.5 * (q.mk[3] + q.mk[2]); tz = 0.5 * (q.mk[5] + q.mk[4]); }
+                    lu_prod<CART, 2>(c, q.wk[0], q.wk[1], q.wk[2], q.wk[3], q.ck * signe, tx, ty, tz, q.rk, b);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) b[e] = ex[rd][1][e][L - 8];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
+        }
+        const double diag = 1. / cf4;
+#pragma unroll
+        for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; lu_stc(res + l + e * nd, x[e]); }
+        double uk = u0, vk = v0, wk = w0, tk = t0;
+        if (!LOWER && a.newton) {
+            double* O = a.out;
+            const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
+            const double r1 = r0 + x[0];
+            const double r_1 = 1. / r1;
+            const double u1 = (r0 * u0 + x[1]) * r_1, v1 = (r0 * v0 + x[2]) * r_1, w1 = (r0 * w0 + x[3]) * r_1;
+            const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
+            const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+            if (kalias) {
+                lu_stc(O + l, r1); lu_stc(O + l + nd, u1); lu_stc(O + l + 2 * nd, v1);
+                lu_stc(O + l + 3 * nd, w1); lu_stc(O + l + 4 * nd, t1);
+                uk = u1; vk = v1; wk = w1; tk = t1;
+            } else {
+                O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
+            }
+        }
+        double tx = 0., ty = 0., tz = 0.;
+        if (CART) tx = tci;
+        else { tx = 0.5 * (q.mo[1] + q.mo[0]); ty = 0.5 * (q.mo[3] + q.mo[2]); tz = 0.5 * (q.mo[5] + q.mo[4]); }
+        lu_prod<CART, 0>(c, u0, v0, w0, t0, q.cf[1] * signe, tx, ty, tz, x, pi);
+        if (CART) tx = tcj;
+        else { tx = 0.5 * (q.mo[7] + q.mo[6]); ty = 0.5 * (q.mo[9] + q.mo[8]); tz = 0.5 * (q.mo[11] + q.mo[10]); }
+        lu_prod<CART, 1>(c, u0, v0, w0, t0, q.cf[2] * signe, tx, ty, tz, x, b);
+#pragma unroll
+        for (int e = 0; e < 5; e++) ex[wr][0][e][L] = b[e];
+        if (CART) tx = tck;
+        else { tx = 0.5 * (q.mo[13] + q.mo[12]); ty = 0.5 * (q.mo[15] + q.mo[14]); tz = 0.5 * (q.mo[17] + q.mo[16]); }
+        lu_prod<CART, 2>(c, uk, vk, wk, tk, q.cf[3] * signe, tx, ty, tz, x, b);
+#pragma unroll
+        for (int e = 0; e < 5; e++) ex[wr][1][e][L] = b[e];
+    };
+
+    const int nsteps = ni + 14;
+    LuSlot<CART> sl[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) fetch(sl[q], q);
+    for (int s0 = 0; s0 < nsteps; s0 += P) {
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            const int s = s0 + q;
+            if (s < nsteps) {
+                body(sl[q], s);
+                fetch(sl[q], s + P);
+            }
+            // the next step reads what the neighbouring lanes of this wave wrote to LDS
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+    }
+}
+
+// fill the 5 variables of a box with a 64-bit pattern (also zeroes the tile counter)
+__global__ void k_fill_box(KZ z, double* __restrict__ p, Box b, long long bits, int* sync) {
+    int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = b.k0 + blockIdx.y;
+    if (sync && t == 0 && blockIdx.y == 0) sync[0] = 0;
+    if (t >= ni * nj) return;
+    int i = b.i0 + t % ni, j = b.j0 + t / ni;
+    long nd = z.nd, l = IDX(z, i, j, k);
+    double v = __longlong_as_double(bits);
+#pragma unroll
+    for (int e = 0; e < 5; e++) p[l + e * nd] = v;
+}
+
 // Copy the Newton-updated LU box back (in-place sub-iterations).
 __global__ void k_copy_box(KZ z, const double* __restrict__ src, double* __restrict__ dst, Box b) {
     int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
@@ -331,10 +592,48 @@ void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
 }
 }  // namespace
 
+namespace {
+void fill_box(const KZ& z, double* p, const LuArgs& a, long long bits, int* sync, hipStream_t st) {
+    Box b = {a.i0, a.i1, a.j0, a.j1, a.k0, a.k1};
+    int n0 = b.i1 - b.i0 + 1, n1 = b.j1 - b.j0 + 1, n2 = b.k1 - b.k0 + 1;
+    hipLaunchKernelGGL(k_fill_box, dim3((n0 * n1 + 255) / 256, n2), dim3(256), 0, st, z, p, b, bits, sync);
+}
+
+template <bool CART>
+void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
+    int nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    // lower: RHS (drodm) -> dW* (dws)
+    fill_box(z, a.dws, a, kLuPending, a.sync, st);
+    hipLaunchKernelGGL((k_lu_pipe<CART, true, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dw, a.dws, ntj, ntk);
+    // upper: dW* -> dW (drodm), Newton update into out
+    fill_box(z, a.dw, a, kLuPending, a.sync, st);
+    if (a.rop_k != a.rop) fill_box(z, a.out, a, kLuPending, nullptr, st);
+    hipLaunchKernelGGL((k_lu_pipe<CART, false, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dws, a.dw, ntj, ntk);
+}
+}  // namespace
+
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
+    if (a.schedule == LU_PIPELINE) {
+        if (z.typ == 2) pipeline<true>(z, a, st); else pipeline<false>(z, a, st);
+        return;
+    }
     if (z.typ == 2) { sweep<true, true>(z, a, st); sweep<true, false>(z, a, st); }
     else { sweep<false, true>(z, a, st); sweep<false, false>(z, a, st); }
+}
+
+// error word of the pipelined sweeps (a bounded wait gave up); clears it
+int lu_pipeline_failed(const LuArgs& a, hipStream_t st) {
+    if (a.schedule != LU_PIPELINE || !a.sync) return 0;
+    int h = 0;
+    if (hipMemcpyAsync(&h, a.sync + 1, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    if (h) {
+        int zero = 0;
+        (void)hipMemcpy(a.sync + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
+    }
+    return h;
 }
 
 void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st) {

@@ -78,7 +78,13 @@ int fm_sync(fm_ctx* ctx);
 /* Residual of the last verified sub-iteration: out[0..4] L2, out[5..9] Linf per equation,
    already divided by DTC like display_ss_iteration.cpp:140-141. */
 int fm_residual_norms(fm_ctx* ctx, int zone, double* out);
-/* Cell sums over the interior: out[0] = sum 1/2 rho |u|^2 vol, out[1] = sum vol. */
+/* CFL monitor of the last verification step (cpcfl3.for, computePT.cpp:629-645):
+   out[0] max, out[1] min, out[2] mean of the local CFL number over the interior cells. */
+int fm_cfl(fm_ctx* ctx, int zone, double* out);
+/* Cell sums over the interior of a time level (POST/post.for + cptaylor.for):
+   out[0] = sum 1/2 rho |u|^2 vol, out[1] = sum vol, out[2] = sum 1/2 rho |curl u|^2 vol with
+   the 4th-order Green-Gauss velocity gradients.  The reference returns out[2]/out[1] and
+   out[0]/out[1].  Ghost cells must be filled. */
 int fm_enstrophy_tke(fm_ctx* ctx, int zone, int level, double* out);
 
 /* Timing helper for benchmarks: events recorded on the context stream around fm_compute. */
@@ -90,6 +96,20 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    milliseconds per run.  phase: FM_PHASE_COE (viscosity + time step / LU coefficients),
    FM_PHASE_RESIDUAL (fused fluxes + balance + BDF update), FM_PHASE_LU (lower + upper sweeps
    with the Newton update), FM_PHASE_BC (ghost-cell fill).  No counterpart in the reference. */
+/* Context options.  FM_OPT_LU_SCHEDULE selects how the LU-SGS sweeps are scheduled on the
+   GPU (both give results bit-identical to the reference's lexicographic sweeps):
+   FM_LU_PIPELINE (default) persistent 8x8-line tiles that trail their neighbours;
+   FM_LU_BLOCKS one launch per diagonal of 8x8x8 blocks.  No counterpart in the reference. */
+#define FM_OPT_LU_SCHEDULE 1
+#define FM_LU_BLOCKS 0
+#define FM_LU_PIPELINE 1
+/* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_FACES (default) evaluates every face flux once and sums
+   them per cell in the reference order; FM_RES_CELLS evaluates the six faces of every cell. */
+#define FM_OPT_RESIDUAL_SCHEDULE 2
+#define FM_RES_CELLS 0
+#define FM_RES_FACES 1
+int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
+
 #define FM_PHASE_COE 0
 #define FM_PHASE_RESIDUAL 1
 #define FM_PHASE_LU 2

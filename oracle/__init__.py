@@ -55,7 +55,7 @@ class _Backend:
 
     def substep(self, z, nitcfg, nitrun=1, first_it=0, lssiter_verif=0, lexit_lu=0, rdm=None,
                 stop=0):
-        cfl = np.zeros(3)
+        cfl = np.array([0., 1e8, 0.])     # computePT.cpp:521 initial max, min, sum
         f = getattr(self.lib, self.prefix + "substep")
         f.restype = ctypes.c_int
         rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nitcfg), ctypes.c_int(nitrun),
@@ -66,6 +66,15 @@ class _Backend:
         if rc != 0:
             raise RuntimeError("%s substep failed (%d)" % (self.prefix, rc))
         return cfl
+
+    def enstrophy(self, z, rop, grad=None):
+        """(tke sum, enstrophy sum, volume sum) of a state (POST/post.for); the velocity
+        gradients go to ``grad`` (9 x NDIMDX) when given."""
+        out = np.zeros(3)
+        f = getattr(self.lib, self.prefix + "enstrophy")
+        f(_i(z.param_int), _d(z.param_real), _d(rop), _d(z.ti), _d(z.tj), _d(z.tk), _d(z.vol), _d(out),
+          _d(grad))
+        return out
 
     def apply_bc(self, z, rop, nstep=1):
         f = getattr(self.lib, self.prefix + "apply_bc")

@@ -1599,13 +1599,145 @@ int fo_apply_bc(int* p, double* pr, int nstep, double* rop, double* ti, double* 
 
 /* ------------------------------------------------------------------------------------ */
 /* One sub-iteration (gsdr3.cpp:152-444; navier_stokes_struct.for:216-496; lhs.cpp)     */
+/* cpcfl3.for, no mesh motion (LALE = 0): local CFL number dt / min(viscous, convective time)
+   of every interior cell; cfl[0] = max, cfl[1] = min, cfl[2] += sum, in the loop order k, j, i.
+   3dcart uses the single face areas (ddi = 2|ti|), 3dfull the sum of the two face areas.     */
+static void fo_cfl(const int* p, const double* pr, const double* rop, const double* xmut,
+                   const double* ti, const double* tj, const double* tk, const double* vol,
+                   double* cfl)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    long nd = Z->nd, ndm = Z->ndm;
+    double rgp = pr[CVINF] * (pr[GAMMA] - 1.);
+    double gam1 = pr[GAMMA] / pr[PRANDT];
+    double gam2 = pr[GAMMA] * rgp;
+    double dt = pr[DTC];
+    int lamin = p[IFLOW] >= 2, cart = p[ITYPZONE] == 2;
+    double surf_c = 0.;
+    if (cart) {
+        double ddi = 2. * sqrt(ti[0] * ti[0]), ddj = 2. * sqrt(tj[0] * tj[0]), ddk = 2. * sqrt(tk[0] * tk[0]);
+        surf_c = 1. / fmax2(fmax2(ddi, ddj), ddk);
+    }
+    for (long k = 1; k <= p[IJKV + 2]; k++)
+        for (long j = 1; j <= p[IJKV + 1]; j++)
+            for (long i = 1; i <= p[IJKV]; i++) {
+                long l = IDX(Z, i, j, k);
+                double surf, vo;
+                if (cart) { surf = surf_c; vo = vol[0]; }
+                else {
+                    long lt = MTR(Z, i, j, k);
+                    long li = lt + Z->mi, lj = lt + Z->mj, lk = lt + Z->mk;
+                    double ddi = sqrt(ti[li] * ti[li] + ti[li + ndm] * ti[li + ndm] + ti[li + 2 * ndm] * ti[li + 2 * ndm]);
+                    ddi = ddi + sqrt(ti[lt] * ti[lt] + ti[lt + ndm] * ti[lt + ndm] + ti[lt + 2 * ndm] * ti[lt + 2 * ndm]);
+                    double ddj = sqrt(tj[lj] * tj[lj] + tj[lj + ndm] * tj[lj + ndm] + tj[lj + 2 * ndm] * tj[lj + 2 * ndm]);
+                    ddj = ddj + sqrt(tj[lt] * tj[lt] + tj[lt + ndm] * tj[lt + ndm] + tj[lt + 2 * ndm] * tj[lt + 2 * ndm]);
+                    double ddk = sqrt(tk[lk] * tk[lk] + tk[lk + ndm] * tk[lk + ndm] + tk[lk + 2 * ndm] * tk[lk + 2 * ndm]);
+                    ddk = ddk + sqrt(tk[lt] * tk[lt] + tk[lt + ndm] * tk[lt + ndm] + tk[lt + 2 * ndm] * tk[lt + 2 * ndm]);
+                    surf = 1. / fmax2(fmax2(ddi, ddj), ddk);
+                    vo = vol[lt];
+                }
+                double u = rop[l + nd], v = rop[l + 2 * nd], w = rop[l + 3 * nd], r = rop[l];
+                double c = sqrt(gam2 * rop[l + 4 * nd]);
+                double ur2 = u * u + v * v + w * w;
+                double sp = sqrt(ur2);
+                double xinvspc = 1. / (sp + c);
+                double conv = 2. * vo * surf * xinvspc;
+                double cfloc;
+                if (lamin) {
+                    double xinvkt = gam1 / xmut[l];
+                    double vis = 2. * r * vo * vo * surf * surf * xinvkt;
+                    double comp = fmin2(vis, conv);
+                    cfloc = dt / comp;
+                } else cfloc = dt / conv;
+                cfl[0] = fmax2(cfl[0], cfloc);
+                cfl[1] = fmin2(cfl[1], cfloc);
+                cfl[2] = cfl[2] + cfloc;
+            }
+}
+
+/* Kinetic energy and enstrophy sums (POST/post.for -> cp_gradu_3d{cart,full}.for with the
+   4-point face values of grad_{i,j,k}_3d{cart,full}.for, c1 = 7/12, c2 = 1/12, then
+   cptaylor.for).  Per gradient component the face contributions are accumulated in the order
+   of the reference loops: -Fk(k), -Fj(j), -Fi(i), +Fi(i+1), +Fj(j+1), +Fk(k+1) (3dcart: only
+   the face direction of the component).  out[0] = sum 1/2 rho |u|^2 vol, out[1] = sum
+   1/2 rho |curl|^2 vol (Green-Gauss gradients divided by the volume, cp_gradu.for),
+   out[2] = sum vol.  grad (optional, 9 x NDIMDX): the velocity gradients (du/dx, du/dy, ...).  */
+static double face_u(const double* q, long l, long inc, double c1, double c2)
+{
+    return (q[l] + q[l - inc]) * c1 - (q[l + inc] + q[l - 2 * inc]) * c2;
+}
+
+void fo_enstrophy(const int* p, const double* pr, const double* rop, const double* ti,
+                  const double* tj, const double* tk, const double* vol, double* out,
+                  double* grad)
+{
+    (void)pr;
+    Ix zz = mkix(p); Ix* Z = &zz;
+    long nd = Z->nd, ndm = Z->ndm;
+    double c1 = 7. / 12, c2 = 1. / 12;
+    int cart = p[ITYPZONE] == 2;
+    long inc[3] = {1, Z->incj, Z->inck};
+    const double* tm[3] = {ti, tj, tk};
+    long minc[3] = {Z->mi, Z->mj, Z->mk};
+    double tke = 0., enst = 0., cpt = 0.;
+    for (long k = 1; k <= p[IJKV + 2]; k++)
+        for (long j = 1; j <= p[IJKV + 1]; j++)
+            for (long i = 1; i <= p[IJKV]; i++) {
+                long l = IDX(Z, i, j, k);
+                long lt = cart ? 0 : MTR(Z, i, j, k);
+                double g[3][3];      /* g[velocity][x,y,z] */
+                for (int v = 0; v < 3; v++) {
+                    const double* q = rop + (1 + v) * nd;
+                    /* face values: low faces of this cell and high faces (low faces of the next) */
+                    double flo[3], fhi[3];
+                    for (int d = 0; d < 3; d++) {
+                        flo[d] = face_u(q, l, inc[d], c1, c2);
+                        fhi[d] = face_u(q, l + inc[d], inc[d], c1, c2);
+                    }
+                    for (int x = 0; x < 3; x++) {
+                        if (cart) {
+                            double t = tm[x][0];
+                            g[v][x] = (0. - flo[x] * t) + fhi[x] * t;
+                        } else {
+                            double a = 0.;
+                            a = a - flo[2] * tm[2][lt + x * ndm];
+                            a = a - flo[1] * tm[1][lt + x * ndm];
+                            a = a - flo[0] * tm[0][lt + x * ndm];
+                            /* cp_gradu_3dfull.for:129-135: the i-face above the last
+                               cell is added with the metric index of the previous loop
+                               (the face below that cell), as the reference does */
+                            long lhi = (i == p[IJKV]) ? lt : lt + minc[0];
+                            a = a + fhi[0] * tm[0][lhi + x * ndm];
+                            a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
+                            a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                            g[v][x] = a;
+                        }
+                    }
+                }
+                double vo = cart ? vol[0] : vol[lt];
+                /* cp_gradu.for: gradients divided by the cell volume */
+                double volinv = 1. / vo;
+                for (int v = 0; v < 3; v++)
+                    for (int x = 0; x < 3; x++) {
+                        g[v][x] = g[v][x] * volinv;
+                        if (grad) grad[l + (3 * v + x) * nd] = g[v][x];
+                    }
+                cpt = cpt + vo;
+                double rho = rop[l], u1 = rop[l + nd], u2 = rop[l + 2 * nd], u3 = rop[l + 3 * nd];
+                tke = tke + 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3) * vo;
+                double w1 = (g[2][1] - g[1][2]), w2 = (g[0][2] - g[2][0]), w3 = (g[1][0] - g[0][1]);
+                enst = enst + 0.5 * rho * (w1 * w1 + w2 * w2 + w3 * w3) * vo;
+            }
+    out[0] = tke; out[1] = enst; out[2] = cpt;
+}
+
 int fo_substep(int* p, double* pr, int nitcfg, int nitrun, int first_it, int lssiter_verif,
                int lexit_lu, const double* x, const double* y, const double* z, double* ro,
                double* ro_m1, double* ro_p1, double* xmut, double* ti, double* tj, double* tk,
                double* vol, double* drodm, double* coe, double* wig, double* rdm, double* cfl,
                int stop)
 {
-    (void)nitrun; (void)cfl; (void)x; (void)y; (void)z;
+    (void)nitrun; (void)x; (void)y; (void)z;
     int nf = p[NIJK + 3], nfk = p[NIJK + 4];
     int sdm[6] = {1, p[IJKV], 1, p[IJKV + 1], 1, p[IJKV + 2]};
     int ind_coe[6] = {1 - nf, p[IJKV] + nf, 1 - nf, p[IJKV + 1] + nf, 1 - nfk, p[IJKV + 2] + nfk};
@@ -1627,6 +1759,8 @@ int fo_substep(int* p, double* pr, int nitcfg, int nitrun, int first_it, int lss
         } else {
             fo_tstb3_global(p, pr, sdm, coe, vol);
         }
+        /* cptst3.for:114-119 */
+        if (lssiter_verif != 0 && cfl) fo_cfl(p, pr, ro_ssiter, xmut, ti, tj, tk, vol, cfl);
     }
     fo_init_rhs(p, nitcfg, sdm, drodm);
     if (p[KFLUDOM] == 2 && nitcfg == 1) fo_wiggle(p, pr, sdm, ro_ssiter, wig);

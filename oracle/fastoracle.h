@@ -42,6 +42,9 @@ void fo_invist(const int* p, const double* pr, const int* ind, const double* rop
 int fo_lu_sgs(int* p, double* pr, const int* ind_lu, double* rotmp, const double* rop,
               double* drodm, const double* ti, const double* tj, const double* tk,
               const double* coe, int mjrnewton);
+void fo_enstrophy(const int* p, const double* pr, const double* rop, const double* ti,
+                  const double* tj, const double* tk, const double* vol, double* out,
+                  double* grad);
 void fo_residual_blocks(const int* p, int nitcfg, int nssiter, const double* drodm, double* rdm);
 
 #ifdef __cplusplus

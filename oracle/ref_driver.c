@@ -351,6 +351,33 @@ done:
     return rc;
 }
 
+/* POST/post.for for one thread owning the zone (computePT_enstrophy.cpp:170-196):
+   out[0] = tke sum, out[1] = enstrophy sum, out[2] = volume sum (the counter). */
+extern void post_(I* ndo, I* nthread, I* ithread, I* nsocket, I* socket, I* mx_synchro,
+                  I* neq_grad, I* param_int, F* param_real, F* tke, F* enst, F* compteur,
+                  I* ijkv_sdm, I* ind_dm_zone, I* ind_dm_socket, I* ind_dm_omp, I* topo_s,
+                  I* socket_topology, I* lok, F* rop, F* ti, F* tj, F* tk, F* vol, F* grad);
+
+int ref_enstrophy(I* p, F* pr, F* rop, F* ti, F* tj, F* tk, F* vol, F* out, F* grad_out)
+{
+    I nd = 0, nthread = 1, ithread = 1, nsocket = 1, socket = 1, mx_synchro = 1000, neq_grad = 3;
+    I ijkv_sdm[3] = {1, 1, 1};
+    I ind_dm[6] = {1, p[IJKV], 1, p[IJKV + 1], 1, p[IJKV + 2]};
+    I ind_socket[6], ind_omp[6];
+    memcpy(ind_socket, ind_dm, sizeof ind_dm);
+    memcpy(ind_omp, ind_dm, sizeof ind_dm);
+    I topo[3] = {1, 1, 1}, topo_s[3] = {1, 1, 1};
+    I* lok = (I*)calloc(mx_synchro * 4, sizeof(I));
+    F* grad = (F*)calloc((size_t)p[NDIMDX] * neq_grad * 3, sizeof(F));
+    F tke = 0., enst = 0., cpt = 0.;
+    post_(&nd, &nthread, &ithread, &nsocket, &socket, &mx_synchro, &neq_grad, p, pr, &tke, &enst,
+          &cpt, ijkv_sdm, ind_dm, ind_socket, ind_omp, topo_s, topo, lok, rop, ti, tj, tk, vol, grad);
+    out[0] = tke; out[1] = enst; out[2] = cpt;
+    if (grad_out) memcpy(grad_out, grad, sizeof(F) * (size_t)p[NDIMDX] * neq_grad * 3);
+    free(lok); free(grad);
+    return 0;
+}
+
 /* Ghost-cell fill alone (FastS.PyTree._applyBC -> applyBC.cpp -> BCzone(lrhs=0) + corners). */
 int ref_apply_bc(I* p, F* pr, I nstep, F* rop, F* ti, F* tj, F* tk, F* x, F* y, F* z, F* xmut)
 {

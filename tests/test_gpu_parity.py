@@ -25,13 +25,15 @@ def orc():
     return oracle.Oracle()
 
 
-def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True):
+def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True, schedule=None):
     zo = make()
     zg = make()
     orc.init_metric(zo)
     orc.apply_bc(zo, zo.ro)
     zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
     ctx = ctxmod.Context(0)
+    if schedule is not None:
+        ctx.set_option(ctxmod.OPT_LU_SCHEDULE, schedule)
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     # same N-1 / N+1 initialisation as the oracle side
@@ -95,6 +97,24 @@ def test_tgv_cart_modes_two_steps(ctxmod, orc, scheme, mode):
     nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
     mk = lambda: cases.tgv_zone((12, 10, 8), H.numerics(scheme, mode), perturb=0.02)
     zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
+@pytest.mark.parametrize("schedule", ["blocks", "pipeline"])
+@pytest.mark.parametrize("mesh", ["cart", "full", "wall"])
+@pytest.mark.parametrize("mode", ["bdf", "bdf_ss2"])
+def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
+    """Both LU-SGS schedules (one launch per block diagonal; persistent trailing tiles) on
+    zones of several tiles in j and k, in-place sub-iterations included."""
+    sch = ctxmod.LU_BLOCKS if schedule == "blocks" else ctxmod.LU_PIPELINE
+    nsub = 2 if mode == "bdf_ss2" else 1
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((21, 19, 26), H.numerics("roe_kap", mode), perturb=0.02)
+    elif mesh == "full":
+        mk = lambda: H.jittered_box((11, 18, 17), H.numerics("ausmpred", mode))
+    else:
+        mk = lambda: H.jittered_box((10, 17, 20), H.numerics("senseur", mode), bcs=H.WALL_J)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2, schedule=sch)
     assert_same(zo, zg, ghosts=True)
 
 
@@ -189,3 +209,75 @@ def test_compute_loop_matches_oracle(ctxmod, orc):
     ctx.close()
     assert np.array_equal(zo.interior(zo.ro), zg.interior(zg.ro))
     assert np.array_equal(zo.ro, zg.ro)
+
+
+@pytest.mark.parametrize("mesh", ["cart", "full"])
+@pytest.mark.parametrize("flow", ["lamin", "euler"])
+def test_cfl_monitor(ctxmod, orc, mesh, flow):
+    """cpcfl3.for on the GPU: max and min exact, the mean to 1e-12 (the sum is reduced in
+    another order)."""
+    num = H.numerics("ausmpred", "bdf")
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((23, 17, 9), num, perturb=0.05, flow=flow)
+    else:
+        mk = lambda: H.jittered_box((12, 11, 9), num, flow=flow)
+    zo, zg = mk(), mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    cfl = orc.substep(zo, 1, first_it=0, lssiter_verif=1, stop=1)
+    ctx = ctxmod.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    ctx.substep(zid, 1, first_it=0, lssiter_verif=1, stop=1)
+    got = ctx.cfl(zid)
+    ctx.close()
+    n = np.prod(zo.dims)
+    assert got[0] == cfl[0] and got[1] == cfl[1]
+    np.testing.assert_allclose(got[2], cfl[2] / n, rtol=1e-12)
+
+
+@pytest.mark.parametrize("schedule", ["cells", "faces"])
+@pytest.mark.parametrize("mesh", ["cart", "full"])
+@pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
+def test_residual_schedules(ctxmod, orc, schedule, mesh, scheme):
+    """Face balance with six faces per cell and with every face once (63-cell i-chunks,
+    4-row j-chunks, 32-plane k-chunks: these sizes straddle all three chunk boundaries)."""
+    num = H.numerics(scheme, "bdf")
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((70, 9, 35), num, perturb=0.02)
+    else:
+        mk = lambda: H.jittered_box((66, 7, 34), num)
+    zo, zg = mk(), mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    orc.substep(zo, 1, first_it=0, stop=1)
+    ctx = ctxmod.Context(0)
+    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, ctxmod.RES_CELLS if schedule == "cells" else ctxmod.RES_FACES)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+    ctx.substep(zid, 1, first_it=0, stop=1)
+    ctx.pull(zid, zg)
+    ctx.close()
+    assert np.array_equal(zo.interior(zo.drodm), zg.interior(zg.drodm))
+
+
+@pytest.mark.parametrize("mesh", ["cart", "full"])
+def test_enstrophy_tke(ctxmod, orc, mesh):
+    """POST/post.for on the GPU: kinetic energy, enstrophy and volume sums against the oracle
+    (sums reduced in another order: 1e-12)."""
+    num = H.numerics("ausmpred", "bdf")
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((31, 17, 12), num, perturb=0.05)
+    else:
+        mk = lambda: H.jittered_box((19, 13, 11), num)
+    zo, zg = mk(), mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    e = orc.enstrophy(zo, zo.ro)
+    ctx = ctxmod.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    got = ctx.tke(zid, 0)
+    ctx.close()
+    np.testing.assert_allclose([got[0], got[2], got[1]], e, rtol=1e-12)

@@ -119,3 +119,50 @@ def test_classification_vs_reference():
         p_ref, degen = ref.classify(z)
         assert np.array_equal(p_ref[:P.N_PARAM_INT], z.param_int[:P.N_PARAM_INT])
         assert not degen.any()
+
+
+@requires_ref
+@pytest.mark.parametrize("mesh", ["cart", "full"])
+@pytest.mark.parametrize("flow", ["lamin", "euler"])
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "local"])
+def test_cfl_monitor_vs_reference(orc, mesh, flow, mode):
+    """cpcfl3.for: max / min / sum of the local CFL number at the first sub-iteration of a
+    verification step, bit for bit."""
+    ref = oracle.Reference()
+    num = H.numerics("ausmpred", mode)
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((9, 8, 7), num, perturb=0.05, flow=flow)
+    else:
+        mk = lambda: H.jittered_box((8, 7, 6), num, flow=flow)
+    zr, zo = mk(), mk()
+    ref.init_metric(zr); orc.init_metric(zo)
+    for be, z in ((ref, zr), (orc, zo)):
+        be.apply_bc(z, z.ro)
+        z.ro_m1[:] = z.ro; z.ro_p1[:] = z.ro
+    cr = ref.substep(zr, 1, first_it=0, lssiter_verif=1)
+    co = orc.substep(zo, 1, first_it=0, lssiter_verif=1)
+    assert cr[0] > 0 and cr[1] < 1e8
+    assert np.array_equal(cr, co)
+
+
+@requires_ref
+@pytest.mark.parametrize("mesh", ["cart", "full"])
+def test_enstrophy_vs_reference(orc, mesh):
+    """POST/post.for (4th-order Green-Gauss gradient + cptaylor.for sums), bit for bit."""
+    ref = oracle.Reference()
+    num = H.numerics("ausmpred", "bdf")
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((10, 9, 8), num, perturb=0.05)
+    else:
+        mk = lambda: H.jittered_box((9, 8, 7), num)
+    zr, zo = mk(), mk()
+    ref.init_metric(zr); orc.init_metric(zo)
+    ref.apply_bc(zr, zr.ro); orc.apply_bc(zo, zo.ro)
+    gr, go = np.zeros(9 * zr.nd), np.zeros(9 * zo.nd)
+    er = ref.enstrophy(zr, zr.ro, gr)
+    eo = orc.enstrophy(zo, zo.ro, go)
+    assert er[1] > 0 and er[0] > 0
+    assert np.array_equal(zr.interior(gr, 9), zo.interior(go, 9))
+    # the per-cell gradients agree bit for bit; the reference's sums run under !$OMP SIMD
+    # (POST/loopI_begin.for), whose partial sums the sequential restatement need not follow
+    np.testing.assert_allclose(eo, er, rtol=1e-13)

@@ -103,5 +103,8 @@ def test_warmup_compute_matches_oracle():
             assert np.array_equal(zo.interior(got), zo.interior(a)), (z[0], sfx)
         h = Internal.getNodeFromName1(z, "ZoneConvergenceHistory")
         assert int(Internal.getNodeFromName1(h, "LastRecordedIteration")[1][0]) == 1
-    _, tke = FP._computeEnstrophy(t, metrics)
-    assert np.isfinite(tke) and tke > 0
+    for z in Internal.getZones(t):
+        cfl = Internal.getValue(Internal.getNodeFromName(z, "CFL_minmaxmoy"))
+        assert cfl[0] >= cfl[2] >= cfl[1] > 0
+    enst, tke = FP._computeEnstrophy(t, metrics)
+    assert np.isfinite(tke) and tke > 0 and np.isfinite(enst) and enst > 0
