@@ -77,3 +77,67 @@ def channel_zone(ni, nj, nk, numerics=None, jitter=0.01, seed=0, name="channel",
                         np.full(nd, st["vz"]), np.full(nd, st["T"])])
     zone.ro[:] = w; zone.ro_m1[:] = w; zone.ro_p1[:] = w
     return zone
+
+
+def tiled_box(nx, ny, nz, numerics=None, h=0.125, perturb=0.02, seed=0, flow="lamin",
+              periodic=(True, True, True), bcs_outer="BCExtrapolate"):
+    """A box of nx[a] x ny[b] x nz[c] blocks (Cartesian, spacing h, a power of two so that
+    every block sees exactly the same coordinates differences), with a TGV-like field plus a
+    seeded perturbation defined on the whole box; faces between blocks (and across the box
+    where ``periodic``) are 1-to-1 matches, other outer faces get ``bcs_outer``.
+    Returns (zones, pairs) with pairs for fast_amd.connect.build."""
+    from . import connect
+    num = P.Numerics() if numerics is None else numerics
+    st = tgv_state()
+    N = (sum(nx), sum(ny), sum(nz))
+    blocks, pairs, outer = connect.tiling(nx, ny, nz, periodic)
+    rng = np.random.default_rng(seed)
+    # global cell-centred field, i fastest
+    xc = (np.arange(N[0]) + 0.5) * h
+    yc = (np.arange(N[1]) + 0.5) * h
+    zc = (np.arange(N[2]) + 0.5) * h
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    L = np.array(N) * h
+    sx, sy, sz = 2 * np.pi / L
+    W = np.stack([np.full(X.shape, st["ro"]),
+                  st["u"] * np.sin(sx * X) * np.cos(sy * Y) * np.cos(sz * Z),
+                  -st["u"] * np.cos(sx * X) * np.sin(sy * Y) * np.cos(sz * Z),
+                  np.zeros(X.shape), np.full(X.shape, st["T"])])
+    scale = np.array([st["ro"], st["u"], st["u"], st["u"], st["T"]])[:, None, None, None]
+    W = W + perturb * scale * rng.uniform(-1.0, 1.0, W.shape)
+    ox = np.concatenate([[0], np.cumsum(nx)]); oy = np.concatenate([[0], np.cumsum(ny)])
+    oz = np.concatenate([[0], np.cumsum(nz)])
+    outer_faces = {}
+    for b, f in outer:
+        outer_faces.setdefault(b, []).append(f)
+    zones = []
+    for n, (a, b, c) in enumerate(blocks):
+        dims = (nx[a], ny[b], nz[c])
+        lo = (ox[a] * h, oy[b] * h, oz[c] * h)
+        hi = (lo[0] + dims[0] * h, lo[1] + dims[1] * h, lo[2] + dims[2] * h)
+        x, y, zz = M.cart_vertices(dims, lo, hi)
+        bcs = [(bcs_outer, f, None) for f in outer_faces.get(n, [])]
+        zone = HostZone("blk%d" % n, dims[0], dims[1], dims[2], num, st, x, y, zz, bcs=bcs, flow=flow)
+        nd = zone.nd
+        full = np.zeros((5, dims[2] + 4, dims[1] + 4, dims[0] + 4))
+        full[:, 2:-2, 2:-2, 2:-2] = W[:, ox[a]:ox[a + 1], oy[b]:oy[b + 1], oz[c]:oz[c + 1]].transpose(0, 3, 2, 1)
+        zone.ro[:] = full.reshape(5 * nd)
+        zone.ro_m1[:] = zone.ro
+        zone.ro_p1[:] = zone.ro
+        zones.append(zone)
+    return zones, pairs, W
+
+
+def whole_box(nx, ny, nz, W, numerics=None, h=0.125, flow="lamin", periodic=True):
+    """The single-zone counterpart of tiled_box (same field, periodic in every direction)."""
+    num = P.Numerics() if numerics is None else numerics
+    st = tgv_state()
+    N = (sum(nx), sum(ny), sum(nz))
+    x, y, zz = M.cart_vertices(N, (0., 0., 0.), (N[0] * h, N[1] * h, N[2] * h))
+    zone = HostZone("box", N[0], N[1], N[2], num, st, x, y, zz, bcs=ALL_PERIODIC, flow=flow)
+    full = np.zeros((5, N[2] + 4, N[1] + 4, N[0] + 4))
+    full[:, 2:-2, 2:-2, 2:-2] = W.transpose(0, 3, 2, 1)
+    zone.ro[:] = full.reshape(5 * zone.nd)
+    zone.ro_m1[:] = zone.ro
+    zone.ro_p1[:] = zone.ro
+    return zone
