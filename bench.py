@@ -30,8 +30,10 @@ HBM_PEAK_GBS = 8000.0
 #   iteration (320 of the LU-SGS iteration + 80 for the two BDF levels)  = 400
 #   coefficients (read W 40, write mu 8 + coe 40), counted separately     =  88
 BYTES_PER_CELL = {"coe": 88.0, "residual": 120.0, "lu": 240.0, "bc": 0.0, "iteration": 400.0}
-KERNELS = {"coe": "k_coe", "residual": "k_residual", "lu": "k_lu_pipe (lower + upper sweeps)",
-           "bc": "k_bc_face"}
+KERNELS = {"coe": "k_coe", "bc": "k_bc_face",
+           "residual": {"cells": "k_residual", "faces": "k_residual_faces"},
+           "lu": {"blocks": "k_lu_block (lower + upper sweeps, one launch per block diagonal)",
+                  "pipeline": "k_lu_pipe (lower + upper sweeps)"}}
 
 
 def parse():
@@ -41,11 +43,32 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--scheme", default="ausmpred")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phase-reps", type=int, default=5)
-    ap.add_argument("--lu", choices=["pipeline", "blocks"], default="pipeline")
+    ap.add_argument("--lu", choices=["pipeline", "blocks"], default="blocks")
+    ap.add_argument("--residual", choices=["cells", "faces"], default="cells")
+    ap.add_argument("--workload", choices=["tgv256", "channel", "zones64"], default="tgv256",
+                    help="tgv256: the headline line; channel: 257x129x33 curvilinear channel with a "
+                         "bump (3dfull, local time step, CFL 10); zones64: 64 abutting blocks "
+                         "(4x4x4 tiling, sizes from default_rng(1) in [16, 96]) joined by transfers")
+    ap.add_argument("--zone-batch", type=int, default=1, choices=[0, 1],
+                    help="zones64: 1 batches the LU-SGS of all zones per block diagonal, 0 loops")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB MiB device buffer 5 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of the solver kernels."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": mib * (1 << 20), "copies": 5}))
 
 
 def make_zone(n, scheme):
@@ -58,7 +81,8 @@ def make_zone(n, scheme):
 
 
 def traffic_from_profiles(phase):
-    """HBM bytes per launch of a phase's kernels from the committed rocprofv3 PMC summary."""
+    """HBM bytes per launch of a phase's kernels ("lu/blocks", "residual/cells", ...) from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_tgv256.json")
     if not os.path.exists(path):
         return None
@@ -97,16 +121,101 @@ def cpu_baseline(n, scheme, iters):
                       % (iters, n, scheme, platform.processor() or platform.machine())}
 
 
+# algorithmic bytes per interior cell for the other workloads (3dfull adds the 80 B of face
+# normals and volume to the residual and to each LU sweep; local time stepping reads no BDF levels)
+BYTES_FULL_LOCAL = {"coe": 168.0, "residual": 160.0, "lu": 400.0, "bc": 0.0, "iteration": 560.0}
+BYTES_CART_LOCAL = {"coe": 88.0, "residual": 80.0, "lu": 240.0, "bc": 0.0, "iteration": 320.0}
+
+
+def zones64_case():
+    """64 abutting blocks: a 4 x 4 x 4 tiling whose per-direction sizes are drawn from
+    default_rng(1) uniformly in [16, 96] (so that neighbouring blocks share whole faces),
+    periodic, laminar AUSM+ o3, implicit LU-SGS with local time stepping at CFL 5."""
+    from fast_amd import cases, connect, params as P
+    d = np.random.default_rng(1).integers(16, 97, size=(3, 4))
+    num = P.Numerics(scheme="ausmpred", slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="local", cfl=5.0)
+    zones, pairs, _ = cases.tiled_box(tuple(int(v) for v in d[0]), tuple(int(v) for v in d[1]),
+                                      tuple(int(v) for v in d[2]), num, flow="lamin")
+    return zones, connect.build(zones, pairs), d
+
+
+def run_other(a):
+    """channel / zones64: the same iteration on another mesh; one JSON line."""
+    from fast_amd import _lib, cases
+    ctx = _lib.Context(0)
+    ctx.set_option(_lib.OPT_ZONE_BATCH, a.zone_batch)
+    if a.workload == "channel":
+        zones, racs = [cases.channel_zone(257, 129, 33)], []
+        desc = ("channel with a bump 257x129x33 (3dfull, jittered), laminar AUSM+ o3, implicit LU-SGS, "
+                "local time step CFL 10, FP64")
+        bpc = BYTES_FULL_LOCAL
+    else:
+        zones, racs, dims = zones64_case()
+        desc = ("64 zones %s x %s x %s (4x4x4 abutting blocks, default_rng(1) in [16, 96]) joined by "
+                "1-to-1 transfers, laminar AUSM+ o3, implicit LU-SGS, local time step CFL 5, FP64"
+                % tuple(list(map(int, r)) for r in dims))
+        bpc = BYTES_CART_LOCAL
+    ids = [ctx.add_zone(z) for z in zones]
+    ctx.set_transfers(racs)
+    ctx.transfer(0)
+    for i, z in zip(ids, zones):
+        ctx.apply_bc(i, 0)
+        buf = np.zeros_like(z.ro)
+        ctx.get(i, _lib.FIELD_RO, buf)
+        ctx.set(i, _lib.FIELD_RO_M1, buf)
+        ctx.set(i, _lib.FIELD_RO_P1, buf)
+    ncell = sum(int(np.prod(z.dims)) for z in zones)
+    del zones
+    ctx.compute(0, 1, 1, nit_c=a.warmup, modulo_verif=0, first_it=0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ms_ev = ctx.compute_timed(a.warmup, 1, 1, nit_c=a.steps, modulo_verif=0, first_it=1)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    value = ncell * a.steps / wall / 1e6
+    tke = ctx.tke(ids[0], 0)
+    if not np.isfinite(tke).all():
+        raise RuntimeError("non-finite state after the benchmark")
+    out = {"metric": "Mcell-iter/s NS residual+LU-SGS", "value": round(value, 2), "unit": "Mcell-iter/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": {"workload": desc, "cells": ncell, "zones": len(ids),
+                                           "zone_batch": a.zone_batch},
+           "event_ms_per_step": round(ms_ev / a.steps, 3),
+           "iteration_hbm_frac": round(value * 1e6 * bpc["iteration"] / 1e9 / HBM_PEAK_GBS, 4)}
+    if len(ids) == 1:
+        phases = {}
+        for name, ph in (("coe", _lib.PHASE_COE), ("residual", _lib.PHASE_RESIDUAL),
+                         ("lu", _lib.PHASE_LU), ("bc", _lib.PHASE_BC)):
+            phases[name] = ctx.time_phase(ids[0], ph, reps=a.phase_reps)
+        out["phase_ms"] = {k: round(v, 4) for k, v in phases.items()}
+        ach = bpc["lu"] * ncell / (phases["lu"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "k_lu_block (lower + upper sweeps)", "achieved": round(ach, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                           "traffic": None, "bytes_per_cell": bpc["lu"]}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
+    if a.workload != "tgv256":
+        run_other(a)
+        return
     from fast_amd import _lib
     z = make_zone(a.n, a.scheme)
     ctx = _lib.Context(0)
     ctx.set_option(_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE if a.lu == "pipeline" else _lib.LU_BLOCKS)
+    ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
+                   _lib.RES_FACES if a.residual == "faces" else _lib.RES_CELLS)
     zid = ctx.add_zone(z)
     ctx.apply_bc(zid, 0)
     buf = np.zeros_like(z.ro)
@@ -138,9 +247,10 @@ def main():
     dom = max(("residual", "lu"), key=lambda k: phases[k])
     dom_ms = phases[dom]
     achieved = BYTES_PER_CELL[dom] * ncell / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 1),
+    kname = KERNELS[dom] if isinstance(KERNELS[dom], str) else KERNELS[dom][a.lu if dom == "lu" else a.residual]
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(dom),
+                "traffic": traffic_from_profiles(dom + "/" + (a.lu if dom == "lu" else a.residual)),
                 "bytes_per_cell": BYTES_PER_CELL[dom],
                 "launch_ms": round(dom_ms, 4),
                 "phase_gbs": {k: round(BYTES_PER_CELL[k] * ncell / (v * 1e-3) / 1e9, 1)
@@ -152,7 +262,7 @@ def main():
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
            "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
                       % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic",
-                      "lu_schedule": a.lu},
+                      "lu_schedule": a.lu, "residual_schedule": a.residual},
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "event_ms_per_step": round(ms_ev / a.steps, 3),

@@ -7,6 +7,8 @@ Drop-in for the FastS compute path of ``Fast/FastS/FastS/PyTree.py``:
 * ``allocate_metric(t)`` / ``_init_metric(t, metrics, hook)``  PyTree.py:267-335
 * ``display_temporal_criteria(t, metrics, nitrun, format=None, ...)``  PyTree.py:2031-2076
 * ``_applyBC(t, metrics, hook1, nstep, var)``           PyTree.py:1155
+* ``_fillGhostcells(zones, tc, metrics, ...)``           transfers + ``_applyBC``; zones joined by
+  CGNS GridConnectivity1to1 nodes (injection raccords, FastC/Com/setInterpTransfersFast.cpp)
 * ``_computeEnstrophy(t, metrics)`` (kinetic-energy part)  PyTree.py:1878
 * ``_createConvergenceHistory(t, nrecs)``               PyTree.py:2152
 * numerics setters ``_setNum2Base`` / ``_setNum2Zones``  (FastC/PyTree.py:115-137)
@@ -136,6 +138,43 @@ def _fen_to_pointrange(p, idir, fen):
     return rng
 
 
+def _addMatch(z, name, donor, rng_r, rng_d, transform=(1, 2, 3)):
+    """A CGNS GridConnectivity1to1 node (what Cassiopee's connectMatch writes) under
+    ZoneGridConnectivity: receiver window rng_r of z matches window rng_d of zone ``donor``."""
+    zgc = Internal.getNodeFromName1(z, "ZoneGridConnectivity")
+    if zgc is None:
+        zgc = Internal.createChild(z, "ZoneGridConnectivity", "ZoneGridConnectivity_t")
+    gc = Internal.createChild(zgc, name, "GridConnectivity1to1_t", donor)
+    Internal.createChild(gc, "PointRange", "IndexRange_t", np.array(rng_r, dtype=np.int32))
+    Internal.createChild(gc, "PointRangeDonor", "IndexRange_t", np.array(rng_d, dtype=np.int32))
+    Internal.createChild(gc, "Transform", '"int[IndexDimension]"', np.array(transform, dtype=np.int32))
+    return gc
+
+
+def _matches(zones):
+    """Injection raccords of every GridConnectivity1to1 of the zones (list of (tree zone,
+    HostZone)); the donor is found by name."""
+    from . import connect
+    index = {z[0]: n for n, (z, hz) in enumerate(zones)}
+    racs = []
+    for n, (z, hz) in enumerate(zones):
+        zgc = Internal.getNodeFromName1(z, "ZoneGridConnectivity")
+        if zgc is None:
+            continue
+        for gc in Internal.getNodesFromType1(zgc, "GridConnectivity1to1_t"):
+            donor = Internal.getValue(gc)
+            if donor not in index:
+                raise ValueError("GridConnectivity1to1 %s: unknown donor zone %r" % (gc[0], donor))
+            d = index[donor]
+            tr = Internal.getNodeFromName1(gc, "Transform")
+            transform = (1, 2, 3) if tr is None else tuple(int(v) for v in np.asarray(tr[1]).ravel())
+            rng_r = np.asarray(Internal.getNodeFromName1(gc, "PointRange")[1])
+            rng_d = np.asarray(Internal.getNodeFromName1(gc, "PointRangeDonor")[1])
+            rcv, dnr = connect.from_pointranges(hz.param_int, rng_r, zones[d][1].param_int, rng_d, transform)
+            racs.append(connect.Raccord(n, d, rcv, dnr))
+    return racs
+
+
 def _zone_bcs(z, p):
     bcs = []
     zbc = Internal.getNodeFromName1(z, "ZoneBC")
@@ -224,25 +263,40 @@ def _init_metric(t, metrics, hook=None):
 
 def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list_graph=None,
            Padding=None, verbose=0):
-    """Build the solver state: metrics, time levels _M1/_P1, ghost cells.  -> (t, tc, metrics)"""
+    """Build the solver state: metrics, time levels _M1/_P1, ghost cells.  -> (t, tc, metrics)
+
+    Zones are joined by the GridConnectivity1to1 nodes of t (injection transfers of the ghost
+    cells, as the reference's tc of a matching mesh); ``tc`` may instead be a list of
+    fast_amd.connect.Raccord.  A Cassiopee connectivity tree is not accepted (it needs
+    Connector)."""
     from . import _lib
-    if tc is not None:
-        raise NotImplementedError("zone connectivity trees (tc) are not supported yet")
+    if tc is not None and not (isinstance(tc, list) and all(hasattr(r, "rcv") for r in tc)):
+        raise NotImplementedError("Cassiopee connectivity trees are not supported: give the matches as "
+                                  "GridConnectivity1to1 nodes of t or a list of fast_amd.connect.Raccord")
     HOOK.clear()
     metrics = allocate_metric(t, verbose)
     _init_metric(t, metrics, HOOK)
     ctx = HOOK["ctx"]
+    racs = list(tc) if tc is not None else _matches(HOOK["zones"])
+    ctx.set_transfers(racs)
+    HOOK["raccords"] = racs
     first_it = 1
+    has_m1s = []
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
         fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
         _tree_to_level(z, hz, "", hz.ro)
         has_m1 = Internal.getNodeFromName1(fs, "Density_M1") is not None
+        has_m1s.append(has_m1)
         if has_m1:
             _tree_to_level(z, hz, "_M1", hz.ro_m1)
         else:
             first_it = 0       # createPrimVars sets FIRST_IT = 0 when _M1 had to be created
             hz.ro_m1[:] = hz.ro
         ctx.set(zid, _lib.FIELD_RO, hz.ro)
+    # ghost cells: transfers between zones, then the physical boundary conditions
+    ctx.transfer(0)
+    for (z, hz), zid, has_m1 in zip(HOOK["zones"], HOOK["ids"], has_m1s):
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
         ctx.apply_bc(zid, 0)
         ctx.get(zid, _lib.FIELD_RO, hz.ro)
         if not has_m1:
@@ -266,8 +320,6 @@ def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, lay
     """NIT time steps (all sub-iterations) of every zone on the GPU; tree fields updated in place."""
     import time
     from . import _lib
-    if tc is not None:
-        raise NotImplementedError("zone connectivity trees (tc) are not supported yet")
     ctx = HOOK["ctx"]
     base = Internal.getBases(t)[0]
     bdef = _get_define(base)
@@ -295,6 +347,22 @@ def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, lay
     if vtune is not None:
         return t1 - t0, 0.0
     return None
+
+
+def _fillGhostcells(zones, tc, metrics, timelevel_target, vars, nstep, omp_mode=0, hook1=None,
+                    nitmax=1, rk=1, exploc=0, num_passage=1):
+    """Transfers between zones then physical boundary conditions on the N level of every zone
+    (Fast/FastS/FastS/PyTree.py:1162-1230 _fillGhostcells: setInterpTransfers then _applyBC)."""
+    from . import _lib
+    ctx = HOOK["ctx"]
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        _tree_to_level(z, hz, "", hz.ro)
+        ctx.set(zid, _lib.FIELD_RO, hz.ro)
+    ctx.transfer(0)
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        ctx.apply_bc(zid, 0, nstep)
+        ctx.get(zid, _lib.FIELD_RO, hz.ro)
+        _level_to_tree(z, hz, "", hz.ro)
 
 
 def _applyBC(t, metrics, hook1=None, nstep=1, var="Density"):

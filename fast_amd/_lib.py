@@ -17,6 +17,7 @@ FIELD_RO, FIELD_RO_M1, FIELD_RO_P1, FIELD_MUT, FIELD_DRODM, FIELD_COE, FIELD_MET
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_int64)
 
 # exported symbols (checked by the CPU test suite against include/fastmi.h)
 SIGNATURES = {
@@ -40,12 +41,17 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_float)]),
     "fm_ctx_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "fm_cfl": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "fm_transfers_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _lp, _lp, _lp]),
+    "fm_transfer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fm_wall_effort": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                                      _dp, _dp, _dp]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
 OPT_LU_SCHEDULE = 1
 LU_BLOCKS, LU_PIPELINE = 0, 1
 OPT_RESIDUAL_SCHEDULE = 2
 RES_CELLS, RES_FACES = 0, 1
+OPT_ZONE_BATCH = 3
 
 
 class FastMIError(RuntimeError):
@@ -142,6 +148,35 @@ class Context:
 
     def apply_bc(self, zone, level=0, nstep=1):
         check(self.lib.fm_apply_bc(self.h, zone, level, nstep))
+
+    def set_transfers(self, raccords):
+        """Injection raccords: objects with zr, zd, rcv, dnr (fast_amd.connect.Raccord)."""
+        n = len(raccords)
+        no_r = np.array([r.zr for r in raccords], dtype=np.int32)
+        no_d = np.array([r.zd for r in raccords], dtype=np.int32)
+        nb = np.array([len(r.rcv) for r in raccords], dtype=np.int64)
+        rcv = np.ascontiguousarray(np.concatenate([r.rcv for r in raccords]) if n else np.zeros(1), dtype=np.int64)
+        dnr = np.ascontiguousarray(np.concatenate([r.dnr for r in raccords]) if n else np.zeros(1), dtype=np.int64)
+        keep = (no_r, no_d, nb, rcv, dnr)
+        check(self.lib.fm_transfers_set(self.h, n, no_r.ctypes.data_as(_ip), no_d.ctypes.data_as(_ip),
+                                        nb.ctypes.data_as(_lp), rcv.ctypes.data_as(_lp),
+                                        dnr.ctypes.data_as(_lp)))
+        del keep
+
+    def wall_effort(self, zone, idir, window, pos=(0., 0., 0.), level=0, fields=True):
+        """(12 sums, 21 x nfaces fields) of a face window (bceffort.for)."""
+        w = np.ascontiguousarray(window, dtype=np.int32)
+        nf = int((w[1] - w[0] + 1) * (w[3] - w[2] + 1) * (w[5] - w[4] + 1))
+        eff = np.zeros(12)
+        fl = np.zeros(21 * nf) if fields else None
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        check(self.lib.fm_wall_effort(self.h, zone, level, idir, w.ctypes.data_as(_ip),
+                                      p.ctypes.data_as(_dp), eff.ctypes.data_as(_dp),
+                                      fl.ctypes.data_as(_dp) if fields else None))
+        return (eff, fl.reshape(21, nf)) if fields else eff
+
+    def transfer(self, level=0):
+        check(self.lib.fm_transfer(self.h, level))
 
     def compute(self, nitrun, nstep_deb, nstep_fin, nit_c=1, modulo_verif=0, first_it=1):
         check(self.lib.fm_compute(self.h, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it))

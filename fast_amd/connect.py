@@ -33,11 +33,12 @@ def _lin(p, i, j, k):
     return (i + nf - 1) + (j + nf - 1) * ni + (k + nfk - 1) * ni * nj
 
 
-def match(pr, face_r, pd, face_d, offset=(0, 0)):
+def match(pr, face_r, pd, face_d, offset=(0, 0), window=None):
     """Injection raccord filling the ghost slab of face ``face_r`` of the receiver (descriptor
     ``pr``) from the donor (``pd``) whose face ``face_d`` touches it.  Axes are aligned (CGNS
-    Transform = identity); ``offset`` shifts the two tangential indices from receiver to donor.
-    Returns (rcv, dnr) index arrays."""
+    Transform = identity); ``offset`` shifts the two tangential indices from receiver to donor;
+    ``window`` restricts the two tangential ranges of the receiver ((lo, hi) 1-based cells,
+    default the whole face).  Returns (rcv, dnr) index arrays."""
     dr, dd = FACES.index(face_r) // 2, FACES.index(face_d) // 2
     if dr != dd or (FACES.index(face_r) % 2) == (FACES.index(face_d) % 2):
         raise NotImplementedError("only opposite faces of aligned zones are matched")
@@ -46,6 +47,11 @@ def match(pr, face_r, pd, face_d, offset=(0, 0)):
     ng = int(pr[P.NIJK + 3]) if dr < 2 else int(pr[P.NIJK + 4])
     tang = [a for a in range(3) if a != dr]
     rng = [np.arange(1, nr[a] + 1) for a in range(3)]
+    if window is not None:
+        for t, (lo, hi) in zip(tang, window):
+            if lo < 1 or hi > nr[t] or lo > hi:
+                raise ValueError("receiver window outside the face")
+            rng[t] = np.arange(lo, hi + 1)
     up = FACES.index(face_r) % 2 == 1
     # ghost layers of the receiver, nearest first
     rng[dr] = np.arange(nr[dr] + 1, nr[dr] + 1 + ng) if up else np.arange(0, -ng, -1)
@@ -89,6 +95,23 @@ def tiling(nx, ny, nz, periodic=(False, False, False)):
             else:
                 outer.append((n, face))
     return blocks, rac, outer
+
+
+def from_pointranges(pr, rng_r, pd, rng_d, transform=(1, 2, 3)):
+    """Raccord of a CGNS GridConnectivity1to1 (PointRange / PointRangeDonor in the ghost
+    inclusive vertex numbering of FastS trees, identity Transform): (rcv, dnr)."""
+    if tuple(int(v) for v in transform) != (1, 2, 3):
+        raise NotImplementedError("only GridConnectivity1to1 with Transform [1, 2, 3] is supported")
+    flat = lambda r: [int(r[0][0]), int(r[0][1]), int(r[1][0]), int(r[1][1]), int(r[2][0]), int(r[2][1])]
+    idr, fenr = P.get_range(flat(rng_r), pr)
+    idd, fend = P.get_range(flat(rng_d), pd)
+    if idr == 0 or idd == 0:
+        raise ValueError("GridConnectivity1to1 window is not on a zone face")
+    dr = (idr - 1) // 2
+    tang = [a for a in range(3) if a != dr]
+    window = [(fenr[2 * t], fenr[2 * t + 1]) for t in tang]
+    offset = [fend[2 * t] - fenr[2 * t] for t in tang]
+    return match(pr, FACES[idr - 1], pd, FACES[idd - 1], offset, window)
 
 
 def build(zones, pairs):

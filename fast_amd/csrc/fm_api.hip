@@ -16,6 +16,7 @@ using namespace fm;
 // param_solver.h slots
 enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 27, ILES = 28,
        ITYPCP = 29, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
+       NEQ_VENT = 40,
        NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
        EXPLOC = 54, NSSITER = 58, SLOPE = 64, PT_BC = 70, NB_RELAX = 71, IMPLICITSOLVER = 74,
        LU_MATCH = 75, SENSORTYPE = 85, BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2, BC_NBDATA = 8 };
@@ -62,6 +63,16 @@ struct fm_ctx {
     hipStream_t stream;
     std::vector<Zone*> zones;
     hipEvent_t e0, e1;
+    // injection transfers between zones (all raccords, flattened)
+    TransferPoint* tr_pts;
+    long tr_n;
+    // diagonal-batched LU-SGS over several zones: cached block lists (per set of LU boxes)
+    // and the per-zone argument table, staged through two pinned host buffers
+    struct LuList { bool cart; std::vector<Box> boxes; LuWork* work; std::vector<int> offs; };
+    std::vector<LuList> lu_lists;
+    LuZone* lu_tab; int lu_tab_cap;
+    LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
+    int lu_batch;            // batch the LU of several zones (default on)
 };
 
 extern "C" const char* fm_last_error(void) { return g_err.c_str(); }
@@ -71,8 +82,9 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     HIPCHK(hipSetDevice(device));
     fm_ctx* c = new fm_ctx();
     c->device = device;
-    c->lu_schedule = LU_PIPELINE;
-    c->res_schedule = RES_FACES;
+    c->lu_schedule = LU_BLOCKS;
+    c->res_schedule = RES_CELLS;
+    c->lu_batch = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
     HIPCHK(hipEventCreate(&c->e0));
@@ -97,6 +109,13 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (Zone* z : c->zones) free_zone(z);
+    if (c->tr_pts) (void)hipFree(c->tr_pts);
+    for (auto& l : c->lu_lists) if (l.work) (void)hipFree(l.work);
+    if (c->lu_tab) (void)hipFree(c->lu_tab);
+    for (int q = 0; q < 2; q++) {
+        if (c->lu_host[q]) (void)hipHostFree(c->lu_host[q]);
+        if (c->lu_ev[q]) (void)hipEventDestroy(c->lu_ev[q]);
+    }
     (void)hipEventDestroy(c->e0);
     (void)hipEventDestroy(c->e1);
     (void)hipStreamDestroy(c->stream);
@@ -170,11 +189,6 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
         const int* size_data = p + pt + BC_NBDATA + 1;
         const double* data = nbdata ? pr + p[pt_bcs + 1 + n + nb] : nullptr;
         if (b.idir < 1 || b.idir > 6) { free_zone(z); return fail(FM_E_ARG, "BC direction %d", b.idir); }
-        bool wall = t == 3 || t == 4 || t == 5 || t == 6 || t == 7 || t == 12;
-        if (wall && kf == 5) {
-            free_zone(z);
-            return fail(FM_E_UNSUPPORTED, "Roe flux with wall boundaries (wall-flux correction) not implemented");
-        }
         if (t == 0) b.kind = 0;
         else if (t == 11) b.kind = 2;
         else if ((t == 4 || t == 6) && p[IFLOW] > 1) b.kind = 1;
@@ -396,8 +410,64 @@ static void coins(fm_ctx* c, Zone* z, double* rop, const int* ind) {
     launch_coins(k, rop, L, c->stream);
 }
 
+// Physical boundary conditions then edges and corners of one zone (gsdr3.cpp:436-439, bcs.cpp).
+static void fill_ghosts(fm_ctx* c, Zone* z, double* rop) {
+    int shift[6], sdm[6] = {1, z->k.ni, 1, z->k.nj, 1, z->k.nk};
+    bczone(c, z, 0, rop, shift, nullptr);
+    coins(c, z, rop, sdm);
+}
+
+// gsdr3.cpp:152-173: the level that receives the transfers and boundary conditions
+static double* level_cl(const Zone* z, int nitcfg) {
+    if (nitcfg == 1 || z->k.itypcp <= 1 || nitcfg % 2 == 1) return z->ro[2];
+    return z->ro[0];
+}
+
+// Injection transfers of every raccord on the given field of every zone
+// (transfert_multiblock.cpp:9-14 -> setInterpTransfersFast).
+static int transfer(fm_ctx* c, double* const* fields) {
+    if (c->tr_n == 0) return FM_OK;
+    TransferArgs a;
+    memset(&a, 0, sizeof a);
+    for (size_t n = 0; n < c->zones.size(); n++) { a.f[n] = fields[n]; a.nd[n] = c->zones[n]->k.nd; }
+    launch_transfer(c->tr_pts, c->tr_n, a, c->stream);
+    HIPCHK(hipGetLastError());
+    return FM_OK;
+}
+
+// Roe wall-flux correction windows in BC order (correct_flux.for:52-133): wall types 3 (slip:
+// bflwallslip.for) and 4, 6, 7, 12 (bflwall.for); the window's normal range is the face index.
+static int wall_corrections(const Zone* z, RhsArgs& a) {
+    a.ncorr = 0;
+    a.ck_vent = z->p[NEQ_VENT] != 3 ? 0. : 1.;
+    a.c_ale = 1. * (z->p[LALE] < 1 ? z->p[LALE] : 1);
+    for (const BcRec& b : z->bcs) {
+        int t = b.type;
+        bool slip = t == 3, wall = t == 4 || t == 6 || t == 7 || t == 12;
+        if (!slip && !wall) continue;
+        int w[6];
+        if (!cl_window(z, b.idir, b.fen, w)) continue;
+        switch (b.idir) {
+        case 1: w[0] = 1; w[1] = 1; break;
+        case 2: w[1] = w[0]; break;
+        case 3: w[2] = 1; w[3] = 1; break;
+        case 4: w[3] = w[2]; break;
+        case 5: w[4] = 1; w[5] = 1; break;
+        default: w[5] = w[4]; break;
+        }
+        if (a.ncorr == kMaxWallCorr)
+            return fail(FM_E_UNSUPPORTED, "more than %d Roe wall windows", kMaxWallCorr);
+        WallCorr& c = a.corr[a.ncorr++];
+        c.idir = b.idir; c.slip = slip;
+        c.neq_mtr = b.idir <= 4 ? z->p[NEQ_IJ] : z->p[NEQ_K];
+        for (int q = 0; q < 6; q++) c.c[q] = w[q];
+    }
+    return FM_OK;
+}
+
 // One sub-iteration of one zone (gsdr3.cpp:152-444 with navier_stokes_struct.for and lhs.cpp).
-static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_verif, int lexit_lu, int stop) {
+static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_verif, int lexit_lu, int stop,
+                   LuArgs* defer = nullptr) {
     KZ& k = z->k;
     hipStream_t st = c->stream;
     int implicit = k.itypcp <= 1;
@@ -427,6 +497,10 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
     RhsArgs a;
     memset(&a, 0, sizeof a);
     a.schedule = c->res_schedule;
+    if (k.kflu == 5) {
+        int rc = wall_corrections(z, a);
+        if (rc) return rc;
+    }
     a.rop = ro_ssiter; a.rop_n = ro; a.rop_n1 = ro_m1;
     a.implicit = implicit; a.nitcfg = nitcfg;
     a.zero_init = (implicit || nitcfg == 1);
@@ -466,6 +540,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             } else {
                 la.rop_k = ro_ssiter; la.out = ro_p1;
             }
+            if (defer) { *defer = la; return FM_OK; }
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
             if (la.out == z->ro_scratch) {
@@ -476,11 +551,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
         }
     }
     if (stop == 3) return FM_OK;
-    if (lexit_lu == 0) {
-        int shift[6];
-        bczone(c, z, 0, ro_cl, shift, nullptr);
-        coins(c, z, ro_cl, sdm);
-    }
+    if (lexit_lu == 0) fill_ghosts(c, z, ro_cl);
     HIPCHK(hipGetLastError());
     return FM_OK;
 }
@@ -511,6 +582,86 @@ extern "C" int fm_zone_substep(fm_ctx* c, int zone, int nitcfg, int nitrun, int 
     return check_lu(c);
 }
 
+// compute_effort.cpp:200-258 -> bceffort.for for one window of one zone: the per-face fields
+// (21 x nfaces) and the 12 sums, summed here in the reference's loop order (k, j, i).
+extern "C" int fm_wall_effort(fm_ctx* c, int zone, int level, int idir, const int* window,
+                              const double* pos, double* effort, double* fields) {
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2 || idir < 1 || idir > 6 || !window || !effort)
+        return fail(FM_E_ARG, "fm_wall_effort: bad arguments");
+    const KZ& k = z->k;
+    int d = (idir - 1) / 2;
+    int lo[3] = {1, 1, 1}, hi[3] = {k.ni, k.nj, k.nk};
+    hi[d] = hi[d] + 1;                       // face indices run to n + 1 in the normal direction
+    for (int a = 0; a < 3; a++)
+        if (window[2 * a] < lo[a] || window[2 * a + 1] > hi[a] || window[2 * a] > window[2 * a + 1])
+            return fail(FM_E_ARG, "fm_wall_effort: window outside the zone");
+    if (window[2 * d] != window[2 * d + 1]) return fail(FM_E_ARG, "fm_wall_effort: window is not one face plane");
+    HIPCHK(hipSetDevice(c->device));
+    long nf = (long)(window[1] - window[0] + 1) * (window[3] - window[2] + 1) * (window[5] - window[4] + 1);
+    double* d_out = nullptr;
+    HIPCHK(hipMalloc(&d_out, sizeof(double) * 33 * nf));
+    EffArgs a;
+    a.idir = idir;
+    for (int q = 0; q < 6; q++) a.w[q] = window[q];
+    for (int q = 0; q < 3; q++) a.pos[q] = pos ? pos[q] : 0.;
+    a.rop = z->ro[level]; a.out = d_out;
+    if (launch_effort(k, a, c->stream)) { (void)hipFree(d_out); return fail(FM_E_UNSUPPORTED, "no effort kernel for this scheme"); }
+    std::vector<double> h((size_t)33 * nf);
+    hipError_t e = hipMemcpyAsync(h.data(), d_out, sizeof(double) * 33 * nf, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(FM_E_HIP, "fm_wall_effort: %s", hipGetErrorString(e));
+    for (int q = 0; q < 12; q++) effort[q] = 0.;
+    for (long t = 0; t < nf; t++)
+        for (int q = 0; q < 12; q++) effort[q] = effort[q] + h[(size_t)(21 + q) * nf + t];
+    if (fields) memcpy(fields, h.data(), sizeof(double) * 21 * nf);
+    return FM_OK;
+}
+
+extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int* no_d,
+                                const int64_t* nb_pts, const int64_t* rcv_pts, const int64_t* dnr_pts) {
+    if (!c || nrac < 0 || (nrac > 0 && (!no_r || !no_d || !nb_pts || !rcv_pts || !dnr_pts)))
+        return fail(FM_E_ARG, "fm_transfers_set: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    if ((long)c->zones.size() > kMaxTransferZones && nrac > 0)
+        return fail(FM_E_UNSUPPORTED, "transfers between more than %d zones", kMaxTransferZones);
+    std::vector<TransferPoint> h;
+    long off = 0;
+    for (int r = 0; r < nrac; r++) {
+        Zone* zr = get_zone(c, no_r[r]);
+        Zone* zd = get_zone(c, no_d[r]);
+        if (!zr || !zd) return fail(FM_E_ARG, "raccord %d: bad zone %d <- %d", r, no_r[r], no_d[r]);
+        if (nb_pts[r] < 0) return fail(FM_E_ARG, "raccord %d: %lld points", r, (long long)nb_pts[r]);
+        for (int64_t q = 0; q < nb_pts[r]; q++, off++) {
+            int64_t ir = rcv_pts[off], id = dnr_pts[off];
+            if (ir < 0 || ir >= zr->k.nd || id < 0 || id >= zd->k.nd)
+                return fail(FM_E_ARG, "raccord %d point %lld: index outside the zone", r, (long long)q);
+            h.push_back(TransferPoint{no_r[r], no_d[r], (long)ir, (long)id});
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->tr_pts) { HIPCHK(hipFree(c->tr_pts)); c->tr_pts = nullptr; }
+    c->tr_n = 0;
+    if (!h.empty()) {
+        HIPCHK(hipMalloc(&c->tr_pts, h.size() * sizeof(TransferPoint)));
+        HIPCHK(hipMemcpy(c->tr_pts, h.data(), h.size() * sizeof(TransferPoint), hipMemcpyHostToDevice));
+        c->tr_n = (long)h.size();
+    }
+    return FM_OK;
+}
+
+extern "C" int fm_transfer(fm_ctx* c, int level) {
+    if (!c || level < 0 || level > 2) return fail(FM_E_ARG, "fm_transfer: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<double*> f;
+    for (Zone* z : c->zones) f.push_back(z->ro[level]);
+    int rc = transfer(c, f.data());
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FM_OK;
+}
+
 extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
     (void)nstep;
     Zone* z = get_zone(c, zone);
@@ -527,24 +678,121 @@ extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
 // computePT.cpp:534-679 for the implemented options: nit_c steps; in each step the
 // sub-iterations nstep_deb..nstep_fin of every zone; the last implicit sub-iteration only
 // computes the residual (lexit_lu) and only on verification steps; time levels rotate.
+// LU-SGS of several zones, diagonal-batched: one launch per block diagonal for all the zones of
+// one mesh kind (3dcart / 3dfull), lower sweeps then upper sweeps, then the copy-back of the
+// zones updated in place.  Same blocks and the same per-block work as the per-zone sweeps.
+static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector<LuArgs>& las) {
+    hipStream_t st = c->stream;
+    for (int cart = 0; cart < 2; cart++) {
+        std::vector<Box> boxes;
+        std::vector<LuZone> tab;
+        for (size_t n = 0; n < zs.size(); n++) {
+            if ((zs[n]->k.typ == 2) != (cart == 1)) continue;
+            const LuArgs& a = las[n];
+            boxes.push_back(Box{a.i0, a.i1, a.j0, a.j1, a.k0, a.k1});
+            tab.push_back(LuZone{zs[n]->k, a});
+        }
+        if (tab.empty()) continue;
+        fm_ctx::LuList* L = nullptr;
+        for (auto& l : c->lu_lists) {
+            if (l.cart != (cart == 1) || l.boxes.size() != boxes.size()) continue;
+            bool same = true;
+            for (size_t q = 0; q < boxes.size() && same; q++)
+                same = !memcmp(&l.boxes[q], &boxes[q], sizeof(Box));
+            if (same) { L = &l; break; }
+        }
+        if (!L) {
+            fm_ctx::LuList nl;
+            nl.cart = cart == 1; nl.boxes = boxes; nl.work = nullptr;
+            std::vector<LuWork> w;
+            lu_work_list(boxes, w, nl.offs);
+            if (!w.empty()) {
+                HIPCHK(hipMalloc(&nl.work, w.size() * sizeof(LuWork)));
+                HIPCHK(hipMemcpy(nl.work, w.data(), w.size() * sizeof(LuWork), hipMemcpyHostToDevice));
+            }
+            c->lu_lists.push_back(nl);
+            L = &c->lu_lists.back();
+        }
+        if (!L->work) continue;
+        if ((int)tab.size() > c->lu_tab_cap) {
+            HIPCHK(hipStreamSynchronize(st));
+            if (c->lu_tab) HIPCHK(hipFree(c->lu_tab));
+            for (int q = 0; q < 2; q++) {
+                if (c->lu_host[q]) HIPCHK(hipHostFree(c->lu_host[q]));
+                HIPCHK(hipHostMalloc(&c->lu_host[q], tab.size() * sizeof(LuZone)));
+                if (!c->lu_ev[q]) HIPCHK(hipEventCreateWithFlags(&c->lu_ev[q], hipEventDisableTiming));
+            }
+            HIPCHK(hipMalloc(&c->lu_tab, tab.size() * sizeof(LuZone)));
+            c->lu_tab_cap = (int)tab.size();
+        }
+        // the table goes through a pinned buffer that the previous copy from it has released
+        int q = c->lu_slot; c->lu_slot ^= 1;
+        HIPCHK(hipEventSynchronize(c->lu_ev[q]));
+        memcpy(c->lu_host[q], tab.data(), tab.size() * sizeof(LuZone));
+        HIPCHK(hipMemcpyAsync(c->lu_tab, c->lu_host[q], tab.size() * sizeof(LuZone), hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(c->lu_ev[q], st));
+        launch_lu_list(cart == 1, true, c->lu_tab, L->work, L->offs, st);
+        launch_lu_list(cart == 1, false, c->lu_tab, L->work, L->offs, st);
+        HIPCHK(hipGetLastError());
+    }
+    for (size_t n = 0; n < zs.size(); n++) {
+        const LuArgs& la = las[n];
+        if (la.out == zs[n]->ro_scratch) {
+            Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+            launch_copy_box(zs[n]->k, zs[n]->ro_scratch, zs[n]->ro[2], lb, st);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return FM_OK;
+}
+
+// Within a sub-iteration every zone first gets its right-hand side and LU-SGS update, then
+// the transfers fill the ghost cells shared with other zones, then every zone's physical
+// boundary conditions and corners are applied (gsdr3.cpp:205-444).
 static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
+    const size_t nz = c->zones.size();
+    // several zones: their LU-SGS sweeps share one launch per block diagonal
+    const bool batch = nz > 1 && c->lu_batch && c->lu_schedule == LU_BLOCKS;
+    std::vector<int> lexit(nz);
+    std::vector<double*> cl(nz);
     for (int it = 0; it < nit_c; it++) {
         int nitrun_loc = nitrun * nit_c + it;
         int verif = modulo_verif > 0 && (nitrun_loc % modulo_verif == 0);
         for (int nstep = nstep_deb; nstep <= nstep_fin; nstep++) {
-            for (Zone* z : c->zones) {
+            int any = 0;
+            std::vector<Zone*> bz;
+            std::vector<LuArgs> bl;
+            for (size_t n = 0; n < nz; n++) {
+                Zone* z = c->zones[n];
                 int implicit = z->k.itypcp <= 1;
                 int nssiter = z->k.nssiter;
                 // souszones_list.cpp:44-84: a verification step computes residuals (and the
                 // CFL monitor) at every sub-iteration and skips LU-SGS at the last one
                 int lssiter_verif = verif, lexit_lu = 0;
+                lexit[n] = -1;
+                cl[n] = level_cl(z, nstep);
                 if (implicit && nstep == nssiter) {
                     if (!verif) continue;           // skip (computePT.cpp:581-582)
                     lexit_lu = 1;
                 }
-                int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 0);
+                LuArgs la;
+                la.i0 = 1; la.i1 = 0;
+                int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 3, batch ? &la : nullptr);
+                if (rc) return rc;
+                if (batch && implicit && lexit_lu == 0 && la.i1 >= la.i0) { bz.push_back(z); bl.push_back(la); }
+                lexit[n] = lexit_lu;
+                any |= lexit_lu == 0;
+            }
+            if (!bz.empty()) {
+                int rc = lu_batched(c, bz, bl);
                 if (rc) return rc;
             }
+            if (!any) continue;
+            int rc = transfer(c, cl.data());
+            if (rc) return rc;
+            for (size_t n = 0; n < nz; n++)
+                if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
+            HIPCHK(hipGetLastError());
         }
         first_it = 1;
         for (Zone* z : c->zones) {
@@ -606,6 +854,11 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
         c->res_schedule = value == FM_RES_FACES ? RES_FACES : RES_CELLS;
         return FM_OK;
     }
+    if (option == FM_OPT_ZONE_BATCH) {
+        if (value != 0 && value != 1) return fail(FM_E_ARG, "zone batch %d", value);
+        c->lu_batch = value;
+        return FM_OK;
+    }
     return fail(FM_E_ARG, "unknown option %d", option);
 }
 
@@ -653,6 +906,10 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     RhsArgs a;
     memset(&a, 0, sizeof a);
     a.schedule = c->res_schedule;
+    if (k.kflu == 5) {
+        int rc = wall_corrections(z, a);
+        if (rc) return rc;
+    }
     a.rop = ro; a.rop_n = ro; a.rop_n1 = z->ro[1]; a.implicit = implicit; a.nitcfg = 1;
     a.zero_init = 1; a.coefW = 0.5; a.rk_in = ro; a.rk_out = z->ro_scratch;
     a.c1 = -1.5; a.c2 = 2.; a.c3 = -0.5;

@@ -235,8 +235,11 @@ __device__ __forceinline__ void inviscid_flux(const FluxConst& f, const double* 
 
 // Centred viscous flux on a Cartesian zone (fluvisq_3dcart_{i,j,k}.for)
 template <int D>
+// gv (optional): the face gradients (u, v, w, T) x (x, y, z) the flux was built from, and the
+// face's volinv in gv[12]; components this direction does not form stay as they were
 __device__ __forceinline__ void visc_cart(const FluxConst& f, const KZ& z, const double* __restrict__ rop,
-                                          const double* __restrict__ xmut, long l, double flu[5]) {
+                                          const double* __restrict__ xmut, long l, double flu[5],
+                                          double* gv = nullptr) {
     long nd = z.nd, incj = z.incj, inck = z.inck;
     const double* U = rop + nd; const double* V = rop + 2 * nd;
     const double* W = rop + 3 * nd; const double* T = rop + 4 * nd;
@@ -268,6 +271,10 @@ __device__ __forceinline__ void visc_cart(const FluxConst& f, const KZ& z, const
         fv = -(f2 * tcx) * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -(f3 * tcx) * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - (gTx * tcx) * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[1] = gUy; gv[2] = gUz; gv[3] = gVx; gv[4] = gVy;
+            gv[6] = gWx; gv[8] = gWz; gv[9] = gTx;
+        }
     } else if (D == 1) {
         long il = l - incj, ir = l;
         long l022 = l - incj - inck, l002 = l - inck, l021 = l - incj + inck, l001 = l + inck;
@@ -294,6 +301,10 @@ __device__ __forceinline__ void visc_cart(const FluxConst& f, const KZ& z, const
         fv = -(2. * f4 * tcy) * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -(f5 * tcy) * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - (gTy * tcy) * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[1] = gUy; gv[3] = gVx; gv[4] = gVy; gv[5] = gVz;
+            gv[7] = gWy; gv[8] = gWz; gv[10] = gTy;
+        }
     } else {
         long il = l - inck, ir = l;
         long l200 = l - 1, l202 = l - 1 - inck, l102 = l + 1 - inck, l100 = l + 1;
@@ -320,14 +331,19 @@ __device__ __forceinline__ void visc_cart(const FluxConst& f, const KZ& z, const
         fv = -f5 * tcz * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -2. * f6 * tcz * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - gTz * tcz * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[2] = gUz; gv[4] = gVy; gv[5] = gVz;
+            gv[6] = gWx; gv[7] = gWy; gv[8] = gWz; gv[11] = gTz;
+        }
     }
+    if (gv) gv[12] = f.volinv;
 }
 
 // Centred viscous flux on a curvilinear zone (fluvisq_3dfull_{i,j,k}.for)
 template <int D>
 __device__ __forceinline__ void visc_full(const FluxConst& f, const KZ& z, const double* __restrict__ rop,
                                           const double* __restrict__ xmut, long l, long lt,
-                                          const double tn[3], double flu[5]) {
+                                          const double tn[3], double flu[5], double* gv = nullptr) {
     long nd = z.nd, ndm = z.ndm, incj = z.incj, inck = z.inck, mi = z.mi, mj = z.mj, mk = z.mk;
     const double* A; const double* B; const double* C;
     long il, lvol, a0, a1, b0, b1, b2, b3, c0, c1, c2, c3, n3a, n3b, n4a, n4b, n5a, n5b, n6a, n6b;
@@ -390,6 +406,13 @@ __device__ __forceinline__ void visc_full(const FluxConst& f, const KZ& z, const
     fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
     fv = fv5 * 0.5 - (g[3][0] * tcx + g[3][1] * tcy + g[3][2] * tcz) * xktvol;
     flu[4] = flu[4] + fv;
+    if (gv) {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+            for (int m = 0; m < 3; m++) gv[3 * e + m] = g[e][m];
+        gv[12] = volinv;
+    }
 }
 
 // Full face flux through the face (l - inc_D, l) of cell (i,j,k)

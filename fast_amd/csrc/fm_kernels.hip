@@ -248,6 +248,86 @@ __device__ __forceinline__ void res_update(const KZ& z, const RhsArgs& a, const 
     }
 }
 
+// shape_tab_mtr.for: which metric components a face BC reads
+__device__ __forceinline__ void shape_tab(int neq_mtr, int idir, int& ic, int& jc, int& kc,
+                                          double& cim, double& cjm, double& ckm) {
+    if (neq_mtr == 3) { ic = 1; jc = 2; kc = 3; cim = 1.; cjm = 1.; ckm = 1.; }
+    else if (neq_mtr == 2) { ic = 1; jc = 2; kc = 2; cim = 1.; cjm = 1.; ckm = 0.; }
+    else if (neq_mtr == 1) {
+        ic = 1; jc = 1; kc = 1;
+        cim = idir <= 2 ? 1. : 0.; cjm = (idir == 3 || idir == 4) ? 1. : 0.; ckm = idir >= 5 ? 1. : 0.;
+    } else { ic = 1; jc = 1; kc = 1; cim = 0.; cjm = 0.; ckm = 0.; }
+}
+
+// ---------------------------------------------------------------------------------------
+// correct_flux.for:44-260 for Roe (lapply = 4), applied to the balance of a cell next to a wall
+// face, BC by BC: the Euler flux of the face is taken back out (template_correction.for,
+// assemble_drodm_corrflu.for: df = flu*sens + flux(li)*c28 with c28 = 0), then the wall flux
+// is put in (bflwall.for, bflwallslip.for slip walls; assemble_drodm_corr.for).
+template <int SLOPE, bool CART>
+__device__ void wall_correction(const FluxConst& f, const KZ& z, const double* __restrict__ rop,
+                                const RhsArgs& a, int i, int j, int k, double acc[5]) {
+    for (int n = 0; n < a.ncorr; n++) {
+        const WallCorr& w = a.corr[n];
+        const int d = (w.idir - 1) / 2;
+        const bool even = (w.idir % 2) == 0;
+        int c[3] = {i, j, k};
+        const int fc = w.c[2 * d];
+        if (c[d] != (even ? fc - 1 : fc)) continue;
+        bool in = true;
+        for (int t = 0; t < 3; t++)
+            if (t != d) in = in && c[t] >= w.c[2 * t] && c[t] <= w.c[2 * t + 1];
+        if (!in) continue;
+        c[d] = fc;
+        double flu[5];
+        if (d == 0) face_flux<5, SLOPE, false, CART, 0>(f, z, rop, c[0], c[1], c[2], flu);
+        else if (d == 1) face_flux<5, SLOPE, false, CART, 1>(f, z, rop, c[0], c[1], c[2], flu);
+        else face_flux<5, SLOPE, false, CART, 2>(f, z, rop, c[0], c[1], c[2], flu);
+        const double sens = even ? 1. : -1.;
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + (flu[e] * sens + 0.);
+        // wall flux from the state averaged across the face
+        const long nd = z.nd, ndm = z.ndm;
+        const long inc = d == 0 ? 1 : d == 1 ? z.incj : z.inck;
+        const long l = IDX(z, c[0], c[1], c[2]), iadrf = l - inc;
+        const long lt = MTR(z, c[0], c[1], c[2]);
+        const double* tijk = d == 0 ? z.ti : d == 1 ? z.tj : z.tk;
+        int ic, jc, kc;
+        double cim, cjm, ckm;
+        shape_tab(w.neq_mtr, w.idir, ic, jc, kc, cim, cjm, ckm);
+        const double tcx = tijk[lt + ndm * (ic - 1)] * cim;
+        const double tcy = tijk[lt + ndm * (jc - 1)] * cjm;
+        const double tcz = tijk[lt + ndm * (kc - 1)] * ckm;
+        const double qen = (0. * tcx + 0. * tcy + 0. * tcz * a.ck_vent) * a.c_ale;
+        const double rgp = z.cv * (z.gam - 1.);
+        const double r = 0.5 * (rop[l] + rop[iadrf]);
+        const double u = 0.5 * (rop[l + nd] + rop[iadrf + nd]);
+        const double v = 0.5 * (rop[l + 2 * nd] + rop[iadrf + 2 * nd]);
+        const double ww = 0.5 * (rop[l + 3 * nd] + rop[iadrf + 3 * nd]);
+        const double u_int = tcx * u + tcy * v + tcz * ww - qen;
+        const double pw = 0.5 * (rop[l + 4 * nd] * rop[l] + rop[iadrf + 4 * nd] * rop[iadrf]) * rgp;
+        if (!w.slip) {
+            flu[0] = 0.;
+            flu[1] = tcx * pw + u_int * u * r;
+            flu[2] = tcy * pw + u_int * v * r;
+            flu[3] = tcz * pw + u_int * ww * r;
+            flu[4] = pw * qen;
+        } else {
+            const double gam2 = z.gam / (z.gam - 1.);
+            const double pi = rop[iadrf + 4 * nd] * rop[iadrf] * rgp;
+            const double h = gam2 * pi + r * (u * u + v * v + ww * ww) * 0.5;
+            flu[0] = u_int * r;
+            flu[1] = tcx * pw + u_int * r * u;
+            flu[2] = tcy * pw + u_int * r * v;
+            flu[3] = tcz * pw + u_int * r * ww;
+            flu[4] = u_int * h + pw * qen;
+        }
+        const double sw = even ? -1. : 1.;
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + flu[e] * sw;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Residual: fused init_rhs + 6 face fluxes + balance + update, one thread per interior cell.
 // Accumulation order per cell as the reference sweep (template_FluxAndBalance.for:192-271):
@@ -289,6 +369,7 @@ __global__ void __launch_bounds__(256) k_residual(KZ z, RhsArgs a) {
     face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k + 1, flu);
 #pragma unroll
     for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+    if (KFLU == 5 && a.ncorr > 0) wall_correction<SLOPE, CART>(f, z, rop, a, i, j, k, acc);
 
     res_update(z, a, rop, l, acc);
 }
@@ -365,17 +446,6 @@ __global__ void __launch_bounds__(320) k_residual_faces(KZ z, RhsArgs a) {
 // ---------------------------------------------------------------------------------------
 // Ghost cells.  Face BC over a window (BCExtrapolate.for, BCPeriodic.for,
 // BCWallViscous_{i,j,k}.for), then correct_coins.for edges and corners.
-// shape_tab_mtr.for: which metric components a face BC reads
-__device__ __forceinline__ void shape_tab(int neq_mtr, int idir, int& ic, int& jc, int& kc,
-                                          double& cim, double& cjm, double& ckm) {
-    if (neq_mtr == 3) { ic = 1; jc = 2; kc = 3; cim = 1.; cjm = 1.; ckm = 1.; }
-    else if (neq_mtr == 2) { ic = 1; jc = 2; kc = 2; cim = 1.; cjm = 1.; ckm = 0.; }
-    else if (neq_mtr == 1) {
-        ic = 1; jc = 1; kc = 1;
-        cim = idir <= 2 ? 1. : 0.; cjm = (idir == 3 || idir == 4) ? 1. : 0.; ckm = idir >= 5 ? 1. : 0.;
-    } else { ic = 1; jc = 1; kc = 1; cim = 0.; cjm = 0.; ckm = 0.; }
-}
-
 // metric index of the boundary face of the ghost column through (c0, c1, c2)
 __device__ __forceinline__ long bc_face_mtr(const KZ& z, const BcArgs& a, int c0, int c1, int c2) {
     int d = (a.idir - 1) / 2;
@@ -881,7 +951,7 @@ static void res_faces(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st)
 
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st) {
     bool lamin = z.iflow == 2;
-    if (a.schedule == RES_FACES) {
+    if (a.schedule == RES_FACES && a.ncorr == 0) {
         if (z.kflu == 5) {
             if (z.slope == 1) res_faces<5, 1>(z, a, lamin, st);
             else if (z.slope == 2) res_faces<5, 2>(z, a, lamin, st);
@@ -938,6 +1008,143 @@ void launch_cfl(const KZ& z, const double* rop, double* part, int nblk, double* 
 
 void launch_tke(const KZ& z, const double* ro, double* part, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_post_partial, dim3(nblk), dim3(256), 0, st, z, ro, part);
+}
+
+}  // namespace fm
+
+// ---------------------------------------------------------------------------------------
+// Injection transfers.  Receivers are ghost cells and donors interior cells, so one launch
+// moves every raccord of every zone pair at once: one thread per point, 5 variables each.
+namespace fm {
+
+__global__ void k_transfer(const TransferPoint* __restrict__ pts, long npts, TransferArgs a) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    TransferPoint q = pts[t];
+    const double* __restrict__ d = a.f[q.zd];
+    double* __restrict__ r = a.f[q.zr];
+    long ndd = a.nd[q.zd], ndr = a.nd[q.zr];
+#pragma unroll
+    for (int v = 0; v < 5; v++) r[q.rcv + v * ndr] = d[q.dnr + v * ndd];
+}
+
+void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a, hipStream_t st) {
+    if (npts <= 0) return;
+    long nb = (npts + 255) / 256;
+    hipLaunchKernelGGL(k_transfer, dim3((unsigned)nb), dim3(256), 0, st, pts, npts, a);
+}
+
+}  // namespace fm
+
+// ---------------------------------------------------------------------------------------
+// Wall efforts, eff_flu<scheme>_<model>_<slope>_<mesh> of template_effort.for:100-440: the
+// balance's own face flux on every face of a window, minus the free-stream pressure, scaled by
+// 2 / (rho_inf V_inf^2) and signed outward; with the face gradients, pressure coefficient,
+// viscosity, density and pressure.  One thread per face; the host sums the 12 contributions
+// in the reference's loop order (k, j, i).
+namespace fm {
+
+template <int KFLU, int SLOPE, bool LAMIN, bool CART, int D>
+__global__ void k_effort(KZ z, EffArgs a) {
+    const int n0 = a.w[1] - a.w[0] + 1, n1 = a.w[3] - a.w[2] + 1, n2 = a.w[5] - a.w[4] + 1;
+    const long nf = (long)n0 * n1 * n2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const int i = a.w[0] + (int)(t % n0), j = a.w[2] + (int)((t / n0) % n1), k = a.w[4] + (int)(t / ((long)n0 * n1));
+    const double* __restrict__ rop = a.rop;
+    FluxConst f = flux_const(z);
+    const long l = IDX(z, i, j, k), nd = z.nd, ndm = z.ndm;
+    const long inc = D == 0 ? 1 : D == 1 ? z.incj : z.inck;
+    double qm[5], qp[5], tn[3], sn, fl[5], gv[13];
+#pragma unroll
+    for (int q = 0; q < 12; q++) gv[q] = 0.;
+    gv[12] = f.volinv;
+    long lt = 0;
+    if (CART) {
+        tn[0] = f.tc[0]; tn[1] = f.tc[1]; tn[2] = f.tc[2]; sn = f.s[D];
+    } else {
+        const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
+        lt = MTR(z, i, j, k);
+        tn[0] = tm[lt]; tn[1] = tm[lt + ndm]; tn[2] = tm[lt + 2 * ndm];
+        sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+    }
+    slope_state<SLOPE>(f, rop, nd, l, inc, qm, qp);
+    inviscid_flux<KFLU, CART, D>(f, rop, z.wig, nd, l, tn, sn, qm, qp, fl);
+    if (LAMIN) {
+        if (CART) visc_cart<D>(f, z, rop, z.xmut, l, fl, gv);
+        else visc_full<D>(f, z, rop, z.xmut, l, lt, tn, fl, gv);
+    }
+    const double pinf = z.pinf;
+    const double norm = 2. / (z.roinf * (z.vinf * z.vinf));
+    const bool odd = (a.idir % 2) == 1;
+    const double sens = odd ? -norm : norm, sens1 = odd ? -1. : 1.;
+    const double flag = CART ? 0. : 1.;
+    const double p1 = qp[0] * qp[4] * f.rgp, p2 = qm[0] * qm[4] * f.rgp;
+    double v[21];
+    v[0] = fl[0] * sens1;
+    v[1] = (fl[1] - pinf * tn[0] * (D == 0 ? 1. : flag)) * sens;
+    v[2] = (fl[2] - pinf * tn[1] * (D == 1 ? 1. : flag)) * sens;
+    v[3] = (fl[3] - pinf * tn[2] * (D == 2 ? 1. : flag)) * sens;
+    if (D == 0) v[1] = (fl[1] - pinf * tn[0]) * sens;
+    if (D == 1) v[2] = (fl[2] - pinf * tn[1]) * sens;
+    if (D == 2) v[3] = (fl[3] - pinf * tn[2]) * sens;
+    v[4] = fl[4] * sens;
+#pragma unroll
+    for (int q = 0; q < 12; q++) v[5 + q] = gv[q] * 2. * gv[12];
+    v[17] = (0.5 * (p1 + p2) - pinf) * norm;
+    v[18] = LAMIN ? 0.5 * (z.xmut[l] + z.xmut[l - inc]) : 0.;
+    // Roe: etat_roe_GD.for leaves the square roots of the densities in qm1, qp1
+    v[19] = KFLU == 5 ? 0.5 * (sqrt(qm[0]) + sqrt(qp[0])) : 0.5 * (qm[0] + qp[0]);
+    v[20] = 0.5 * (p1 + p2);
+    long x1, x2;
+    if (D == 0) { x1 = z.xi; x2 = (long)z.xi * z.xj; }
+    else if (D == 1) { x1 = 1; x2 = (long)z.xi * z.xj; }
+    else { x1 = 1; x2 = z.xi; }
+    const long x3 = x1 + x2, lx = XYZ(z, i, j, k);
+    const double f4 = 0.25 * (z.x[lx] + z.x[lx + x1] + z.x[lx + x2] + z.x[lx + x3]);
+    const double f5 = 0.25 * (z.y[lx] + z.y[lx + x1] + z.y[lx + x2] + z.y[lx + x3]);
+    const double f6 = 0.25 * (z.z[lx] + z.z[lx + x1] + z.z[lx + x2] + z.z[lx + x3]);
+    double* o = a.out;
+#pragma unroll
+    for (int q = 0; q < 21; q++) o[t + q * nf] = v[q];
+    o[t + 21 * nf] = v[1]; o[t + 22 * nf] = v[2]; o[t + 23 * nf] = v[3];
+    o[t + 24 * nf] = v[3] * (f5 - a.pos[1]) - v[2] * (f6 - a.pos[2]);
+    o[t + 25 * nf] = v[1] * (f6 - a.pos[2]) - v[3] * (f4 - a.pos[0]);
+    o[t + 26 * nf] = v[2] * (f4 - a.pos[0]) - v[1] * (f5 - a.pos[1]);
+    o[t + 27 * nf] = sn;
+    o[t + 28 * nf] = v[0];
+#pragma unroll
+    for (int e = 1; e < 5; e++) o[t + (28 + e) * nf] = fl[e] * sens1;
+}
+
+template <int KFLU, int SLOPE, bool LAMIN, bool CART>
+static void effort_d(const KZ& z, const EffArgs& a, dim3 g, hipStream_t st) {
+    int d = (a.idir - 1) / 2;
+    if (d == 0) hipLaunchKernelGGL((k_effort<KFLU, SLOPE, LAMIN, CART, 0>), g, dim3(256), 0, st, z, a);
+    else if (d == 1) hipLaunchKernelGGL((k_effort<KFLU, SLOPE, LAMIN, CART, 1>), g, dim3(256), 0, st, z, a);
+    else hipLaunchKernelGGL((k_effort<KFLU, SLOPE, LAMIN, CART, 2>), g, dim3(256), 0, st, z, a);
+}
+
+template <int KFLU, int SLOPE>
+static void effort_m(const KZ& z, const EffArgs& a, dim3 g, hipStream_t st) {
+    bool lamin = z.iflow == 2;
+    if (lamin) { if (z.typ == 2) effort_d<KFLU, SLOPE, true, true>(z, a, g, st); else effort_d<KFLU, SLOPE, true, false>(z, a, g, st); }
+    else { if (z.typ == 2) effort_d<KFLU, SLOPE, false, true>(z, a, g, st); else effort_d<KFLU, SLOPE, false, false>(z, a, g, st); }
+}
+
+int launch_effort(const KZ& z, const EffArgs& a, hipStream_t st) {
+    long nf = (long)(a.w[1] - a.w[0] + 1) * (a.w[3] - a.w[2] + 1) * (a.w[5] - a.w[4] + 1);
+    if (nf <= 0) return 0;
+    dim3 g((unsigned)((nf + 255) / 256));
+    if (z.kflu == 5) {
+        if (z.slope == 1) effort_m<5, 1>(z, a, g, st);
+        else if (z.slope == 2) effort_m<5, 2>(z, a, g, st);
+        else if (z.slope == 3) effort_m<5, 3>(z, a, g, st);
+        else return -1;
+    } else if (z.kflu == 1 && z.slope == 2) effort_m<1, 2>(z, a, g, st);
+    else if (z.kflu == 2 && z.slope == 2) effort_m<2, 2>(z, a, g, st);
+    else return -1;
+    return 0;
 }
 
 }  // namespace fm

@@ -1,11 +1,17 @@
 // Kernel launch interface shared by the kernels and the C ABI host code.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include "fm_device.hpp"
 
 namespace fm {
 
 struct Box { int i0, i1, j0, j1, k0, k1; };
+
+// Roe wall-flux correction window (correct_flux.for, lapply = 4): the face cells of a wall BC
+// (indice_cl_sdm with the normal range set to the face index) and which wall flux to use.
+struct WallCorr { int idir, slip, neq_mtr, c[6]; };
+constexpr int kMaxWallCorr = 12;
 
 struct RhsArgs {
     const double* rop;      // state feeding the RHS (rop_ssiter)
@@ -18,6 +24,9 @@ struct RhsArgs {
     int implicit, nitcfg;
     double c1, c2, c3;      // BDF coefficients
     int schedule;           // RES_FACES (each face once) or RES_CELLS (six faces per cell)
+    int ncorr;              // Roe: wall faces whose Euler flux is replaced, in BC order
+    WallCorr corr[kMaxWallCorr];
+    double ck_vent, c_ale;  // shape_tab_mtr.for mesh-motion factors of the wall flux
 };
 enum { RES_CELLS = 0, RES_FACES = 1 };
 
@@ -50,11 +59,32 @@ struct BcArgs {
     int inc_bc[3];        // outpres: indbci strides of the BC window
 };
 
+// Injection transfers between zones (FastC/Com/setInterpTransfersFast.cpp:640-900, type 0):
+// rcv[zr][rcv_pt + v nd_r] = dnr[zd][dnr_pt + v nd_d] for the 5 variables of every point.
+constexpr int kMaxTransferZones = 128;   // keeps the kernel arguments at about 2 KB
+struct TransferPoint { int zr, zd; long rcv, dnr; };
+struct TransferArgs {
+    double* f[kMaxTransferZones];   // field level of every zone
+    long nd[kMaxTransferZones];     // its NDIMDX
+};
+void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a, hipStream_t st);
+
+// Wall efforts (bceffort.for): per face of a window, 21 fields and the 12 contributions to the
+// sums; out = 33 x nfaces (fields 0-20, then the 12 contributions)
+struct EffArgs { int idir, w[6]; double pos[3]; const double* rop; double* out; };
+int launch_effort(const KZ& z, const EffArgs& a, hipStream_t st);
+
 void launch_metric(const KZ& z, hipStream_t st);
 void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, int lamin, hipStream_t st);
 void launch_wiggle(const KZ& z, const double* rop, hipStream_t st);
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st);
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
+// diagonal-batched LU-SGS (blocks schedule) over several zones of the same mesh kind
+struct LuZone { KZ z; LuArgs a; };
+struct LuWork { int zone, bi, bj, bk; };
+void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs);
+void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
+                    const std::vector<int>& offs, hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);
 void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st);
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);

@@ -19,6 +19,8 @@
 // The off-diagonal products b = B(W_nb) dW_nb are formed by the cell that owns W_nb, exactly
 // as the reference forms them for the cell that consumes them (same operands, same order).
 #include "fm_kernels.hpp"
+#include <array>
+#include <vector>
 
 namespace fm {
 
@@ -147,10 +149,8 @@ struct LuLds {
 //  2. wave 0 runs the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles
 //     cell ii = s - jj - kk at step s; results go straight to global memory.
 template <bool CART, bool LOWER, int CI>
-__global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int bk_lo, int nbi, int nbj) {
-    const int bj = blockIdx.x, bk = bk_lo + blockIdx.y, bi = diag - bj - bk;
-    if (bi < 0 || bi >= nbi || bj >= nbj) return;
-    __shared__ LuLds<CART, CI> S;
+__device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, int bj, int bk,
+                                         LuLds<CART, CI>& S) {
     constexpr int NV = LuLds<CART, CI>::NV;
     const int tid = threadIdx.x;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
@@ -298,6 +298,27 @@ __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int 
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     (void)NV;
+}
+
+template <bool CART, bool LOWER, int CI>
+__global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int bk_lo, int nbi, int nbj) {
+    const int bj = blockIdx.x, bk = bk_lo + blockIdx.y, bi = diag - bj - bk;
+    if (bi < 0 || bi >= nbi || bj >= nbj) return;
+    __shared__ LuLds<CART, CI> S;
+    lu_block<CART, LOWER, CI>(z, a, bi, bj, bk, S);
+}
+
+// Several zones in one launch per block diagonal: block b of the launch takes its zone and its
+// block coordinates from the work list (zones are independent: each keeps its own diagonal
+// order, diagonal d of every zone goes into launch d).
+template <bool CART, bool LOWER, int CI>
+__global__ __launch_bounds__(256) void k_lu_block_list(const LuZone* __restrict__ zones,
+                                                       const LuWork* __restrict__ work) {
+    const LuWork w = work[blockIdx.x];
+    __shared__ LuLds<CART, CI> S;
+    const KZ z = zones[w.zone].z;
+    const LuArgs a = zones[w.zone].a;
+    lu_block<CART, LOWER, CI>(z, a, w.bi, w.bj, w.bk, S);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -612,6 +633,51 @@ void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k_lu_pipe<CART, false, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dws, a.dw, ntj, ntk);
 }
 }  // namespace
+
+// Block lists of a set of zones for the diagonal-batched sweeps: for every zone (LU box of
+// boxes[n]) its 8x8x8 blocks in sweep coordinates, grouped by diagonal; offs[d]..offs[d+1]
+// are the blocks of launch d.
+void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs) {
+    constexpr int CI = 8;
+    int nd = 0;
+    std::vector<std::array<int, 3>> nb(boxes.size());
+    for (size_t n = 0; n < boxes.size(); n++) {
+        const Box& b = boxes[n];
+        nb[n] = {(b.i1 - b.i0 + 1 + CI - 1) / CI, (b.j1 - b.j0 + 1 + 7) / 8, (b.k1 - b.k0 + 1 + 7) / 8};
+        if (b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0) nb[n] = {0, 0, 0};
+        int d = nb[n][0] + nb[n][1] + nb[n][2] - 2;
+        if (nb[n][0] > 0 && d > nd) nd = d;
+    }
+    work.clear(); offs.assign(1, 0);
+    for (int d = 0; d < nd; d++) {
+        for (size_t n = 0; n < boxes.size(); n++) {
+            if (nb[n][0] == 0) continue;
+            for (int bk = 0; bk < nb[n][2]; bk++)
+                for (int bj = 0; bj < nb[n][1]; bj++) {
+                    int bi = d - bj - bk;
+                    if (bi >= 0 && bi < nb[n][0]) work.push_back(LuWork{(int)n, bi, bj, bk});
+                }
+        }
+        offs.push_back((int)work.size());
+    }
+}
+
+void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
+                    const std::vector<int>& offs, hipStream_t st) {
+    constexpr int CI = 8;
+    for (size_t d = 0; d + 1 < offs.size(); d++) {
+        int n = offs[d + 1] - offs[d];
+        if (n <= 0) continue;
+        const LuWork* w = work + offs[d];
+        if (cart) {
+            if (lower) hipLaunchKernelGGL((k_lu_block_list<true, true, CI>), dim3(n), dim3(256), 0, st, zones, w);
+            else hipLaunchKernelGGL((k_lu_block_list<true, false, CI>), dim3(n), dim3(256), 0, st, zones, w);
+        } else {
+            if (lower) hipLaunchKernelGGL((k_lu_block_list<false, true, CI>), dim3(n), dim3(256), 0, st, zones, w);
+            else hipLaunchKernelGGL((k_lu_block_list<false, false, CI>), dim3(n), dim3(256), 0, st, zones, w);
+        }
+    }
+}
 
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;

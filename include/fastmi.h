@@ -15,6 +15,12 @@
  *   fm_compute           fasts._computePT (Compute/computePT.cpp:44-721, PyTree.py:42-236)
  *   fm_zone_substep      gsdr3 for one sub-iteration (Compute/gsdr3.cpp:57-484)
  *   fm_apply_bc          fasts._applyBC   (BC/applyBC.cpp:33-255, PyTree.py:1155)
+ *   fm_wall_effort       fasts.compute_effort (BC/compute_effort.cpp:34-289, bceffort.for)
+ *   fm_transfers_set     the flattened tc raccords (param_int_tc) that FastC builds from the
+ *                        connectivity tree (Fast/FastC/FastC/PyTree.py, used by
+ *                        Fast/FastC/FastC/Com/setInterpTransfersFast.cpp:640-760)
+ *   fm_transfer          setInterpTransfersFast for injection raccords
+ *                        (Fast/FastC/FastC/Com/setInterpTransfersFast.cpp:88-900, type 0)
  *   fm_residual_norms    display_ss_iteration (DISPLAY/display_ss_iteration.cpp:29-191)
  *   fm_enstrophy_tke     fasts._computePT_enstrophy (POST/computePT_enstrophy.cpp:127-227)
  *   fm_zone_set_field / fm_zone_get_field   the numpy buffers the reference borrows in place
@@ -23,6 +29,7 @@
  */
 #ifndef FASTMI_H
 #define FASTMI_H
+#include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -67,6 +74,33 @@ int fm_zone_substep(fm_ctx* ctx, int zone, int nitcfg, int nitrun, int first_it,
 /* Ghost-cell fill of a time level (0 N, 1 N-1, 2 N+1). */
 int fm_apply_bc(fm_ctx* ctx, int zone, int level, int nstep);
 
+/* Wall efforts of one face window of a zone (replaces fasts.compute_effort,
+   BC/compute_effort.cpp:34-289 -> BC/bceffort.for -> eff_flu<scheme>_* of
+   Compute/template_effort.for; FastS.PyTree._computeStress, PyTree.py:2536-2566).
+   window: 6 ints (i0,i1,j0,j1,k0,k1), the face index in the normal direction of idir (1..6;
+   1 or n+1) and the cells across it; level: time level whose state is used (0 N, 1 N-1,
+   2 N+1); pos: moment reference point (3 doubles, NULL = origin).
+   effort[12]: forces x,y,z (pressure minus free-stream pressure, plus viscous stress, scaled by
+   2/(rho_inf V_inf^2), outward), moments about pos, wetted area, and the 5 raw face-flux sums.
+   fields (NULL or 21 x nfaces, i fastest within the window): the teff variables of
+   createStressNodes (Density, MomentumX/Y/Z, EnergyStagnationDensity = the scaled fluxes,
+   12 face gradients of u, v, w, T, CoefPressure, ViscosityMolecular, Density2, Pressure).
+   Synchronous. */
+int fm_wall_effort(fm_ctx* ctx, int zone, int level, int idir, const int* window,
+                   const double* pos, double* effort, double* fields);
+
+/* Injection raccords between zones (1-to-1 matches), replacing any set before.  Raccord r
+   fills nb_pts[r] receiver cells of zone no_r[r] from donor cells of zone no_d[r]; rcv_pts and
+   dnr_pts hold the points of all raccords one after the other, as 0-based linear indices into
+   the zones' field arrays (the rcvPts / donorPts of setInterpTransfersFast.cpp:753-756).
+   nrac = 0 removes every raccord.  At most 128 zones.  fm_compute then runs, at every
+   sub-iteration, the right-hand side and LU-SGS of all zones, the transfers, and the boundary
+   conditions of all zones (gsdr3.cpp:205-444). */
+int fm_transfers_set(fm_ctx* ctx, int nrac, const int* no_r, const int* no_d,
+                     const int64_t* nb_pts, const int64_t* rcv_pts, const int64_t* dnr_pts);
+/* Apply the transfers to a time level of every zone (0 N, 1 N-1, 2 N+1).  Synchronous. */
+int fm_transfer(fm_ctx* ctx, int level);
+
 /* nit_c time steps of all zones, sub-iterations nstep_deb..nstep_fin each, time levels
    rotated after each step (computePT.cpp:534-679).  modulo_verif: residuals are computed at
    the last sub-iteration of steps where (nitrun+it) % modulo_verif == 0.  Asynchronous on
@@ -98,16 +132,21 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    with the Newton update), FM_PHASE_BC (ghost-cell fill).  No counterpart in the reference. */
 /* Context options.  FM_OPT_LU_SCHEDULE selects how the LU-SGS sweeps are scheduled on the
    GPU (both give results bit-identical to the reference's lexicographic sweeps):
-   FM_LU_PIPELINE (default) persistent 8x8-line tiles that trail their neighbours;
-   FM_LU_BLOCKS one launch per diagonal of 8x8x8 blocks.  No counterpart in the reference. */
+   FM_LU_BLOCKS (default) one launch per diagonal of 8x8x8 blocks;
+   FM_LU_PIPELINE persistent 8x8-line tiles that trail their neighbours.  No counterpart in
+   the reference. */
 #define FM_OPT_LU_SCHEDULE 1
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
-/* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_FACES (default) evaluates every face flux once and sums
-   them per cell in the reference order; FM_RES_CELLS evaluates the six faces of every cell. */
+/* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS (default) evaluates the six faces of every cell;
+   FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order. */
 #define FM_OPT_RESIDUAL_SCHEDULE 2
 #define FM_RES_CELLS 0
 #define FM_RES_FACES 1
+/* FM_OPT_ZONE_BATCH: 1 (default) runs the LU-SGS sweeps of all zones together, one launch per
+   block diagonal over a work list of (zone, block) pairs (the blocks schedule); 0 sweeps the
+   zones one after the other.  Same results. */
+#define FM_OPT_ZONE_BATCH 3
 int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
 
 #define FM_PHASE_COE 0

@@ -76,6 +76,25 @@ class _Backend:
           _d(grad))
         return out
 
+    def effort(self, z, idir, ind, pos=(0., 0., 0.), rop=None, fields=True):
+        """Wall efforts of the face window ``ind`` (face index in the normal direction idir,
+        cells across it) of the state ``rop`` (bceffort.for): the 12 sums and, when ``fields``,
+        the 21 per-face fields (21 x nfaces, i fastest)."""
+        rop = z.ro if rop is None else rop
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        nf = int((ind[1] - ind[0] + 1) * (ind[3] - ind[2] + 1) * (ind[5] - ind[4] + 1))
+        out = np.zeros(12)
+        flu = np.zeros(21 * nf) if fields else None
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        f = getattr(self.lib, self.prefix + "effort")
+        f.restype = ctypes.c_int
+        rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(idir), _i(ind), _d(pos), _d(rop),
+               _d(z.xmut), _d(z.wig), _d(z.x), _d(z.y), _d(z.z), _d(z.ti), _d(z.tj), _d(z.tk),
+               _d(z.vol), _d(out), _d(flu))
+        if rc != 0:
+            raise RuntimeError("%s effort failed (%d)" % (self.prefix, rc))
+        return (out, flu.reshape(21, nf)) if fields else out
+
     def apply_bc(self, z, rop, nstep=1):
         f = getattr(self.lib, self.prefix + "apply_bc")
         rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nstep), _d(rop), _d(z.ti),

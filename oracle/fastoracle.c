@@ -15,7 +15,8 @@
 #include <stdio.h>
 
 enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 27, ITYPCP = 29,
-       SIZE_SSDOM = 30, KFLUDOM = 33, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
+       SIZE_SSDOM = 30, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
+       NEQ_VENT = 40,
        NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
        NSSITER = 58, SLOPE = 64, PT_BC = 70, LU_MATCH = 75, SENSORTYPE = 85,
        BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2, BC_NBDATA = 8 };
@@ -568,7 +569,7 @@ static void inviscid_flux(const FC* f, long l, int d, const double tn[3], double
 }
 
 /* centred viscous flux, 3dcart: fluvisq_3dcart_{i,j,k}.for */
-static void visc_cart(const FC* f, long l, int d, double flu[5])
+static void visc_cart(const FC* f, long l, int d, double flu[5], double* gv)
 {
     const Ix* Z = f->Z;
     long nd = Z->nd, inci = 1, incj = Z->incj, inck = Z->inck;
@@ -602,6 +603,10 @@ static void visc_cart(const FC* f, long l, int d, double flu[5])
         fv = -(f2 * tcx) * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -(f3 * tcx) * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - (gTx * tcx) * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[1] = gUy; gv[2] = gUz; gv[3] = gVx; gv[4] = gVy;
+            gv[6] = gWx; gv[8] = gWz; gv[9] = gTx;
+        }
     } else if (d == 1) {
         long il = l - incj, ir = l;
         long l022 = l - incj - inck, l002 = l - inck, l021 = l - incj + inck, l001 = l + inck;
@@ -628,6 +633,10 @@ static void visc_cart(const FC* f, long l, int d, double flu[5])
         fv = -(2. * f4 * tcy) * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -(f5 * tcy) * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - (gTy * tcy) * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[1] = gUy; gv[3] = gVx; gv[4] = gVy; gv[5] = gVz;
+            gv[7] = gWy; gv[8] = gWz; gv[10] = gTy;
+        }
     } else {
         long il = l - inck, ir = l;
         long l200 = l - inci, l202 = l - inci - inck, l102 = l + inci - inck, l100 = l + inci;
@@ -654,11 +663,16 @@ static void visc_cart(const FC* f, long l, int d, double flu[5])
         fv = -f5 * tcz * xmutvol; fv5 = fv5 + fv * (V[ir] + V[il]); flu[2] = flu[2] + fv;
         fv = -2. * f6 * tcz * xmutvol; fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
         fv = fv5 * 0.5 - gTz * tcz * xktvol; flu[4] = flu[4] + fv;
+        if (gv) {
+            gv[0] = gUx; gv[2] = gUz; gv[4] = gVy; gv[5] = gVz;
+            gv[6] = gWx; gv[7] = gWy; gv[8] = gWz; gv[11] = gTz;
+        }
     }
+    if (gv) gv[12] = f->volinv;
 }
 
 /* centred viscous flux, 3dfull: fluvisq_3dfull_{i,j,k}.for */
-static void visc_full(const FC* f, long l, long lt, int d, const double tn[3], double flu[5])
+static void visc_full(const FC* f, long l, long lt, int d, const double tn[3], double flu[5], double* gv)
 {
     const Ix* Z = f->Z;
     long nd = Z->nd, ndm = Z->ndm;
@@ -729,6 +743,11 @@ static void visc_full(const FC* f, long l, long lt, int d, const double tn[3], d
     fv5 = fv5 + fv * (W[ir] + W[il]); flu[3] = flu[3] + fv;
     fv = fv5 * 0.5 - (g[3][0] * tcx + g[3][1] * tcy + g[3][2] * tcz) * xktvol;
     flu[4] = flu[4] + fv;
+    if (gv) {
+        for (int e = 0; e < 4; e++)
+            for (int m = 0; m < 3; m++) gv[3 * e + m] = g[e][m];
+        gv[12] = volinv;
+    }
 }
 
 static void face_flux(const FC* f, long i, long j, long k, int d, double flu[5])
@@ -750,8 +769,107 @@ static void face_flux(const FC* f, long i, long j, long k, int d, double flu[5])
     slope_state(f, l, inc, qm, qp);
     inviscid_flux(f, l, d, tn, sn, f->cart, qm, qp, flu);
     if (f->lamin) {
-        if (f->cart) visc_cart(f, l, d, flu);
-        else visc_full(f, l, lt, d, tn, flu);
+        if (f->cart) visc_cart(f, l, d, flu, NULL);
+        else visc_full(f, l, lt, d, tn, flu, NULL);
+    }
+}
+
+static int cl_window(const int* p, int idir, const int* fen, int ind_cl[6]);
+static void shape_tab(int neq_mtr, int idir, int* ic, int* jc, int* kc, double* cim, double* cjm,
+                      double* ckm);
+
+/* correct_flux.for:44-260 for Roe (lapply = 4): at every wall face (BCWallInviscid /
+   BCSymmetryPlane, BCWall / BCWallViscous and variants) of the zone, in BC order,
+   1. the Euler face flux that the balance added is taken back out (template_correction.for,
+      corr_fluroe_*, assemble_drodm_corrflu.for: df = flu*sens + flux(li)*c28, c28 = 0);
+   2. the wall flux is put in: bflwall.for (pressure and the convective term of the face
+      averaged state, no mass or energy flux) or bflwallslip.for (slip wall: convective fluxes
+      with the normal velocity of the averaged state), assemble_drodm_corr.for.
+   Both act on the interior cell next to the face; the window is indice_cl_sdm.for with
+   npass = 0 and its normal range set to the face index (correct_flux.for:116-133).       */
+static void correct_flux_roe(const int* p, const double* pr, const FC* f, double* drodm)
+{
+    const Ix* Z = f->Z;
+    long nd = Z->nd, ndm = Z->ndm;
+    int pt_bcs = p[PT_BC], nb_bc = pt_bcs > 0 ? p[pt_bcs] : 0;
+    double rgp = pr[CVINF] * (pr[GAMMA] - 1.);
+    double gam2 = pr[GAMMA] / (pr[GAMMA] - 1.);
+    double ck_vent = p[NEQ_VENT] != 3 ? 0. : 1.;
+    double c_ale = 1. * (p[LALE] < 1 ? p[LALE] : 1);
+    FC g = *f;
+    g.lamin = 0;      /* iflow = 1: only the Euler flux is corrected */
+    for (int n = 0; n < nb_bc; n++) {
+        int pt = p[pt_bcs + 1 + n], idir = p[pt + BC_IDIR], t = p[pt + BC_TYPE];
+        int slip = t == 3, wall = t == 4 || t == 6 || t == 7 || t == 12;
+        if (!slip && !wall) continue;
+        int cl[6];
+        if (cl_window(p, idir, p + pt + BC_FEN, cl)) continue;
+        switch (idir) {
+        case 1: cl[0] = 1; cl[1] = 1; break;
+        case 2: cl[1] = cl[0]; break;
+        case 3: cl[2] = 1; cl[3] = 1; break;
+        case 4: cl[3] = cl[2]; break;
+        case 5: cl[4] = 1; cl[5] = 1; break;
+        default: cl[5] = cl[4]; break;
+        }
+        int d = (idir - 1) / 2;
+        long inc = d == 0 ? 1 : d == 1 ? Z->incj : Z->inck;
+        int even = idir % 2 == 0;
+        long shift = even ? inc : 0;
+        double flu[5];
+        /* 1. take the Euler face flux back out */
+        double sens = even ? 1. : -1.;
+        for (long k = cl[4]; k <= cl[5]; k++)
+            for (long j = cl[2]; j <= cl[3]; j++)
+                for (long i = cl[0]; i <= cl[1]; i++) {
+                    long l = IDX(Z, i, j, k), l0 = l - shift;
+                    face_flux(&g, i, j, k, d, flu);
+                    for (int e = 0; e < 5; e++) {
+                        double df = flu[e] * sens + 0.;
+                        drodm[l0 + e * nd] = drodm[l0 + e * nd] + df;
+                    }
+                }
+        /* 2. add the wall flux */
+        const double* tijk = d == 0 ? f->ti : d == 1 ? f->tj : f->tk;
+        int neq_mtr = idir <= 4 ? p[NEQ_IJ] : p[NEQ_K];
+        int ic, jc, kc;
+        double cim, cjm, ckm;
+        shape_tab(neq_mtr, idir, &ic, &jc, &kc, &cim, &cjm, &ckm);
+        double sw = even ? -1. : 1.;
+        const double* rop = f->rop;
+        for (long k = cl[4]; k <= cl[5]; k++)
+            for (long j = cl[2]; j <= cl[3]; j++)
+                for (long i = cl[0]; i <= cl[1]; i++) {
+                    long l = IDX(Z, i, j, k), iadrf = l - inc, l0 = l - shift;
+                    long lt = MTR(Z, i, j, k);
+                    double tcx = tijk[lt + ndm * (ic - 1)] * cim;
+                    double tcy = tijk[lt + ndm * (jc - 1)] * cjm;
+                    double tcz = tijk[lt + ndm * (kc - 1)] * ckm;
+                    double qen = (0. * tcx + 0. * tcy + 0. * tcz * ck_vent) * c_ale;
+                    double r = 0.5 * (rop[l] + rop[iadrf]);
+                    double u = 0.5 * (rop[l + nd] + rop[iadrf + nd]);
+                    double v = 0.5 * (rop[l + 2 * nd] + rop[iadrf + 2 * nd]);
+                    double w = 0.5 * (rop[l + 3 * nd] + rop[iadrf + 3 * nd]);
+                    double u_int = tcx * u + tcy * v + tcz * w - qen;
+                    if (wall) {
+                        double pw = 0.5 * (rop[l + 4 * nd] * rop[l] + rop[iadrf + 4 * nd] * rop[iadrf]) * rgp;
+                        flu[0] = 0.;
+                        flu[1] = tcx * pw + u_int * u * r;
+                        flu[2] = tcy * pw + u_int * v * r;
+                        flu[3] = tcz * pw + u_int * w * r;
+                        flu[4] = pw * qen;
+                    } else {
+                        double pi = rop[iadrf + 4 * nd] * rop[iadrf] * rgp;
+                        double h = gam2 * pi + r * (u * u + v * v + w * w) * 0.5;
+                        double pw = 0.5 * (rop[l + 4 * nd] * rop[l] + rop[iadrf + 4 * nd] * rop[iadrf]) * rgp;
+                        flu[0] = u_int * r;
+                        flu[1] = tcx * pw + u_int * r * u;
+                        flu[2] = tcy * pw + u_int * r * v;
+                        flu[3] = tcz * pw + u_int * r * w;
+                        flu[4] = u_int * h + pw * qen;
+                    }
+                    for (int e = 0; e < 5; e++) drodm[l0 + e * nd] = drodm[l0 + e * nd] + flu[e] * sw;
+                }
     }
 }
 
@@ -759,11 +877,11 @@ static void face_flux(const FC* f, long i, long j, long k, int d, double flu[5])
    flu_send_scater_vec1.for: per (j,k) row, k-faces, j-faces, i-faces, then the block-edge
    max faces.  The per-cell accumulation order is therefore
    +Fk(k) +Fj(j) +Fi(i) -Fi(i+1) -Fj(j+1) -Fk(k+1).                                     */
-static int fo_flux(const int* p, const double* pr, const int* ind, const double* rop,
-                   double* drodm, const double* wig, const double* ti, const double* tj,
-                   const double* tk, const double* vol, const double* xmut)
+/* the constants of the flux templates (template_FluxAndBalance.for, template_effort.for) */
+static int fc_init(FC* fp, const Ix* Z, const int* p, const double* pr, const double* rop,
+                   const double* wig, const double* ti, const double* tj, const double* tk,
+                   const double* vol, const double* xmut)
 {
-    Ix zz = mkix(p); Ix* Z = &zz;
     FC f;
     memset(&f, 0, sizeof f);
     f.Z = Z; f.rop = rop; f.xmut = xmut; f.wig = wig; f.ti = ti; f.tj = tj; f.tk = tk; f.vol = vol;
@@ -790,6 +908,17 @@ static int fo_flux(const int* p, const double* pr, const int* ind, const double*
         f.s[0] = fabs(f.tc[0]); f.s[1] = fabs(f.tc[1]); f.s[2] = fabs(f.tc[2]);
         f.volinv = 0.5 / vol[0];
     }
+    *fp = f;
+    return 0;
+}
+
+static int fo_flux(const int* p, const double* pr, const int* ind, const double* rop,
+                   double* drodm, const double* wig, const double* ti, const double* tj,
+                   const double* tk, const double* vol, const double* xmut)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    FC f;
+    if (fc_init(&f, Z, p, pr, rop, wig, ti, tj, tk, vol, xmut)) return -1;
     long nd = Z->nd;
     double flu[5];
 #define PLUS(L) for (int e = 0; e < 5; e++) { drodm[(L) + e * nd] = drodm[(L) + e * nd] + flu[e]; }
@@ -831,6 +960,100 @@ static int fo_flux(const int* p, const double* pr, const int* ind, const double*
     }
 #undef PLUS
 #undef MINUS
+    if (f.kflu == 5) correct_flux_roe(p, pr, &f, drodm);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Wall efforts: bceffort.for -> eff_flu<scheme>_<model>_<slope>_<mesh>, generated from
+   template_effort.for:100-440.  On every face of the window ind (the face index in the
+   normal direction idir, cells across it), in loop order k, j, i:
+   the full face flux (inviscid + viscous, the balance's own fluxes) minus the free-stream
+   pressure, normalised by 2 / (rho_inf V_inf^2) and signed outward (sens), stored with the
+   face gradients, pressure coefficient, viscosity, density and pressure (21 fields, the
+   teff variables of createStressNodes); the 12 sums are forces (1-3), moments about pos
+   of the face centre (4-6), the wetted area (7) and the raw fluxes (8-12).
+   flu: 21 x nfaces (face-major within each field) or NULL.                               */
+int fo_effort(const int* p, const double* pr, int idir, const int* ind, const double* pos,
+              const double* rop, const double* xmut, const double* wig, const double* x,
+              const double* y, const double* z, const double* ti, const double* tj,
+              const double* tk, const double* vol, double* eff, double* flu)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    FC f;
+    if (fc_init(&f, Z, p, pr, rop, wig, ti, tj, tk, vol, xmut)) return -1;
+    int d = (idir - 1) / 2;
+    long nd = Z->nd, ndm = Z->ndm;
+    long n0 = ind[1] - ind[0] + 1, n1 = ind[3] - ind[2] + 1, n2 = ind[5] - ind[4] + 1;
+    long nf = n0 * n1 * n2;
+    double pinf = pr[PINF];
+    double norm = 2. / (pr[ROINF] * (pr[VINF] * pr[VINF]));
+    double sens = norm, sens1 = 1.;
+    if (idir % 2 == 1) { sens = -sens; sens1 = -sens1; }
+    double flagi = 1., flagj = 1., flagk = 1.;
+    if (p[ITYPZONE] == 2) { flagi = 0.; flagj = 0.; flagk = 0.; }
+    long inc = d == 0 ? 1 : d == 1 ? Z->incj : Z->inck;
+    long x1, x2;
+    if (d == 0) { x1 = Z->xi; x2 = Z->xi * Z->xj; }
+    else if (d == 1) { x1 = 1; x2 = Z->xi * Z->xj; }
+    else { x1 = 1; x2 = Z->xi; }
+    long x3 = x1 + x2;
+    for (int q = 0; q < 12; q++) eff[q] = 0.;
+    for (long k = ind[4]; k <= ind[5]; k++)
+        for (long j = ind[2]; j <= ind[3]; j++)
+            for (long i = ind[0]; i <= ind[1]; i++) {
+                long l = IDX(Z, i, j, k), lt = 0;
+                double qm[5], qp[5], tn[3], sn, fl[5], gv[13];
+                for (int q = 0; q < 12; q++) gv[q] = 0.;
+                gv[12] = f.volinv;
+                if (f.cart) {
+                    tn[0] = f.tc[0]; tn[1] = f.tc[1]; tn[2] = f.tc[2]; sn = f.s[d];
+                } else {
+                    const double* t = d == 0 ? ti : d == 1 ? tj : tk;
+                    lt = MTR(Z, i, j, k);
+                    tn[0] = t[lt]; tn[1] = t[lt + ndm]; tn[2] = t[lt + 2 * ndm];
+                    sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+                }
+                slope_state(&f, l, inc, qm, qp);
+                inviscid_flux(&f, l, d, tn, sn, f.cart, qm, qp, fl);
+                if (f.lamin) {
+                    if (f.cart) visc_cart(&f, l, d, fl, gv);
+                    else visc_full(&f, l, lt, d, tn, fl, gv);
+                }
+                double p1 = qp[0] * qp[4] * f.rgp, p2 = qm[0] * qm[4] * f.rgp;
+                double tcx = tn[0], tcy = tn[1], tcz = tn[2];
+                double v[21];
+                v[0] = fl[0] * sens1;
+                v[1] = (fl[1] - pinf * tcx * (d == 0 ? 1. : flagi)) * sens;
+                v[2] = (fl[2] - pinf * tcy * (d == 1 ? 1. : flagj)) * sens;
+                v[3] = (fl[3] - pinf * tcz * (d == 2 ? 1. : flagk)) * sens;
+                if (d == 0) v[1] = (fl[1] - pinf * tcx) * sens;
+                if (d == 1) v[2] = (fl[2] - pinf * tcy) * sens;
+                if (d == 2) v[3] = (fl[3] - pinf * tcz) * sens;
+                v[4] = fl[4] * sens;
+                for (int q = 0; q < 12; q++) v[5 + q] = gv[q] * 2. * gv[12];
+                v[17] = (0.5 * (p1 + p2) - pinf) * norm;
+                v[18] = f.lamin ? 0.5 * (xmut[l] + xmut[l - inc]) : 0.;
+                /* Roe: etat_roe_GD.for:11-12 leaves qm1, qp1 holding the square roots */
+                v[19] = f.kflu == 5 ? 0.5 * (sqrt(qm[0]) + sqrt(qp[0])) : 0.5 * (qm[0] + qp[0]);
+                v[20] = 0.5 * (p1 + p2);
+                long lx = XYZ(Z, i, j, k);
+                double f4 = 0.25 * (x[lx] + x[lx + x1] + x[lx + x2] + x[lx + x3]);
+                double f5 = 0.25 * (y[lx] + y[lx + x1] + y[lx + x2] + y[lx + x3]);
+                double f6 = 0.25 * (z[lx] + z[lx + x1] + z[lx + x2] + z[lx + x3]);
+                double m1 = v[3] * (f5 - pos[1]) - v[2] * (f6 - pos[2]);
+                double m2 = v[1] * (f6 - pos[2]) - v[3] * (f4 - pos[0]);
+                double m3 = v[2] * (f4 - pos[0]) - v[1] * (f5 - pos[1]);
+                eff[0] = eff[0] + v[1]; eff[1] = eff[1] + v[2]; eff[2] = eff[2] + v[3];
+                eff[3] = eff[3] + m1; eff[4] = eff[4] + m2; eff[5] = eff[5] + m3;
+                eff[6] = eff[6] + sn;
+                eff[7] = eff[7] + v[0];
+                for (int e = 1; e < 5; e++) eff[7 + e] = eff[7 + e] + fl[e] * sens1;
+                if (flu) {
+                    long lf = (i - ind[0]) + (j - ind[2]) * n0 + (k - ind[4]) * n0 * n1;
+                    for (int q = 0; q < 21; q++) flu[lf + q * nf] = v[q];
+                }
+            }
     return 0;
 }
 
@@ -1501,8 +1724,7 @@ static void correct_coins_all(const int* p, const int* ind_loop, double* rop)
 /* BCzone.cpp:60-865 for one thread owning the zone (npass = 0).  Supported types:
    0 extrapolate, 1 farfield, 3/5 (and 4 in Euler) inviscid wall, 4/6 viscous adiabatic
    wall, 10 outflow, 11 autoperiod, 13 inflow (free-stream state), 16 outpres (pressure data).
-   Returns -1 otherwise, and for Roe with walls (correct_flux.for is not restated).        */
-static int is_wall_type(int t) { return t == 3 || t == 4 || t == 5 || t == 6 || t == 7 || t == 12; }
+   Returns -1 otherwise.                                                                    */
 
 static int fo_bczone(int* p, double* pr, const double* ti, const double* tj, const double* tk,
                      int lrhs, int* shift_lu, double* rop, int* lcorner_out)
@@ -1545,7 +1767,6 @@ static int fo_bczone(int* p, double* pr, const double* ti, const double* tj, con
         if (cl_window(p, idir, p + pt + BC_FEN, cl)) continue;
         if (lrhs == 1 && (t == 0 || t == 1 || t == 2 || t == 10 || t == 11 || t == 14 || t == 16 ||
                           t == 17 || t == 20 || t == 21)) t = 0;
-        if (p[KFLUDOM] == 5 && is_wall_type(t)) return -1;  /* Roe wall-flux correction (correct_flux.for) */
         const double* tijk = idir <= 2 ? ti : idir <= 4 ? tj : tk;
         int neq_mtr = idir <= 4 ? p[NEQ_IJ] : p[NEQ_K];
         int nbdata = p[pt + BC_NBDATA];

@@ -47,6 +47,12 @@ void fo_enstrophy(const int* p, const double* pr, const double* rop, const doubl
                   double* grad);
 void fo_residual_blocks(const int* p, int nitcfg, int nssiter, const double* drodm, double* rdm);
 
+/* wall efforts of one window (bceffort.for); eff[12], flu 21 x nfaces or NULL */
+int fo_effort(const int* p, const double* pr, int idir, const int* ind, const double* pos,
+              const double* rop, const double* xmut, const double* wig, const double* x,
+              const double* y, const double* z, const double* ti, const double* tj,
+              const double* tk, const double* vol, double* eff, double* flu);
+
 #ifdef __cplusplus
 }
 #endif

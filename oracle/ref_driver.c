@@ -388,3 +388,39 @@ int ref_apply_bc(I* p, F* pr, I nstep, F* rop, F* ti, F* tj, F* tk, F* x, F* y, 
     correct_coins_(&nd, p, ind_dm, rop);
     return 0;
 }
+
+/* Wall efforts of one window (FastS.PyTree._computeStress -> compute_effort.cpp:200-258 ->
+   bceffort.for -> eff_flu<scheme>_<model>_<slope>_<mesh>, one thread owning the window).
+   ind: face window (the face index in the normal direction, cells across it), idir 1..6.
+   The effort descriptor holds what createStressNodes (PyTree.py:2405-2497) stores: NEQ 21,
+   NDIMDX = number of faces, the window (EFF_LOOP 23-28), the flux-array origin and strides
+   (EFF_I0 29-33), the zone number (EFF_NONZ 34) and the direction (EFF_IDIR 35).
+   out: the 12 sums of the routine; flu: 21 fields per face (may be NULL). */
+extern void bceffort_(I* ndom, I* ithread, I* param_int, F* param_real, I* param_int_eff,
+                      I* ind_loop, F* effort, F* xyz_ref, F* rop, F* flu, F* wig, F* xmut,
+                      F* x, F* y, F* z, F* ti, F* tj, F* tk, F* vol, F* venti, F* ventj, F* ventk);
+
+int ref_effort(I* p, F* pr, I idir, I* ind, F* pos, F* rop, F* xmut, F* wig, F* x, F* y, F* z,
+               F* ti, F* tj, F* tk, F* vol, F* out, F* flu_out)
+{
+    I nd = 0, ithread = 1;
+    I pe[64];
+    memset(pe, 0, sizeof pe);
+    I n0 = ind[1] - ind[0] + 1, n1 = ind[3] - ind[2] + 1, n2 = ind[5] - ind[4] + 1;
+    I nf = n0 * n1 * n2;
+    pe[NEQ] = 21; pe[NDIMDX] = nf;
+    for (int q = 0; q < 6; q++) pe[23 + q] = ind[q];
+    pe[29] = ind[0]; pe[30] = ind[2]; pe[31] = ind[4]; pe[32] = n0; pe[33] = n0 * n1;
+    pe[34] = 0; pe[35] = idir;
+    F eff[12];
+    memset(eff, 0, sizeof eff);
+    F* flu = (F*)calloc((size_t)nf * 21, sizeof(F));
+    F vent[3] = {0., 0., 0.};
+    F* mut = p[IFLOW] > 1 ? xmut : rop;       /* compute_effort.cpp:124-129 */
+    bceffort_(&nd, &ithread, p, pr, pe, ind, eff, pos, rop, flu, wig, mut, x, y, z, ti, tj, tk,
+              vol, vent, vent, vent);
+    memcpy(out, eff, sizeof eff);
+    if (flu_out) memcpy(flu_out, flu, sizeof(F) * (size_t)nf * 21);
+    free(flu);
+    return 0;
+}

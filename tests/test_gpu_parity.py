@@ -164,6 +164,21 @@ def test_boundary_conditions(ctxmod, orc, case, mode, scheme):
     assert_same(zo, zg, ghosts=True)
 
 
+@pytest.mark.parametrize("case", ["channel", "inviscid", "wall_j"])
+@pytest.mark.parametrize("mode", ["explicit", "bdf", "local"])
+@pytest.mark.parametrize("scheme", ["roe_min", "roe_kap", "roe_nul"])
+def test_roe_wall_flux_correction(ctxmod, orc, case, mode, scheme):
+    """Roe with walls: the Euler flux of every wall face is replaced by the wall flux
+    (correct_flux.for: corr_fluroe_* then bflwall / bflwallslip), ghost cells included."""
+    nsub = {"explicit": 3, "bdf": 1, "local": 1}[mode]
+    if case == "wall_j":
+        mk = lambda: H.jittered_box((10, 9, 6), H.numerics(scheme, mode), bcs=H.WALL_J)
+    else:
+        mk = lambda: BC_CASES[case](scheme, mode)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2)
+    assert_same(zo, zg, ghosts=True)
+
+
 def test_norms_match_residual_blocks(ctxmod, orc):
     """GPU L2/Linf against the oracle's per-block sums (cprdu3s1.for), relative 1e-12."""
     mk = lambda: cases.tgv_zone((24, 20, 16), H.numerics("roe_min", "bdf"), perturb=0.02)
@@ -281,3 +296,40 @@ def test_enstrophy_tke(ctxmod, orc, mesh):
     got = ctx.tke(zid, 0)
     ctx.close()
     np.testing.assert_allclose([got[0], got[2], got[1]], e, rtol=1e-12)
+
+
+EFFORT_CASES = {
+    "cart": lambda num: cases.tgv_zone((13, 9, 7), num, perturb=0.05),
+    "wall": lambda num: H.jittered_box((10, 9, 6), num, bcs=H.WALL_J),
+    "channel": lambda num: cases.channel_zone(16, 9, 5, num),
+    "inviscid": lambda num: H.jittered_box((9, 8, 5), num, bcs=H.INVISCID_J, flow="euler"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EFFORT_CASES))
+@pytest.mark.parametrize("scheme", ["ausmpred", "senseur", "roe_min"])
+def test_wall_effort(ctxmod, orc, case, scheme):
+    """fm_wall_effort (compute_effort.cpp / bceffort.for) against the oracle: per-face fields
+    and the 12 sums bit for bit (both sum in the reference's loop order)."""
+    num = H.numerics(scheme, "bdf")
+    zo, zg = EFFORT_CASES[case](num), EFFORT_CASES[case](num)
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    orc.substep(zo, 1, first_it=0)
+    orc.apply_bc(zo, zo.ro)
+    ctx = ctxmod.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+    ctx.substep(zid, 1, first_it=0)
+    ctx.apply_bc(zid, 0)
+    ni, nj, nk = zo.dims
+    pos = (0.3, -0.2, 0.1)
+    for idir, ind in [(3, [1, ni, 1, 1, 1, nk]), (4, [1, ni, nj + 1, nj + 1, 1, nk]),
+                      (1, [1, 1, 1, nj, 1, nk]), (6, [2, ni - 1, 1, nj, nk + 1, nk + 1])]:
+        eo, fo = orc.effort(zo, idir, ind, pos)
+        eg, fg = ctx.wall_effort(zid, idir, ind, pos, level=0)
+        assert np.array_equal(fo, fg), (case, idir)
+        assert np.array_equal(eo, eg), (case, idir)
+    ctx.close()

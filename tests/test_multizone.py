@@ -87,3 +87,128 @@ def test_split_box_equals_single_zone(orc, scheme):
     Wg = whole.interior(whole.ro)
     for z, blk in zip(zones, blocks_of(Wg, NX, NY, NZ)):
         assert np.array_equal(z.interior(z.ro), blk), z.name
+
+
+# ---------------------------------------------------------------------------------------
+# GPU: fm_transfers_set / fm_compute with several zones against the oracle driven the same way
+def gpu_run(lib, zones, racs, nsub, nsteps, modulo_verif=0):
+    ctx = lib.Context(0)
+    ids = [ctx.add_zone(z) for z in zones]
+    ctx.set_transfers(racs)
+    ctx.transfer(0)
+    for i in ids:
+        ctx.apply_bc(i, 0)
+    for i, z in zip(ids, zones):
+        t = np.zeros_like(z.ro)
+        ctx.get(i, lib.FIELD_RO, t)
+        ctx.set(i, lib.FIELD_RO_M1, t); ctx.set(i, lib.FIELD_RO_P1, t)
+    ctx.compute(0, 1, nsub, nit_c=nsteps, modulo_verif=modulo_verif, first_it=0)
+    ctx.sync()
+    for i, z in zip(ids, zones):
+        ctx.pull(i, z)
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
+def test_gpu_split_box_explicit(orc, scheme):
+    """Euler RK3 on 6 blocks of a periodic box: GPU zones joined by transfers equal the oracle
+    driven the same way, which equals the single zone bit for bit."""
+    from fast_amd import _lib
+    num = H.numerics(scheme, "explicit")
+    zo, pairs, _ = cases.tiled_box(NX, NY, NZ, num, flow="euler")
+    zg, _, _ = cases.tiled_box(NX, NY, NZ, num, flow="euler")
+    racs = connect.build(zo, pairs)
+    init(orc, zo, racs)
+    for step in range(2):
+        oracle_step(orc, zo, racs, 3, 0 if step == 0 else 1)
+    gpu_run(_lib, zg, racs, 3, 2)
+    for a, b in zip(zo, zg):
+        assert np.array_equal(a.ro, b.ro), a.name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["bdf", "bdf_ss2"])
+def test_gpu_tiles_implicit_walls(orc, mode):
+    """Laminar implicit BDF with LU-SGS per zone (block-Jacobi across zones, as across the
+    reference's thread boxes) on uneven blocks, periodic in i and k, viscous walls in j."""
+    from fast_amd import _lib
+    num = H.numerics("ausmpred", mode)
+    nx, ny, nz = (5, 7), (6, 5), (4, 3)
+    mk = lambda: cases.tiled_box(nx, ny, nz, num, flow="lamin", periodic=(True, False, True),
+                                 bcs_outer="BCWallViscous", seed=3)
+    zo, pairs, _ = mk()
+    zg, _, _ = mk()
+    racs = connect.build(zo, pairs)
+    init(orc, zo, racs)
+    nsub = int(zo[0].param_int[P.NSSITER])
+    nsteps = 2
+    for step in range(nsteps):
+        for it in range(1, nsub + 1):
+            verif = step % 2 == 0
+            lexit = 1 if it == nsub else 0
+            if lexit and not verif:
+                continue
+            for z in zo:
+                orc.substep(z, it, first_it=0 if step == 0 else 1, lssiter_verif=int(verif),
+                            lexit_lu=lexit, stop=3)
+            if lexit:
+                continue
+            levels = [ro_cl(z, it) for z in zo]
+            connect.apply(zo, racs, levels)
+            for z, lv in zip(zo, levels):
+                orc.apply_bc(z, lv)
+        for z in zo:
+            H.rotate(z)
+    gpu_run(_lib, zg, racs, nsub, nsteps, modulo_verif=2)
+    for a, b in zip(zo, zg):
+        assert np.array_equal(a.ro, b.ro), a.name
+        assert np.array_equal(a.ro_m1, b.ro_m1), a.name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["bdf", "bdf_ss2", "local"])
+def test_gpu_zone_batch_equals_zone_loop(mode):
+    """The diagonal-batched LU-SGS over all zones (a work list of (zone, block) pairs) gives the
+    per-zone sweeps bit for bit, on uneven blocks and on unconnected zones of both mesh kinds."""
+    from fast_amd import _lib
+    num = H.numerics("ausmpred", mode)
+
+    def run(batch, make):
+        zones, racs = make()
+        ctx = _lib.Context(0)
+        ctx.set_option(_lib.OPT_ZONE_BATCH, batch)
+        ids = [ctx.add_zone(z) for z in zones]
+        ctx.set_transfers(racs)
+        ctx.transfer(0)
+        for i in ids:
+            ctx.apply_bc(i, 0)
+        for i, z in zip(ids, zones):
+            t = np.zeros_like(z.ro)
+            ctx.get(i, _lib.FIELD_RO, t)
+            ctx.set(i, _lib.FIELD_RO_M1, t); ctx.set(i, _lib.FIELD_RO_P1, t)
+        nsub = int(zones[0].param_int[P.NSSITER])
+        ctx.compute(0, 1, nsub, nit_c=2, modulo_verif=2, first_it=0)
+        ctx.sync()
+        for i, z in zip(ids, zones):
+            ctx.pull(i, z)
+        ctx.close()
+        return zones
+
+    def tiles():
+        zones, pairs, _ = cases.tiled_box((9, 17, 6), (12, 7), (10, 8), num, flow="lamin",
+                                          periodic=(True, False, True), bcs_outer="BCWallViscous", seed=4)
+        return zones, connect.build(zones, pairs)
+
+    def mixed():
+        zones = [cases.tgv_zone((19, 11, 9), num, perturb=0.02, name="a"),
+                 H.jittered_box((10, 13, 9), num, bcs=H.WALL_J),
+                 cases.channel_zone(17, 10, 5, num)]
+        return zones, []
+
+    for make in (tiles, mixed):
+        a = run(0, make)
+        b = run(1, make)
+        for za, zb in zip(a, b):
+            assert np.array_equal(za.ro, zb.ro), za.name
+            assert np.array_equal(za.ro_m1, zb.ro_m1), za.name

@@ -71,8 +71,6 @@ requires_ref = pytest.mark.skipif(not oracle.Reference.available(),
 @pytest.mark.parametrize("mode", ["explicit", "bdf", "bdf_ss2", "local"])
 @pytest.mark.parametrize("mesh", ["cart", "full", "wall", "channel", "inviscid", "outflow", "outpres"])
 def test_oracle_vs_reference(orc, scheme, mode, mesh):
-    if mesh not in ("cart", "full") and scheme.startswith("roe"):
-        pytest.skip("Roe with walls needs the wall-flux correction (correct_flux.for), not implemented")
     ref = oracle.Reference()
     nsub = {"explicit": 3, "bdf": 1, "bdf_ss2": 2, "local": 1}[mode]
     if mesh == "cart":
@@ -166,3 +164,68 @@ def test_enstrophy_vs_reference(orc, mesh):
     # the per-cell gradients agree bit for bit; the reference's sums run under !$OMP SIMD
     # (POST/loopI_begin.for), whose partial sums the sequential restatement need not follow
     np.testing.assert_allclose(eo, er, rtol=1e-13)
+
+
+def wall_windows(z):
+    """Face windows (face index, cells across) of the j faces and of the imin face."""
+    ni, nj, nk = z.dims
+    return [(3, [1, ni, 1, 1, 1, nk]), (4, [1, ni, nj + 1, nj + 1, 1, nk]),
+            (1, [1, 1, 1, nj, 1, nk]), (6, [1, ni, 1, nj, nk + 1, nk + 1])]
+
+
+@requires_ref
+@pytest.mark.parametrize("scheme", ["ausmpred", "senseur", "roe_min", "roe_kap"])
+@pytest.mark.parametrize("mesh", ["cart", "wall", "channel", "inviscid"])
+def test_effort_vs_reference(orc, scheme, mesh):
+    """_computeStress (compute_effort.cpp -> bceffort.for -> eff_flu*): per-face fields bit for
+    bit; the 12 sums (reduced under !$OMP SIMD in the reference) to 1e-13."""
+    ref = oracle.Reference()
+    num = H.numerics(scheme, "bdf")
+    if mesh == "cart":
+        mk = lambda: cases.tgv_zone((9, 8, 7), num, perturb=0.05)
+    elif mesh == "wall":
+        mk = lambda: H.jittered_box((8, 9, 5), num, bcs=H.WALL_J)
+    elif mesh == "channel":
+        mk = lambda: cases.channel_zone(10, 8, 4, num)
+    else:
+        mk = lambda: H.jittered_box((7, 8, 5), num, bcs=H.INVISCID_J, flow="euler")
+    zr, zo = mk(), mk()
+    for be, z in ((ref, zr), (orc, zo)):
+        be.init_metric(z)
+        be.apply_bc(z, z.ro)
+        z.ro_m1[:] = z.ro; z.ro_p1[:] = z.ro
+        be.substep(z, 1, first_it=0)        # viscosity and SENSEUR sensor of this state
+        be.apply_bc(z, z.ro)
+    pos = (0.3, -0.2, 0.1)
+    for idir, ind in wall_windows(zr):
+        er, fr = ref.effort(zr, idir, ind, pos)
+        eo, fo = orc.effort(zo, idir, ind, pos)
+        assert np.array_equal(fr, fo), (idir, int(np.argmax(np.any(fr != fo, axis=1))))
+        np.testing.assert_allclose(eo, er, rtol=1e-13, atol=1e-13 * np.abs(er).max())
+        assert er[6] > 0
+
+
+def test_effort_closed_form_uniform_flow(orc):
+    """A uniform state on a Cartesian box: no viscous stress, the pressure part of the force
+    on a j face is (p - p_inf) * area * 2 / (rho_inf V_inf^2) along -+y, the wetted area is the
+    face area, and the moment about the face centre is zero."""
+    num = H.numerics("ausmpred", "bdf")
+    z = cases.tgv_zone((6, 5, 4), num)
+    st = z.state
+    nd = z.nd
+    w = [st["ro"] * 1.1, 0.0, 0.0, 0.0, st["T"]]
+    for e in range(5):
+        z.ro[e * nd:(e + 1) * nd] = w[e]
+    z.ro_m1[:] = z.ro; z.ro_p1[:] = z.ro
+    orc.init_metric(z)
+    orc.substep(z, 1, first_it=0)
+    ni, nj, nk = z.dims
+    p = w[0] * w[4] * st["cv"] * (st["gamma"] - 1.)
+    norm = 2. / (z.param_real[P.ROINF] * z.param_real[P.VINF] ** 2)
+    area = (2 * np.pi / ni) * (2 * np.pi / nk)
+    ind = [1, ni, nj + 1, nj + 1, 1, nk]
+    eff, flu = orc.effort(z, 4, ind, pos=(np.pi, 2 * np.pi, np.pi))
+    np.testing.assert_allclose(eff[1], (p - z.param_real[P.PINF]) * area * ni * nk * norm, rtol=1e-12)
+    np.testing.assert_allclose(eff[6], area * ni * nk, rtol=1e-12)
+    assert abs(eff[0]) < 1e-12 and abs(eff[2]) < 1e-12
+    np.testing.assert_allclose(eff[3:6], 0., atol=1e-11 * abs(eff[1]))

@@ -108,3 +108,78 @@ def test_warmup_compute_matches_oracle():
         assert cfl[0] >= cfl[2] >= cfl[1] > 0
     enst, tke = FP._computeEnstrophy(t, metrics)
     assert np.isfinite(tke) and tke > 0 and np.isfinite(enst) and enst > 0
+
+
+def tiled_tree(num, flow="euler"):
+    from fast_amd import connect
+    zones, pairs, _ = cases.tiled_box((3, 5), (4, 2), (4,), num, flow=flow)
+    t = tree_of(zones, num, model="Euler" if flow == "euler" else "NSLaminar")
+    tz = Internal.getZones(t)
+    for n, (zr, fr, zd, fd) in enumerate(pairs):
+        pr, pd = zones[zr].param_int, zones[zd].param_int
+        idr = connect.FACES.index(fr) + 1
+        idd = connect.FACES.index(fd) + 1
+        FP._addMatch(tz[zr], "match%d" % n, tz[zd][0],
+                     FP._fen_to_pointrange(pr, idr, P.face_window(pr, fr)[1]),
+                     FP._fen_to_pointrange(pd, idd, P.face_window(pd, fd)[1]))
+    return t, zones, pairs
+
+
+def test_gridconnectivity1to1_gives_the_tiling_raccords():
+    """GridConnectivity1to1 nodes (PointRange / PointRangeDonor) read back as the same
+    injection raccords as the tiling itself."""
+    from fast_amd import connect
+    num = H.numerics("roe_min", "explicit")
+    t, zones, pairs = tiled_tree(num)
+    b = Internal.getBases(t)[0]
+    tz = Internal.getZones(t)
+    got = FP._matches([(z, FP._host_zone(b, z)) for z in tz])
+    ref = connect.build(zones, pairs)
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        assert (g.zr, g.zd) == (r.zr, r.zd)
+        assert np.array_equal(g.rcv, r.rcv) and np.array_equal(g.dnr, r.dnr)
+
+
+def test_partial_window_match():
+    """A window covering part of a face fills only that part of the ghost slab."""
+    from fast_amd import connect
+    num = H.numerics("roe_min", "explicit")
+    zones, _, _ = cases.tiled_box((3, 4), (6,), (5,), num)
+    pr, pd = zones[0].param_int, zones[1].param_int
+    rcv, dnr = connect.match(pr, "imax", pd, "imin", offset=(0, 0), window=((2, 4), (1, 3)))
+    assert len(rcv) == 2 * 3 * 3
+    assert rcv[0] == connect._lin(pr, 4, 2, 1) and dnr[0] == connect._lin(pd, 1, 2, 1)
+    with pytest.raises(NotImplementedError):
+        connect.from_pointranges(pr, [[1, 1], [1, 2], [1, 2]], pd, [[1, 1], [1, 2], [1, 2]], (2, 1, 3))
+
+
+@pytest.mark.gpu
+def test_warmup_compute_tiled_tree_matches_single_zone():
+    """warmup + _compute on 6 blocks joined by GridConnectivity1to1 (explicit RK3, Euler)
+    equal the single periodic zone stepped by the oracle, bit for bit."""
+    import oracle
+    orc = oracle.Oracle()
+    num = H.numerics("ausmpred", "explicit")
+    t, zones, pairs = tiled_tree(num)
+    t, tc, metrics = FP.warmup(t, None)
+    assert len(FP.HOOK["raccords"]) == len(pairs)
+    nit = 2
+    FP._compute(t, metrics, 0, NIT=nit)
+    _, _, W = cases.tiled_box((3, 5), (4, 2), (4,), num, flow="euler")
+    whole = cases.whole_box((3, 5), (4, 2), (4,), W, num, flow="euler")
+    orc.init_metric(whole)
+    orc.apply_bc(whole, whole.ro)
+    whole.ro_m1[:] = whole.ro; whole.ro_p1[:] = whole.ro
+    for step in range(nit):
+        for it in range(1, 4):
+            orc.substep(whole, it, first_it=0 if step == 0 else 1)
+        H.rotate(whole)
+    Wg = whole.interior(whole.ro)
+    ox, oy = np.cumsum([0, 3, 5]), np.cumsum([0, 4, 2])
+    for n, z in enumerate(Internal.getZones(t)):
+        a, bb = n % 2, n // 2
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        got = np.stack([Internal.getNodeFromName1(fs, v)[1][2:-2, 2:-2, 2:-2].transpose(2, 1, 0)
+                        for v in FP.VARS])
+        assert np.array_equal(got, Wg[:, :, oy[bb]:oy[bb + 1], ox[a]:ox[a + 1]]), z[0]
