@@ -9,7 +9,9 @@ Drop-in for the FastS compute path of ``Fast/FastS/FastS/PyTree.py``:
 * ``_applyBC(t, metrics, hook1, nstep, var)``           PyTree.py:1155
 * ``_fillGhostcells(zones, tc, metrics, ...)``           transfers + ``_applyBC``; zones joined by
   CGNS GridConnectivity1to1 nodes (injection raccords, FastC/Com/setInterpTransfersFast.cpp)
-* ``_computeEnstrophy(t, metrics)`` (kinetic-energy part)  PyTree.py:1878
+* ``_computeEnstrophy(t, metrics)``                      PyTree.py:1878
+* ``createStressNodes(t, BC, windows)`` / ``_computeStress(t, teff, metrics, xyz_ref)``
+  PyTree.py:2277-2566 (wall efforts, bceffort.for)
 * ``_createConvergenceHistory(t, nrecs)``               PyTree.py:2152
 * numerics setters ``_setNum2Base`` / ``_setNum2Zones``  (FastC/PyTree.py:115-137)
 
@@ -433,3 +435,110 @@ def _computeEnstrophy(t, metrics):
         s = ctx.tke(zid, 0)
         k += s[0]; v += s[1]; e += s[2]
     return e / v, k / v
+
+
+# ---------------------------------------------------------------------------------------
+# Wall efforts (FastS.PyTree.createStressNodes / _computeStress, PyTree.py:2277-2566)
+EFF_VARS = ["Density", "MomentumX", "MomentumY", "MomentumZ", "EnergyStagnationDensity",
+            "gradxVelocityX", "gradyVelocityX", "gradzVelocityX", "gradxVelocityY", "gradyVelocityY",
+            "gradzVelocityY", "gradxVelocityZ", "gradyVelocityZ", "gradzVelocityZ",
+            "gradxTemperature", "gradyTemperature", "gradzTemperature", "CoefPressure",
+            "ViscosityMolecular", "Density2", "Pressure"]
+EFF_LOOP, EFF_I0, EFF_NONZ, EFF_IDIR = 23, 29, 34, 35
+
+
+def createStressNodes(t, BC=None, windows=None):
+    """A tree of surface zones, one per selected wall window, to receive the efforts
+    (PyTree.py:2277-2530).  BC: list of BC types (e.g. ['BCWall', 'BCWallViscous']); windows:
+    list of (zone name, i0, i1, j0, j1, k0, k1, 'imin'..'kmax') in the ghost-inclusive vertex
+    numbering of the zone.  Each surface zone carries the face window of its NS zone in
+    Parameter_int (EFF_LOOP 23-28, EFF_I0 29-33, EFF_NONZ 34, EFF_IDIR 35) and the 21 centre
+    fields of the reference, zero until _computeStress."""
+    teff = Internal.newCGNSTree()
+    no_z = 0
+    for b0 in Internal.getBases(t):
+        b = Internal.newCGNSBase(b0[0], parent=teff)
+        for z in Internal.getNodesFromType1(b0, "Zone_t"):
+            own = Internal.getNodeFromName1(z, ".Solver#ownData")
+            pnode = Internal.getNodeFromName1(own, "Parameter_int") if own is not None else None
+            if pnode is None:
+                raise ValueError("createStressNodes must be called after warmup (zone %s)" % z[0])
+            p = np.asarray(pnode[1])
+            sel = []
+            if BC is not None:
+                zbc = Internal.getNodeFromName1(z, "ZoneBC")
+                for bc in (Internal.getNodesFromType1(zbc, "BC_t") if zbc is not None else []):
+                    if Internal.getValue(bc) in BC:
+                        rng = np.asarray(Internal.getNodeFromType1(bc, "IndexRange_t")[1])
+                        sel.append((bc[0], rng, None))
+            for w in (windows or []):
+                if w[0] == z[0]:
+                    sel.append(("window", np.array([[w[1], w[2]], [w[3], w[4]], [w[5], w[6]]]), w[7]))
+            for inum, (name, rng, face) in enumerate(sel):
+                flat = [int(rng[0][0]), int(rng[0][1]), int(rng[1][0]), int(rng[1][1]),
+                        int(rng[2][0]), int(rng[2][1])]
+                idir, fen = P.get_range(flat, p)
+                if face is not None:
+                    idir = ["imin", "imax", "jmin", "jmax", "kmin", "kmax"].index(face) + 1
+                if idir == 0:
+                    raise ValueError("window %s of zone %s is not on a face" % (name, z[0]))
+                d = (idir - 1) // 2
+                win = list(fen)
+                nface = win[2 * d] if idir % 2 == 1 else win[2 * d] + 1
+                win[2 * d] = win[2 * d + 1] = nface
+                tang = [a for a in range(3) if a != d]
+                nt = [win[2 * a + 1] - win[2 * a] + 1 for a in tang]
+                zp = Internal.newZone("%s_%s%d" % (z[0], name, inum), nt[0] + 1, nt[1] + 1, 1, parent=b)
+                g = Internal.getNodeFromName1(z, "GridCoordinates")
+                nf, nfk = int(p[P.NIJK + 3]), int(p[P.NIJK + 4])
+                sl = [slice(None)] * 3
+                for a in range(3):
+                    g0 = nf if a < 2 else nfk
+                    if a == d:
+                        sl[a] = slice(win[2 * a] + g0 - 1, win[2 * a] + g0)
+                    else:
+                        sl[a] = slice(win[2 * a] + g0 - 1, win[2 * a + 1] + g0 + 1)
+                gc = Internal.createChild(zp, "GridCoordinates", "GridCoordinates_t")
+                for cn in ("CoordinateX", "CoordinateY", "CoordinateZ"):
+                    a3 = np.asarray(Internal.getNodeFromName1(g, cn)[1])[tuple(sl)]
+                    Internal.createChild(gc, cn, "DataArray_t",
+                                         np.ascontiguousarray(a3).reshape(nt[0] + 1, nt[1] + 1, 1, order="F"))
+                fs = Internal.createChild(zp, "FlowSolution#Centers", "FlowSolution_t")
+                Internal.createChild(fs, "GridLocation", "GridLocation_t", "CellCenter")
+                for v in EFF_VARS:
+                    Internal.createChild(fs, v, "DataArray_t", np.zeros((nt[0], nt[1], 1), order="F"))
+                pe = np.array(p, dtype=np.int32, copy=True)
+                pe[P.NIJK:P.NIJK + 5] = [nt[0], nt[1], 1, nf, 0]
+                pe[P.NEQ] = len(EFF_VARS)
+                pe[P.NDIMDX] = nt[0] * nt[1]
+                pe[EFF_LOOP:EFF_LOOP + 6] = win
+                n0 = win[1] - win[0] + 1
+                pe[EFF_I0:EFF_I0 + 5] = [win[0], win[2], win[4], n0, n0 * (win[3] - win[2] + 1)]
+                pe[EFF_NONZ] = no_z
+                pe[EFF_IDIR] = idir
+                o = Internal.createUniqueChild(zp, ".Solver#ownData", "UserDefinedData_t")
+                Internal.createUniqueChild(o, "Parameter_int", "DataArray_t", pe)
+            no_z += 1
+    return teff
+
+
+def _computeStress(t, teff, metrics, xyz_ref=(0., 0., 0.)):
+    """Efforts on the windows of teff from the N level of every zone, fields stored in teff
+    (compute_effort.cpp -> bceffort.for).  Returns the 11 sums of the reference: forces x, y, z
+    and moments about xyz_ref (pressure minus the free-stream pressure plus viscous stress,
+    scaled by 2 / (rho_inf V_inf^2)), the wetted area, and the mass, x-, y-, z-momentum flux
+    sums."""
+    ctx = HOOK["ctx"]
+    effort = np.zeros(11)
+    for z in Internal.getZones(teff):
+        pe = np.asarray(Internal.getNodeFromName1(Internal.getNodeFromName1(z, ".Solver#ownData"),
+                                                  "Parameter_int")[1])
+        zid = HOOK["ids"][int(pe[EFF_NONZ])]
+        win = [int(v) for v in pe[EFF_LOOP:EFF_LOOP + 6]]
+        eff, fl = ctx.wall_effort(zid, int(pe[EFF_IDIR]), win, xyz_ref, level=0)
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        for q, v in enumerate(EFF_VARS):
+            node = Internal.getNodeFromName1(fs, v)
+            node[1][...] = fl[q].reshape(node[1].shape, order="F")
+        effort[:] = effort + eff[:11]
+    return effort

@@ -183,3 +183,71 @@ def test_warmup_compute_tiled_tree_matches_single_zone():
         got = np.stack([Internal.getNodeFromName1(fs, v)[1][2:-2, 2:-2, 2:-2].transpose(2, 1, 0)
                         for v in FP.VARS])
         assert np.array_equal(got, Wg[:, :, oy[bb]:oy[bb + 1], ox[a]:ox[a + 1]]), z[0]
+
+
+def _with_own(t):
+    """Descriptor vectors under .Solver#ownData, as allocate_metric leaves them."""
+    b = Internal.getBases(t)[0]
+    hz = []
+    for z in Internal.getZones(t):
+        h = FP._host_zone(b, z)
+        o = Internal.createUniqueChild(z, ".Solver#ownData", "UserDefinedData_t")
+        Internal.createUniqueChild(o, "Parameter_int", "DataArray_t", h.param_int)
+        hz.append(h)
+    return hz
+
+
+def test_create_stress_nodes_windows():
+    """One surface zone per selected wall BC, with its face window (face index n+1 on a max
+    face), the face vertices of the NS zone and the 21 effort fields."""
+    num = H.numerics("ausmpred", "bdf")
+    t = tree_of([cases.channel_zone(12, 8, 4, num)], num)
+    hz = _with_own(t)[0]
+    teff = FP.createStressNodes(t, BC=["BCWallViscous"])
+    zs = Internal.getZones(teff)
+    assert len(zs) == 2
+    ni, nj, nk = hz.dims
+    wins = []
+    for z in zs:
+        pe = Internal.getNodeFromName1(Internal.getNodeFromName1(z, ".Solver#ownData"), "Parameter_int")[1]
+        wins.append((int(pe[FP.EFF_IDIR]), [int(v) for v in pe[FP.EFF_LOOP:FP.EFF_LOOP + 6]]))
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        assert Internal.getNodeFromName1(fs, "CoefPressure")[1].shape == (ni, nk, 1)
+        x = Internal.getNodeFromName1(Internal.getNodeFromName1(z, "GridCoordinates"), "CoordinateY")[1]
+        assert x.shape == (ni + 1, nk + 1, 1)
+    assert wins == [(3, [1, ni, 1, 1, 1, nk]), (4, [1, ni, nj + 1, nj + 1, 1, nk])]
+    # the surface vertices are the NS zone's wall vertices (ghost-inclusive numbering, 2 layers)
+    g = Internal.getNodeFromName1(Internal.getZones(t)[0], "GridCoordinates")
+    y = Internal.getNodeFromName1(g, "CoordinateY")[1]
+    ys = Internal.getNodeFromName1(Internal.getNodeFromName1(zs[1], "GridCoordinates"), "CoordinateY")[1]
+    assert np.array_equal(ys[:, :, 0], y[2:ni + 3, nj + 2, 2:nk + 3])
+
+
+@pytest.mark.gpu
+def test_compute_stress_matches_oracle():
+    """_computeStress after warmup + _compute: the sums of the two channel walls equal the
+    oracle's efforts of the same state, and the teff fields hold the per-face values."""
+    import oracle
+    orc = oracle.Oracle()
+    num = H.numerics("ausmpred", "bdf")
+    mk = lambda: cases.channel_zone(14, 8, 4, num)
+    t = tree_of([mk()], num)
+    t, tc, metrics = FP.warmup(t, None)
+    FP._compute(t, metrics, 0, NIT=2)
+    teff = FP.createStressNodes(t, BC=["BCWallViscous"])
+    eff = FP._computeStress(t, teff, metrics, xyz_ref=(1.0, 0.0, 0.0))
+    zo = mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    for step in range(2):
+        orc.substep(zo, 1, first_it=0 if step == 0 else 1)
+        H.rotate(zo)
+    ni, nj, nk = zo.dims
+    tot = np.zeros(11)
+    for z, (idir, win) in zip(Internal.getZones(teff), [(3, [1, ni, 1, 1, 1, nk]), (4, [1, ni, nj + 1, nj + 1, 1, nk])]):
+        e, f = orc.effort(zo, idir, win, (1.0, 0.0, 0.0))
+        tot = tot + e[:11]
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        cp = Internal.getNodeFromName1(fs, "CoefPressure")[1]
+        assert np.array_equal(cp.ravel(order="F"), f[17])
+    assert np.array_equal(eff, tot)
