@@ -52,6 +52,8 @@ def parse():
                     help="tgv256: the headline line; channel: 257x129x33 curvilinear channel with a "
                          "bump (3dfull, local time step, CFL 10); zones64: 64 abutting blocks "
                          "(4x4x4 tiling, sizes from default_rng(1) in [16, 96]) joined by transfers")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the interchangeable schedules first and keep the fastest (fast_amd.tune)")
     ap.add_argument("--zone-batch", type=int, default=1, choices=[0, 1],
                     help="zones64: 1 batches the LU-SGS of all zones per block diagonal, 0 loops")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -224,6 +226,11 @@ def main():
     ctx.set(zid, _lib.FIELD_RO_P1, buf)
     del buf
     ncell = a.n ** 3
+    tuned = None
+    if a.autotune:
+        from fast_amd import tune
+        tuned = tune.autotune(ctx, zid)
+        a.lu, a.residual = tuned["lu"][0], tuned["residual"][0]
     # warm-up (first step BDF1, then BDF2)
     ctx.compute(0, 1, 1, nit_c=a.warmup, modulo_verif=0, first_it=0)
     ctx.sync()
@@ -262,7 +269,9 @@ def main():
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
            "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
                       % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic",
-                      "lu_schedule": a.lu, "residual_schedule": a.residual},
+                      "lu_schedule": a.lu, "residual_schedule": a.residual,
+                      "autotuned": None if tuned is None else {k: {n: round(t, 4) for n, t in v[1].items()}
+                                                               for k, v in tuned.items()}},
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "event_ms_per_step": round(ms_ev / a.steps, 3),

@@ -23,15 +23,17 @@ def flags():
             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-Wno-unused-result"]
 
 
-def build(verbose=False, jobs=None):
-    """Compile every source to an object in parallel, then link libfastmi.so."""
-    objdir = os.path.join(CSRC, "_build")
+def build(verbose=False, jobs=None, out=None, extra=(), objdir=None):
+    """Compile every source to an object in parallel, then link libfastmi.so (or ``out``);
+    ``extra`` adds compiler flags (e.g. tuning macros such as -DFM_RES_WAVES=3)."""
+    lib = out or LIB
+    objdir = objdir or os.path.join(CSRC, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for s in SOURCES:
         o = os.path.join(objdir, s.replace(".hip", ".o"))
-        cmd = [HIPCC] + flags() + ["-c", os.path.join(CSRC, s), "-o", o]
+        cmd = [HIPCC] + flags() + list(extra) + ["-c", os.path.join(CSRC, s), "-o", o]
         if verbose:
             print(" ".join(cmd))
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -46,11 +48,11 @@ def build(verbose=False, jobs=None):
             sys.stderr.write(out.decode(errors="replace"))
     if failed:
         raise RuntimeError("hipcc compilation failed")
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -334,8 +334,12 @@ __device__ void wall_correction(const FluxConst& f, const KZ& z, const double* _
 //   init, +Fk(k), +Fj(j), +Fi(i), -Fi(i+1), -Fj(j+1), -Fk(k+1).
 // Update: core3as2.for (implicit, writes the Newton residual into drodm) or core3ark3.for
 // (explicit, keeps drodm and writes the new stage state).
+// FM_RES_WAVES: waves per SIMD the register allocation of k_residual aims at (2: no spills)
+#ifndef FM_RES_WAVES
+#define FM_RES_WAVES 2
+#endif
 template <int KFLU, int SLOPE, bool LAMIN, bool CART>
-__global__ void __launch_bounds__(256) k_residual(KZ z, RhsArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES_WAVES))) k_residual(KZ z, RhsArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     int k = blockIdx.z + 1;

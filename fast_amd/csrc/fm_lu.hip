@@ -131,7 +131,7 @@ __device__ __forceinline__ void lu_prod_global(const KZ& z, const LUC& c, const 
 
 // LDS image of one block: the block's own fields (state, coefficients, right-hand side and,
 // for 3dfull, the three mean face normals), the products of the halo cells finished by earlier
-// launches, and the double-buffered j/k product exchange.  Rows are indexed by lane so that
+// launches, and the exchange between the solving wave and the product waves.  Rows are indexed by lane so that
 // the sweep reads (one cell per lane, lanes on different i) hit 16 distinct 8-byte banks.
 template <bool CART, int CI>
 struct LuLds {
@@ -140,14 +140,17 @@ struct LuLds {
     double hI[5][64];
     double hJ[5][CI][8];
     double hK[5][CI][8];
-    double ex[2][2][5][64];
+    double xs[5][64];        // result of the cell each lane solved at this step
+    double wk[4][64];        // its Newton-updated state (upper sweep in place)
+    double px[3][5][64];     // its i, j, k products
 };
 
 // One block of CI x 8 x 8 cells (i x j x k, in sweep order) per launch diagonal.
 //  1. the 4 waves stage the block into LDS with coalesced loads (16 consecutive i per line)
 //     and form the products of the halo cells;
-//  2. wave 0 runs the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles
-//     cell ii = s - jj - kk at step s; results go straight to global memory.
+//  2. the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles cell
+//     ii = s - jj - kk at step s; wave 0 solves, waves 1-3 form the products; results go
+//     straight to global memory.
 template <bool CART, bool LOWER, int CI>
 __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, int bj, int bk,
                                          LuLds<CART, CI>& S) {
@@ -216,10 +219,14 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         }
     }
     __syncthreads();
-    if (tid >= 64) return;
 
-    // 2. skewed wavefront, one wave
-    const int L = tid, jj = L & 7, kk = L >> 3;
+    // 2. skewed wavefront.  At step s, wave 0 solves cell ii = s - jj - kk of every line;
+    //    then waves 1, 2, 3 form that cell's products for its i, j and k consumers, which
+    //    wave 0 reads at step s + 1 (its own lane for i, the lane above in j or in k).  The
+    //    two barriers per step replace one wave doing all of it: the critical path of a step
+    //    is the solve plus one product instead of the solve plus three products.
+    const int wv = tid >> 6;
+    const int L = tid & 63, jj = L & 7, kk = L >> 3;
     const int oj = oj0 + jj, ok = ok0 + kk;
     const bool lane_ok = oj < nj && ok < nk;
     const int j = J(oj), k = K(ok);
@@ -227,12 +234,11 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
     // the Cartesian face normals are read once: inside the loop the compiler could not keep
     // them in registers across the stores to dw
     const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
-    double pi[5] = {0., 0., 0., 0., 0.};
     const int nsteps = ci + 14;
     for (int s = 0; s < nsteps; s++) {
         const int ii = s - jj - kk;
-        const int rd = (s + 1) & 1, wr = s & 1;
-        if (lane_ok && ii >= 0 && ii < ci) {
+        const bool act = lane_ok && ii >= 0 && ii < ci;
+        if (wv == 0 && act) {
             const int oi = oi0 + ii;
             const long l = IDX(z, I(oi), j, k);
             const double r0 = S.st[0][ii][L], u0 = S.st[1][ii][L], v0 = S.st[2][ii][L],
@@ -251,21 +257,20 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
                 if (o == 0) continue;
                 if (D == 0) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = ii == 0 ? S.hI[e][L] : pi[e];
+                    for (int e = 0; e < 5; e++) b[e] = ii == 0 ? S.hI[e][L] : S.px[0][e][L];
                 } else if (D == 1) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = jj == 0 ? S.hJ[e][ii][kk] : S.ex[rd][0][e][L - 1];
+                    for (int e = 0; e < 5; e++) b[e] = jj == 0 ? S.hJ[e][ii][kk] : S.px[1][e][L - 1];
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = kk == 0 ? S.hK[e][ii][jj] : S.ex[rd][1][e][L - 8];
+                    for (int e = 0; e < 5; e++) b[e] = kk == 0 ? S.hK[e][ii][jj] : S.px[2][e][L - 8];
                 }
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
             }
             const double diag = 1. / cf4;
 #pragma unroll
-            for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; }
-            double uk = u0, vk = v0, wk = w0, tk = t0;
+            for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; S.xs[e][L] = x[e]; }
             if (!LOWER && a.newton) {
                 double* __restrict__ O = a.out;
                 const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
@@ -275,27 +280,35 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
                 const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
                 const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
                 O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
-                if (kalias) { uk = u1; vk = v1; wk = w1; tk = t1; }
+                if (kalias) { S.wk[0][L] = u1; S.wk[1][L] = v1; S.wk[2][L] = w1; S.wk[3][L] = t1; }
             }
-            // products of this cell for its three consumers
-            double tx, ty, tz;
-            if (CART) { tx = tci; ty = tz = 0.; }
-            else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
-            lu_prod<CART, 0>(c, u0, v0, w0, t0, S.st[6][ii][L] * signe, tx, ty, tz, x, pi);
-            if (CART) tx = tcj;
-            else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
-            lu_prod<CART, 1>(c, u0, v0, w0, t0, S.st[7][ii][L] * signe, tx, ty, tz, x, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) S.ex[wr][0][e][L] = b[e];
-            if (CART) tx = tck;
-            else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
-            lu_prod<CART, 2>(c, uk, vk, wk, tk, S.st[8][ii][L] * signe, tx, ty, tz, x, b);
-#pragma unroll
-            for (int e = 0; e < 5; e++) S.ex[wr][1][e][L] = b[e];
         }
-        // the next step reads what the neighbouring lanes of this wave wrote; LDS accesses of
-        // one wave complete in issue order, so only the compiler must not reorder them
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __syncthreads();
+        if (wv > 0 && act) {
+            // products of the cell just solved for its three consumers
+            double x[5], b[5];
+#pragma unroll
+            for (int e = 0; e < 5; e++) x[e] = S.xs[e][L];
+            double uu = S.st[1][ii][L], vv = S.st[2][ii][L], ww = S.st[3][ii][L], tt = S.st[4][ii][L];
+            double tx, ty, tz;
+            if (wv == 1) {
+                if (CART) { tx = tci; ty = tz = 0.; }
+                else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
+                lu_prod<CART, 0>(c, uu, vv, ww, tt, S.st[6][ii][L] * signe, tx, ty, tz, x, b);
+            } else if (wv == 2) {
+                if (CART) { tx = tcj; ty = tz = 0.; }
+                else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
+                lu_prod<CART, 1>(c, uu, vv, ww, tt, S.st[7][ii][L] * signe, tx, ty, tz, x, b);
+            } else {
+                if (kalias) { uu = S.wk[0][L]; vv = S.wk[1][L]; ww = S.wk[2][L]; tt = S.wk[3][L]; }
+                if (CART) { tx = tck; ty = tz = 0.; }
+                else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
+                lu_prod<CART, 2>(c, uu, vv, ww, tt, S.st[8][ii][L] * signe, tx, ty, tz, x, b);
+            }
+#pragma unroll
+            for (int e = 0; e < 5; e++) S.px[wv - 1][e][L] = b[e];
+        }
+        __syncthreads();
     }
     (void)NV;
 }

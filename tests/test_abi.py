@@ -36,3 +36,29 @@ def test_missing_library_fails_loudly(tmp_path):
     from fast_amd import _lib
     with pytest.raises(_lib.FastMIError):
         _lib.load(str(tmp_path / "libfastmi.so"))
+
+
+def test_autotune_picks_the_fastest_schedule():
+    """The autotuner sets, for every knob, the value with the smallest measured time."""
+    from fast_amd import tune, _lib
+
+    class FakeCtx:
+        def __init__(self):
+            self.opts = {}
+            self.cost = {(_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_CELLS): 2.0,
+                         (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_FACES): 1.5,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_BLOCKS): 3.0,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0}
+
+        def set_option(self, opt, v):
+            self.opts[opt] = v
+
+        def time_phase(self, zone, phase, reps=1):
+            opt = _lib.OPT_RESIDUAL_SCHEDULE if phase == _lib.PHASE_RESIDUAL else _lib.OPT_LU_SCHEDULE
+            return self.cost[(opt, self.opts[opt])]
+
+    c = FakeCtx()
+    out = tune.autotune(c, 0)
+    assert out["residual"][0] == "faces" and out["lu"][0] == "blocks"
+    assert c.opts[_lib.OPT_RESIDUAL_SCHEDULE] == _lib.RES_FACES
+    assert c.opts[_lib.OPT_LU_SCHEDULE] == _lib.LU_BLOCKS
