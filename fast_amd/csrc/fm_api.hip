@@ -51,6 +51,7 @@ struct Zone {
     double* ro_scratch;      // Newton target when the LU sweep works in place
     double* lu_dws;          // dW* between the pipelined LU sweeps
     int* lu_sync;            // tile counter and error word of the pipelined LU sweeps
+    double* lu_xch;          // tile-boundary exchange of the pipelined LU sweeps (on first use)
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
     long n_metric;
@@ -101,6 +102,7 @@ static void free_zone(Zone* z) {
     for (BcRec& b : z->bcs) if (b.pext) (void)hipFree(b.pext);
     if (z->lu_dws) (void)hipFree(z->lu_dws);
     if (z->lu_sync) (void)hipFree(z->lu_sync);
+    if (z->lu_xch) (void)hipFree(z->lu_xch);
     delete z;
 }
 
@@ -166,7 +168,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->p.assign(p, p + n_int);
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
-    z->lu_dws = nullptr; z->lu_sync = nullptr;
+    z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr;
     z->norm_part = z->norm_out = nullptr; z->have_norms = 0; z->have_cfl = 0;
     KZ& k = z->k;
     k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
@@ -410,6 +412,16 @@ static void coins(fm_ctx* c, Zone* z, double* rop, const int* ind) {
     launch_coins(k, rop, L, c->stream);
 }
 
+// The pipelined LU schedule's exchange buffer, sized for the largest LU box of the zone (one
+// ghost layer past each face, where walls extend it).
+static int ensure_exchange(fm_ctx* c, Zone* z) {
+    if (c->lu_schedule != LU_PIPELINE || z->lu_xch) return FM_OK;
+    long n = lu_exchange_size(z->k.ni + 2, z->k.nj + 2, z->k.nk + 2);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMalloc(&z->lu_xch, sizeof(double) * n));
+    return FM_OK;
+}
+
 // Physical boundary conditions then edges and corners of one zone (gsdr3.cpp:436-439, bcs.cpp).
 static void fill_ghosts(fm_ctx* c, Zone* z, double* rop) {
     int shift[6], sdm[6] = {1, z->k.ni, 1, z->k.nj, 1, z->k.nk};
@@ -532,6 +544,8 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.k0 = shift_lu[4]; la.k1 = shift_lu[5];
             la.dw = k.drodm; la.newton = 1;
             la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
+            if (int rc = ensure_exchange(c, z)) return rc;
+            la.xch = z->lu_xch;
             la.rop = ro_ssiter;
             if (ro_ssiter == ro_p1) {
                 // in place (sub-iterations >= 2): the K terms of the upper sweep see the state
@@ -917,6 +931,8 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.i0 = 1; la.i1 = k.ni; la.j0 = 1; la.j1 = k.nj; la.k0 = 1; la.k1 = k.nk;
     la.dw = k.drodm; la.rop = ro; la.rop_k = ro; la.out = z->ro_scratch; la.newton = 1;
     la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
+    if (int rc = ensure_exchange(c, z)) return rc;
+    la.xch = z->lu_xch;
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipEventRecord(c->e0, st));

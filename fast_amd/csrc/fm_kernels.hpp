@@ -41,6 +41,7 @@ struct LuArgs {
     // error word of the persistent tiles
     double* dws;
     int* sync;
+    double* xch;                  // pipelined schedule: tile-boundary exchange buffer
     int schedule;                 // LU_BLOCKS or LU_PIPELINE
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1 };
@@ -86,6 +87,8 @@ void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std:
 void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);
+// doubles of the tile-boundary exchange buffer the pipelined sweeps of an LU box need
+long lu_exchange_size(int ni, int nj, int nk);
 void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st);
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);
 void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st);

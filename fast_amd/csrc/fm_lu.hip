@@ -341,10 +341,12 @@ __global__ __launch_bounds__(256) void k_lu_block_list(const LuZone* __restrict_
 // producers by about eight steps and the whole sweep takes about ni + nj + nk steps instead
 // of one launch per block diagonal.
 //
-// Cross-tile values travel through global memory and validate themselves: before a sweep the
-// output box is filled with a signalling-NaN pattern that arithmetic never produces; results
-// are stored with agent-scope (L2-coherent) stores, and a consumer re-reads a neighbour value
-// with agent-scope loads until it is no longer the pattern.  Every wait is bounded: past the
+// Cross-tile values travel through a small exchange buffer and validate themselves: before a
+// sweep the buffer is filled with a signalling-NaN pattern that arithmetic never produces; the
+// lanes on a tile's last j (k) line store their results there with agent-scope stores (seen by
+// every XCD), and the consumer tile re-reads them with agent-scope loads until they are no longer
+// the pattern.  Every other result is an ordinary store: only 2 of the 15 lines of a tile cross
+// to another tile, so the write-through traffic is small.  Every wait is bounded: past the
 // bound, or once any wave has given up, the error word is set and the kernel runs to its end
 // with whatever it read, so the grid always drains; the host then reports the failure.
 // Tiles are taken from a counter in dependency order (by j-tile + k-tile), so a tile only
@@ -404,9 +406,18 @@ __device__ __forceinline__ void lu_settle(double* v, int n, const double* base, 
     }
 }
 
+// Exchange layout: for the j boundary of tile row tj, value e of cell (oi, ok) (sweep offsets)
+// at xj[e * NXJ + (tj * nk + ok) * ni + oi]; for the k boundary, value e of (oi, oj) at
+// xk[e * NXK + (tk * nj + oj) * ni + oi], e = 0..4 the result, 5..8 the Newton-updated
+// u, v, w, T of an upper sweep in place.
+long lu_exchange_size(int ni, int nj, int nk) {
+    long ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    return 5 * ntj * nk * (long)ni + 9 * ntk * nj * (long)ni;
+}
+
 template <bool CART, bool LOWER, int P>
 __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __restrict__ in,
-                                                double* res, int ntj, int ntk) {
+                                                double* __restrict__ res, int ntj, int ntk) {
     __shared__ double ex[2][2][5][64];
     __shared__ int s_tile;
     int* ticket = a.sync;
@@ -428,6 +439,13 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
     const bool kalias = !LOWER && a.rop_k != a.rop;
     const double* kstate = kalias ? a.out : (LOWER ? rop : a.rop_k);
     const bool has_j = jj == 0 && oj > 0, has_k = kk == 0 && ok > 0;
+    // this lane's line feeds the next tile in j (k)
+    const bool put_j = jj == 7 && oj + 1 < nj, put_k = kk == 7 && ok + 1 < nk;
+    const long NXJ = (long)ntj * nk * ni, NXK = (long)ntk * nj * ni;
+    double* const xj = a.xch;
+    double* const xk = a.xch + 5 * NXJ;
+    const long xjin = ((long)(jt - 1) * nk + ok) * ni, xjout = ((long)jt * nk + ok) * ni;
+    const long xkin = ((long)(kt - 1) * nj + oj) * ni, xkout = ((long)kt * nj + oj) * ni;
     const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
     const double* tm[3] = {z.ti, z.tj, z.tk};
     const long minc[3] = {z.mi, z.mj, z.mk};
@@ -452,7 +470,7 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
         if (has_j) {
             const long l1 = IDX(z, i, j + sgn, k);
 #pragma unroll
-            for (int e = 0; e < 5; e++) q.rj[e] = lu_ldc(res + l1 + e * nd);
+            for (int e = 0; e < 5; e++) q.rj[e] = lu_ldc(xj + xjin + ii + e * NXJ);
 #pragma unroll
             for (int e = 0; e < 4; e++) q.wj[e] = rop[l1 + (e + 1) * nd];
             q.cj = z.coe[l1 + 2 * nd];
@@ -465,9 +483,9 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
         if (has_k) {
             const long l1 = IDX(z, i, j, k + sgn);
 #pragma unroll
-            for (int e = 0; e < 5; e++) q.rk[e] = lu_ldc(res + l1 + e * nd);
+            for (int e = 0; e < 5; e++) q.rk[e] = lu_ldc(xk + xkin + ii + e * NXK);
 #pragma unroll
-            for (int e = 0; e < 4; e++) q.wk[e] = kalias ? lu_ldc(kstate + l1 + (e + 1) * nd) : kstate[l1 + (e + 1) * nd];
+            for (int e = 0; e < 4; e++) q.wk[e] = kalias ? lu_ldc(xk + xkin + ii + (5 + e) * NXK) : kstate[l1 + (e + 1) * nd];
             q.ck = z.coe[l1 + 3 * nd];
             if (!CART) {
                 const long lt1 = MTR(z, i, j, k + sgn);
@@ -500,8 +518,7 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
                 for (int e = 0; e < 5; e++) b[e] = pi[e];
             } else if (D == 1) {
                 if (jj == 0) {
-                    const long l1 = IDX(z, i, j + sgn, k);
-                    lu_settle(q.rj, 5, res + l1, nd, err);
+                    lu_settle(q.rj, 5, xj + xjin + ii, NXJ, err);
                     double tx = 0., ty = 0., tz = 0.;
                     if (CART) tx = tcj;
                     else { tx = 0.5 * (q.mj[1] + q.mj[0]); ty = 0.5 * (q.mj[3] + q.mj[2]); tz = 0.5 * (q.mj[5] + q.mj[4]); }
@@ -512,9 +529,8 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
                 }
             } else {
                 if (kk == 0) {
-                    const long l1 = IDX(z, i, j, k + sgn);
-                    lu_settle(q.rk, 5, res + l1, nd, err);
-                    if (kalias) lu_settle(q.wk, 4, kstate + l1 + nd, nd, err);
+                    lu_settle(q.rk, 5, xk + xkin + ii, NXK, err);
+                    if (kalias) lu_settle(q.wk, 4, xk + xkin + ii + 5 * NXK, NXK, err);
                     double tx = 0., ty = 0., tz = 0.;
                     if (CART) tx = tck;
                     else { tx = 0.5 * (q.mk[1] + q.mk[0]); ty = 0.5 * (q.mk[3] + q.mk[2]); tz = 0.5 * (q.mk[5] + q.mk[4]); }
@@ -529,7 +545,12 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
         }
         const double diag = 1. / cf4;
 #pragma unroll
-        for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; lu_stc(res + l + e * nd, x[e]); }
+        for (int e = 0; e < 5; e++) {
+            x[e] = x[e] * diag;
+            res[l + e * nd] = x[e];
+            if (put_j) lu_stc(xj + xjout + ii + e * NXJ, x[e]);
+            if (put_k) lu_stc(xk + xkout + ii + e * NXK, x[e]);
+        }
         double uk = u0, vk = v0, wk = w0, tk = t0;
         if (!LOWER && a.newton) {
             double* O = a.out;
@@ -539,12 +560,13 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
             const double u1 = (r0 * u0 + x[1]) * r_1, v1 = (r0 * v0 + x[2]) * r_1, w1 = (r0 * w0 + x[3]) * r_1;
             const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
             const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+            O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
             if (kalias) {
-                lu_stc(O + l, r1); lu_stc(O + l + nd, u1); lu_stc(O + l + 2 * nd, v1);
-                lu_stc(O + l + 3 * nd, w1); lu_stc(O + l + 4 * nd, t1);
                 uk = u1; vk = v1; wk = w1; tk = t1;
-            } else {
-                O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
+                if (put_k) {
+                    lu_stc(xk + xkout + ii + 5 * NXK, u1); lu_stc(xk + xkout + ii + 6 * NXK, v1);
+                    lu_stc(xk + xkout + ii + 7 * NXK, w1); lu_stc(xk + xkout + ii + 8 * NXK, t1);
+                }
             }
         }
         double tx = 0., ty = 0., tz = 0.;
@@ -581,19 +603,13 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
     }
 }
 
-// fill the 5 variables of a box with a 64-bit pattern (also zeroes the tile counter)
-__global__ void k_fill_box(KZ z, double* __restrict__ p, Box b, long long bits, int* sync) {
-    int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = b.k0 + blockIdx.y;
-    if (sync && t == 0 && blockIdx.y == 0) sync[0] = 0;
-    if (t >= ni * nj) return;
-    int i = b.i0 + t % ni, j = b.j0 + t / ni;
-    long nd = z.nd, l = IDX(z, i, j, k);
-    double v = __longlong_as_double(bits);
-#pragma unroll
-    for (int e = 0; e < 5; e++) p[l + e * nd] = v;
+// fill n doubles with a 64-bit pattern and zero the tile counter
+__global__ void k_fill_lin(double* __restrict__ p, long n, long long bits, int* sync) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sync && t == 0) sync[0] = 0;
+    if (t < n) p[t] = __longlong_as_double(bits);
 }
+
 
 // Copy the Newton-updated LU box back (in-place sub-iterations).
 __global__ void k_copy_box(KZ z, const double* __restrict__ src, double* __restrict__ dst, Box b) {
@@ -627,22 +643,21 @@ void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
 }  // namespace
 
 namespace {
-void fill_box(const KZ& z, double* p, const LuArgs& a, long long bits, int* sync, hipStream_t st) {
-    Box b = {a.i0, a.i1, a.j0, a.j1, a.k0, a.k1};
-    int n0 = b.i1 - b.i0 + 1, n1 = b.j1 - b.j0 + 1, n2 = b.k1 - b.k0 + 1;
-    hipLaunchKernelGGL(k_fill_box, dim3((n0 * n1 + 255) / 256, n2), dim3(256), 0, st, z, p, b, bits, sync);
+
+void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
 
 template <bool CART>
 void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
-    int nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    long nx = lu_exchange_size(ni, nj, nk);
     // lower: RHS (drodm) -> dW* (dws)
-    fill_box(z, a.dws, a, kLuPending, a.sync, st);
+    fill_exchange(a, nx, st);
     hipLaunchKernelGGL((k_lu_pipe<CART, true, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dw, a.dws, ntj, ntk);
     // upper: dW* -> dW (drodm), Newton update into out
-    fill_box(z, a.dw, a, kLuPending, a.sync, st);
-    if (a.rop_k != a.rop) fill_box(z, a.out, a, kLuPending, nullptr, st);
+    fill_exchange(a, nx, st);
     hipLaunchKernelGGL((k_lu_pipe<CART, false, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dws, a.dw, ntj, ntk);
 }
 }  // namespace
