@@ -54,6 +54,8 @@ def parse():
                          "(4x4x4 tiling, sizes from default_rng(1) in [16, 96]) joined by transfers")
     ap.add_argument("--autotune", action="store_true",
                     help="time the interchangeable schedules first and keep the fastest (fast_amd.tune)")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="1 replays captured HIP graphs of the time step (FM_OPT_GRAPH)")
     ap.add_argument("--zone-batch", type=int, default=1, choices=[0, 1],
                     help="zones64: 1 batches the LU-SGS of all zones per block diagonal, 0 loops")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -147,6 +149,7 @@ def run_other(a):
     from fast_amd import _lib, cases
     ctx = _lib.Context(0)
     ctx.set_option(_lib.OPT_ZONE_BATCH, a.zone_batch)
+    ctx.set_option(_lib.OPT_GRAPH, a.graph)
     if a.workload == "channel":
         zones, racs = [cases.channel_zone(257, 129, 33)], []
         desc = ("channel with a bump 257x129x33 (3dfull, jittered), laminar AUSM+ o3, implicit LU-SGS, "
@@ -183,7 +186,7 @@ def run_other(a):
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic", "config": {"workload": desc, "cells": ncell, "zones": len(ids),
-                                           "zone_batch": a.zone_batch},
+                                           "zone_batch": a.zone_batch, "graph": a.graph},
            "event_ms_per_step": round(ms_ev / a.steps, 3),
            "iteration_hbm_frac": round(value * 1e6 * bpc["iteration"] / 1e9 / HBM_PEAK_GBS, 4)}
     if len(ids) == 1:
@@ -218,6 +221,7 @@ def main():
     ctx.set_option(_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE if a.lu == "pipeline" else _lib.LU_BLOCKS)
     ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
                    _lib.RES_FACES if a.residual == "faces" else _lib.RES_CELLS)
+    ctx.set_option(_lib.OPT_GRAPH, a.graph)
     zid = ctx.add_zone(z)
     ctx.apply_bc(zid, 0)
     buf = np.zeros_like(z.ro)
@@ -269,7 +273,7 @@ def main():
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
            "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
                       % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic",
-                      "lu_schedule": a.lu, "residual_schedule": a.residual,
+                      "lu_schedule": a.lu, "residual_schedule": a.residual, "graph": a.graph,
                       "autotuned": None if tuned is None else {k: {n: round(t, 4) for n, t in v[1].items()}
                                                                for k, v in tuned.items()}},
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),

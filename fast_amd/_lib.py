@@ -52,6 +52,7 @@ LU_BLOCKS, LU_PIPELINE = 0, 1
 OPT_RESIDUAL_SCHEDULE = 2
 RES_CELLS, RES_FACES = 0, 1
 OPT_ZONE_BATCH = 3
+OPT_GRAPH = 4
 
 
 class FastMIError(RuntimeError):

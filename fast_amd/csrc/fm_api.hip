@@ -74,6 +74,18 @@ struct fm_ctx {
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
     int lu_batch;            // batch the LU of several zones (default on)
+    // time steps captured as HIP graphs (FM_OPT_GRAPH): one per distinct configuration
+    struct GraphEntry {
+        std::vector<const void*> ptrs;  // every zone's three level pointers
+        int first_it, verif, n0, n1;
+        long epoch;
+        int state;                      // 1: ran once eagerly, 2: captured
+        hipGraphExec_t exec;
+        std::vector<std::pair<long, LuZone*>> tabs;   // LU tables per (sub-iteration, mesh kind)
+    };
+    int graph_on;
+    long graph_epoch;                   // bumped by anything a captured kernel argument depends on
+    std::vector<GraphEntry*> graphs;
 };
 
 extern "C" const char* fm_last_error(void) { return g_err.c_str(); }
@@ -113,6 +125,11 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
     for (Zone* z : c->zones) free_zone(z);
     if (c->tr_pts) (void)hipFree(c->tr_pts);
     for (auto& l : c->lu_lists) if (l.work) (void)hipFree(l.work);
+    for (auto* g : c->graphs) {
+        if (g->state == 2) (void)hipGraphExecDestroy(g->exec);
+        for (auto& t : g->tabs) (void)hipFree(t.second);
+        delete g;
+    }
     if (c->lu_tab) (void)hipFree(c->lu_tab);
     for (int q = 0; q < 2; q++) {
         if (c->lu_host[q]) (void)hipHostFree(c->lu_host[q]);
@@ -271,6 +288,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     HIPCHK(hipMemcpy(yd, y, sizeof(double) * nxyz, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(zd, zc, sizeof(double) * nxyz, hipMemcpyHostToDevice));
     c->zones.push_back(z);
+    c->graph_epoch++;
     if (zone_id) *zone_id = (int)c->zones.size() - 1;
     return FM_OK;
 }
@@ -285,6 +303,7 @@ extern "C" int fm_zone_set_param_real(fm_ctx* c, int zone, const double* pr, int
     if (!z || !pr || n_real < 73) return fail(FM_E_ARG, "fm_zone_set_param_real: bad arguments");
     z->pr.assign(pr, pr + n_real);
     fill_kz_params(z);
+    c->graph_epoch++;        // captured steps hold the old values in their kernel arguments
     return FM_OK;
 }
 
@@ -662,6 +681,7 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
         HIPCHK(hipMemcpy(c->tr_pts, h.data(), h.size() * sizeof(TransferPoint), hipMemcpyHostToDevice));
         c->tr_n = (long)h.size();
     }
+    c->graph_epoch++;
     return FM_OK;
 }
 
@@ -689,13 +709,11 @@ extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
     return FM_OK;
 }
 
-// computePT.cpp:534-679 for the implemented options: nit_c steps; in each step the
-// sub-iterations nstep_deb..nstep_fin of every zone; the last implicit sub-iteration only
-// computes the residual (lexit_lu) and only on verification steps; time levels rotate.
 // LU-SGS of several zones, diagonal-batched: one launch per block diagonal for all the zones of
 // one mesh kind (3dcart / 3dfull), lower sweeps then upper sweeps, then the copy-back of the
 // zones updated in place.  Same blocks and the same per-block work as the per-zone sweeps.
-static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector<LuArgs>& las) {
+static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector<LuArgs>& las,
+                      fm_ctx::GraphEntry* ge = nullptr, int nstep = 0, bool capture = false) {
     hipStream_t st = c->stream;
     for (int cart = 0; cart < 2; cart++) {
         std::vector<Box> boxes;
@@ -728,6 +746,22 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
             L = &c->lu_lists.back();
         }
         if (!L->work) continue;
+        if (ge) {
+            // a captured step reads a table of its own, written once before its capture
+            long key = 2L * nstep + cart;
+            LuZone* d = nullptr;
+            for (auto& t : ge->tabs) if (t.first == key) d = t.second;
+            if (!d) {
+                if (capture) return fail(FM_E_HIP, "graph capture: LU table missing");
+                HIPCHK(hipMalloc(&d, tab.size() * sizeof(LuZone)));
+                ge->tabs.push_back({key, d});
+            }
+            if (!capture) HIPCHK(hipMemcpy(d, tab.data(), tab.size() * sizeof(LuZone), hipMemcpyHostToDevice));
+            launch_lu_list(cart == 1, true, d, L->work, L->offs, st);
+            launch_lu_list(cart == 1, false, d, L->work, L->offs, st);
+            HIPCHK(hipGetLastError());
+            continue;
+        }
         if ((int)tab.size() > c->lu_tab_cap) {
             HIPCHK(hipStreamSynchronize(st));
             if (c->lu_tab) HIPCHK(hipFree(c->lu_tab));
@@ -763,50 +797,103 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
 // Within a sub-iteration every zone first gets its right-hand side and LU-SGS update, then
 // the transfers fill the ghost cells shared with other zones, then every zone's physical
 // boundary conditions and corners are applied (gsdr3.cpp:205-444).
-static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
+// One time step of every zone: sub-iterations n0..n1 (gsdr3 per sub-iteration).
+static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx::GraphEntry* ge, bool capture) {
     const size_t nz = c->zones.size();
     // several zones: their LU-SGS sweeps share one launch per block diagonal
     const bool batch = nz > 1 && c->lu_batch && c->lu_schedule == LU_BLOCKS;
     std::vector<int> lexit(nz);
     std::vector<double*> cl(nz);
+    for (int nstep = n0; nstep <= n1; nstep++) {
+        int any = 0;
+        std::vector<Zone*> bz;
+        std::vector<LuArgs> bl;
+        for (size_t n = 0; n < nz; n++) {
+            Zone* z = c->zones[n];
+            int implicit = z->k.itypcp <= 1;
+            int nssiter = z->k.nssiter;
+            // souszones_list.cpp:44-84: a verification step computes residuals (and the
+            // CFL monitor) at every sub-iteration and skips LU-SGS at the last one
+            int lssiter_verif = verif, lexit_lu = 0;
+            lexit[n] = -1;
+            cl[n] = level_cl(z, nstep);
+            if (implicit && nstep == nssiter) {
+                if (!verif) continue;           // skip (computePT.cpp:581-582)
+                lexit_lu = 1;
+            }
+            LuArgs la;
+            la.i0 = 1; la.i1 = 0;
+            int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 3, batch ? &la : nullptr);
+            if (rc) return rc;
+            if (batch && implicit && lexit_lu == 0 && la.i1 >= la.i0) { bz.push_back(z); bl.push_back(la); }
+            lexit[n] = lexit_lu;
+            any |= lexit_lu == 0;
+        }
+        if (!bz.empty()) {
+            int rc = lu_batched(c, bz, bl, ge, nstep, capture);
+            if (rc) return rc;
+        }
+        if (!any) continue;
+        int rc = transfer(c, cl.data());
+        if (rc) return rc;
+        for (size_t n = 0; n < nz; n++)
+            if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
+        HIPCHK(hipGetLastError());
+    }
+    return FM_OK;
+}
+
+// computePT.cpp:534-679 for the implemented options: nit_c steps; in each step the
+// sub-iterations nstep_deb..nstep_fin of every zone; the last implicit sub-iteration only
+// computes the residual (lexit_lu) and only on verification steps; time levels rotate.
+// Within a sub-iteration every zone first gets its right-hand side and LU-SGS update, then
+// the transfers fill the ghost cells shared with other zones, then every zone's physical
+// boundary conditions and corners are applied (gsdr3.cpp:205-444).
+//
+// With FM_OPT_GRAPH a step is run eagerly the first time its configuration (level pointers,
+// first_it, verification, sub-iteration range) appears, captured into a HIP graph the second
+// time, and replayed from then on.  The levels rotate with period 3, so a run settles on a
+// few graphs.
+static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
     for (int it = 0; it < nit_c; it++) {
         int nitrun_loc = nitrun * nit_c + it;
         int verif = modulo_verif > 0 && (nitrun_loc % modulo_verif == 0);
-        for (int nstep = nstep_deb; nstep <= nstep_fin; nstep++) {
-            int any = 0;
-            std::vector<Zone*> bz;
-            std::vector<LuArgs> bl;
-            for (size_t n = 0; n < nz; n++) {
-                Zone* z = c->zones[n];
-                int implicit = z->k.itypcp <= 1;
-                int nssiter = z->k.nssiter;
-                // souszones_list.cpp:44-84: a verification step computes residuals (and the
-                // CFL monitor) at every sub-iteration and skips LU-SGS at the last one
-                int lssiter_verif = verif, lexit_lu = 0;
-                lexit[n] = -1;
-                cl[n] = level_cl(z, nstep);
-                if (implicit && nstep == nssiter) {
-                    if (!verif) continue;           // skip (computePT.cpp:581-582)
-                    lexit_lu = 1;
-                }
-                LuArgs la;
-                la.i0 = 1; la.i1 = 0;
-                int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 3, batch ? &la : nullptr);
-                if (rc) return rc;
-                if (batch && implicit && lexit_lu == 0 && la.i1 >= la.i0) { bz.push_back(z); bl.push_back(la); }
-                lexit[n] = lexit_lu;
-                any |= lexit_lu == 0;
-            }
-            if (!bz.empty()) {
-                int rc = lu_batched(c, bz, bl);
-                if (rc) return rc;
-            }
-            if (!any) continue;
-            int rc = transfer(c, cl.data());
+        if (!c->graph_on || c->lu_schedule != LU_BLOCKS) {
+            int rc = time_step(c, first_it, verif, nstep_deb, nstep_fin, nullptr, false);
             if (rc) return rc;
-            for (size_t n = 0; n < nz; n++)
-                if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
-            HIPCHK(hipGetLastError());
+        } else {
+            std::vector<const void*> ptrs;
+            for (Zone* z : c->zones) for (int q = 0; q < 3; q++) ptrs.push_back(z->ro[q]);
+            fm_ctx::GraphEntry* ge = nullptr;
+            for (auto* g : c->graphs)
+                if (g->ptrs == ptrs && g->first_it == first_it && g->verif == verif && g->n0 == nstep_deb &&
+                    g->n1 == nstep_fin && g->epoch == c->graph_epoch) { ge = g; break; }
+            if (!ge) {
+                ge = new fm_ctx::GraphEntry();
+                ge->ptrs = ptrs; ge->first_it = first_it; ge->verif = verif; ge->n0 = nstep_deb;
+                ge->n1 = nstep_fin; ge->epoch = c->graph_epoch; ge->state = 0;
+                c->graphs.push_back(ge);
+            }
+            if (ge->state == 0) {
+                int rc = time_step(c, first_it, verif, nstep_deb, nstep_fin, ge, false);
+                if (rc) return rc;
+                ge->state = 1;
+            } else {
+                if (ge->state == 1) {
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                    int rc = time_step(c, first_it, verif, nstep_deb, nstep_fin, ge, true);
+                    hipGraph_t g;
+                    hipError_t e = hipStreamEndCapture(c->stream, &g);
+                    if (rc) return rc;
+                    if (e != hipSuccess) return fail(FM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+                    e = hipGraphInstantiate(&ge->exec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (e != hipSuccess) return fail(FM_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+                    ge->state = 2;
+                }
+                HIPCHK(hipGraphLaunch(ge->exec, c->stream));
+            }
         }
         first_it = 1;
         for (Zone* z : c->zones) {
@@ -861,16 +948,25 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
     if (option == FM_OPT_LU_SCHEDULE) {
         if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE) return fail(FM_E_ARG, "LU schedule %d", value);
         c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : LU_BLOCKS;
+        c->graph_epoch++;
         return FM_OK;
     }
     if (option == FM_OPT_RESIDUAL_SCHEDULE) {
         if (value != FM_RES_CELLS && value != FM_RES_FACES) return fail(FM_E_ARG, "residual schedule %d", value);
         c->res_schedule = value == FM_RES_FACES ? RES_FACES : RES_CELLS;
+        c->graph_epoch++;
         return FM_OK;
     }
     if (option == FM_OPT_ZONE_BATCH) {
         if (value != 0 && value != 1) return fail(FM_E_ARG, "zone batch %d", value);
         c->lu_batch = value;
+        c->graph_epoch++;
+        return FM_OK;
+    }
+    if (option == FM_OPT_GRAPH) {
+        if (value != 0 && value != 1) return fail(FM_E_ARG, "graph %d", value);
+        c->graph_on = value;
+        c->graph_epoch++;
         return FM_OK;
     }
     return fail(FM_E_ARG, "unknown option %d", option);

@@ -212,3 +212,48 @@ def test_gpu_zone_batch_equals_zone_loop(mode):
         for za, zb in zip(a, b):
             assert np.array_equal(za.ro, zb.ro), za.name
             assert np.array_equal(za.ro_m1, zb.ro_m1), za.name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["tiles", "single"])
+def test_gpu_graph_replays_equal_eager(case):
+    """Time steps captured in HIP graphs: every replay (one step per call, with the state read
+    back and compared in between) equals the eagerly launched run bit for bit; the three level
+    rotations and the verification steps each get their own graph."""
+    from fast_amd import _lib
+    num = H.numerics("ausmpred", "bdf")
+
+    def make():
+        if case == "tiles":
+            zones, pairs, _ = cases.tiled_box((9, 12), (10, 7), (8,), num, flow="lamin",
+                                              periodic=(True, False, True), bcs_outer="BCWallViscous", seed=6)
+            return zones, connect.build(zones, pairs)
+        return [cases.tgv_zone((20, 14, 11), num, perturb=0.03)], []
+
+    def start(graph):
+        zones, racs = make()
+        ctx = _lib.Context(0)
+        ctx.set_option(_lib.OPT_GRAPH, graph)
+        ids = [ctx.add_zone(z) for z in zones]
+        ctx.set_transfers(racs)
+        ctx.transfer(0)
+        for i, z in zip(ids, zones):
+            ctx.apply_bc(i, 0)
+            t = np.zeros_like(z.ro)
+            ctx.get(i, _lib.FIELD_RO, t)
+            ctx.set(i, _lib.FIELD_RO_M1, t); ctx.set(i, _lib.FIELD_RO_P1, t)
+        return ctx, ids, zones
+
+    ce, ie, ze = start(0)
+    cg, ig, zg = start(1)
+    nsub = int(ze[0].param_int[P.NSSITER])
+    for step in range(9):
+        for ctx, ids in ((ce, ie), (cg, ig)):
+            ctx.compute(step, 1, nsub, nit_c=1, modulo_verif=3, first_it=0 if step == 0 else 1)
+            ctx.sync()
+        for i, j in zip(ie, ig):
+            for f in (_lib.FIELD_RO, _lib.FIELD_RO_M1):
+                ta = np.zeros_like(ze[ie.index(i)].ro); tb = np.zeros_like(ta)
+                ce.get(i, f, ta); cg.get(j, f, tb)
+                assert np.array_equal(ta, tb), (step, i, f)
+    ce.close(); cg.close()
