@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 #   coefficients (read W 40, write mu 8 + coe 40), counted separately     =  88
 BYTES_PER_CELL = {"coe": 88.0, "residual": 120.0, "lu": 240.0, "bc": 0.0, "iteration": 400.0}
 KERNELS = {"coe": "k_coe", "bc": "k_bc_face",
-           "residual": {"cells": "k_residual", "faces": "k_residual_faces"},
+           "residual": {"cells": "k_residual", "faces": "k_residual_faces", "kcol": "k_residual_kcol"},
            "lu": {"blocks": "k_lu_block (lower + upper sweeps, one launch per block diagonal)",
                   "pipeline": "k_lu_pipe (lower + upper sweeps)"}}
 
@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phase-reps", type=int, default=5)
     ap.add_argument("--lu", choices=["pipeline", "blocks"], default="blocks")
-    ap.add_argument("--residual", choices=["cells", "faces"], default="cells")
+    ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="cells")
     ap.add_argument("--workload", choices=["tgv256", "channel", "zones64"], default="tgv256",
                     help="tgv256: the headline line; channel: 257x129x33 curvilinear channel with a "
                          "bump (3dfull, local time step, CFL 10); zones64: 64 abutting blocks "
@@ -220,7 +220,7 @@ def main():
     ctx = _lib.Context(0)
     ctx.set_option(_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE if a.lu == "pipeline" else _lib.LU_BLOCKS)
     ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
-                   _lib.RES_FACES if a.residual == "faces" else _lib.RES_CELLS)
+                   {"cells": _lib.RES_CELLS, "faces": _lib.RES_FACES, "kcol": _lib.RES_KCOL}[a.residual])
     ctx.set_option(_lib.OPT_GRAPH, a.graph)
     zid = ctx.add_zone(z)
     ctx.apply_bc(zid, 0)

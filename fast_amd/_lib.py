@@ -50,7 +50,7 @@ PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
 OPT_LU_SCHEDULE = 1
 LU_BLOCKS, LU_PIPELINE = 0, 1
 OPT_RESIDUAL_SCHEDULE = 2
-RES_CELLS, RES_FACES = 0, 1
+RES_CELLS, RES_FACES, RES_KCOL = 0, 1, 2
 OPT_ZONE_BATCH = 3
 OPT_GRAPH = 4
 

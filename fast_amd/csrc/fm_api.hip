@@ -60,7 +60,7 @@ struct Zone {
 struct fm_ctx {
     int device;
     int lu_schedule;         // LU_PIPELINE or LU_BLOCKS
-    int res_schedule;        // RES_FACES or RES_CELLS
+    int res_schedule;        // RES_CELLS, RES_FACES or RES_KCOL
     hipStream_t stream;
     std::vector<Zone*> zones;
     hipEvent_t e0, e1;
@@ -952,8 +952,9 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
         return FM_OK;
     }
     if (option == FM_OPT_RESIDUAL_SCHEDULE) {
-        if (value != FM_RES_CELLS && value != FM_RES_FACES) return fail(FM_E_ARG, "residual schedule %d", value);
-        c->res_schedule = value == FM_RES_FACES ? RES_FACES : RES_CELLS;
+        if (value != FM_RES_CELLS && value != FM_RES_FACES && value != FM_RES_KCOL)
+            return fail(FM_E_ARG, "residual schedule %d", value);
+        c->res_schedule = value == FM_RES_FACES ? RES_FACES : value == FM_RES_KCOL ? RES_KCOL : RES_CELLS;
         c->graph_epoch++;
         return FM_OK;
     }

@@ -378,6 +378,53 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES
     res_update(z, a, rop, l, acc);
 }
 
+// Residual, a column of RES_KCOL_N cells up k per thread: the k face between two cells of the
+// column is evaluated once (as the top face of the lower cell and the bottom face of the upper
+// one), and the planes the k stencils share stay in the cache of the block.  Per cell the sum
+// order is the reference's, so the result is the one of k_residual.
+template <int KFLU, int SLOPE, bool LAMIN, bool CART>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES_WAVES))) k_residual_kcol(KZ z, RhsArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k0 = blockIdx.z * RES_KCOL_N + 1;
+    if (i > z.ni || j > z.nj || k0 > z.nk) return;
+    const double* __restrict__ rop = a.rop;
+    long nd = z.nd;
+    FluxConst f = flux_const(z);
+    double fk[5], flu[5], acc[5];
+    face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k0, fk);
+    const int k1 = k0 + RES_KCOL_N - 1 < z.nk ? k0 + RES_KCOL_N - 1 : z.nk;
+    for (int k = k0; k <= k1; k++) {
+        long l = IDX(z, i, j, k);
+        if (a.zero_init) {
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = 0.;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = z.drodm[l + e * nd] * a.coefH;
+        }
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + fk[e];
+        face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j, k, flu);
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + flu[e];
+        face_flux<KFLU, SLOPE, LAMIN, CART, 0>(f, z, rop, i, j, k, flu);
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + flu[e];
+        face_flux<KFLU, SLOPE, LAMIN, CART, 0>(f, z, rop, i + 1, j, k, flu);
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j + 1, k, flu);
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k + 1, fk);
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk[e];
+        if (KFLU == 5 && a.ncorr > 0) wall_correction<SLOPE, CART>(f, z, rop, a, i, j, k, acc);
+        res_update(z, a, rop, l, acc);
+    }
+}
+
 // Residual with every face flux evaluated once.  Block: 64 i-lanes x 5 j-rows of threads
 // marching RES_KC planes in k.  Lane tx of rows 0..3 evaluates the i-face at i0 + tx (its cell
 // i0 + tx takes face i + 1 from lane tx + 1, so a block covers 63 cells); row r evaluates the
@@ -942,6 +989,18 @@ static void res_lc(const KZ& z, const RhsArgs& a, bool lamin, dim3 g, hipStream_
 }
 
 template <int KFLU, int SLOPE>
+static void res_kcol(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st) {
+    dim3 g((z.ni + 63) / 64, (z.nj + 3) / 4, (z.nk + RES_KCOL_N - 1) / RES_KCOL_N);
+    if (lamin) {
+        if (z.typ == 2) hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, true, true>), g, blk2, 0, st, z, a);
+        else hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, true, false>), g, blk2, 0, st, z, a);
+    } else {
+        if (z.typ == 2) hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, false, true>), g, blk2, 0, st, z, a);
+        else hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, false, false>), g, blk2, 0, st, z, a);
+    }
+}
+
+template <int KFLU, int SLOPE>
 static void res_faces(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st) {
     dim3 g((z.ni + 62) / 63, (z.nj + 3) / 4, (z.nk + RES_KC - 1) / RES_KC), b(64, 5);
     if (lamin) {
@@ -955,6 +1014,17 @@ static void res_faces(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st)
 
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st) {
     bool lamin = z.iflow == 2;
+    if (a.schedule == RES_KCOL) {
+        if (z.kflu == 5) {
+            if (z.slope == 1) res_kcol<5, 1>(z, a, lamin, st);
+            else if (z.slope == 2) res_kcol<5, 2>(z, a, lamin, st);
+            else if (z.slope == 3) res_kcol<5, 3>(z, a, lamin, st);
+            else return -1;
+        } else if (z.kflu == 1 && z.slope == 2) res_kcol<1, 2>(z, a, lamin, st);
+        else if (z.kflu == 2 && z.slope == 2) res_kcol<2, 2>(z, a, lamin, st);
+        else return -1;
+        return 0;
+    }
     if (a.schedule == RES_FACES && a.ncorr == 0) {
         if (z.kflu == 5) {
             if (z.slope == 1) res_faces<5, 1>(z, a, lamin, st);

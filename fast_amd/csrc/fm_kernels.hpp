@@ -23,12 +23,14 @@ struct RhsArgs {
     double coefH, coefW;    // RK coefficients
     int implicit, nitcfg;
     double c1, c2, c3;      // BDF coefficients
-    int schedule;           // RES_FACES (each face once) or RES_CELLS (six faces per cell)
+    int schedule;           // RES_CELLS (six faces per cell), RES_FACES (each face once) or
+                            // RES_KCOL (a thread walks RES_KCOL_N cells up k, k faces once)
     int ncorr;              // Roe: wall faces whose Euler flux is replaced, in BC order
     WallCorr corr[kMaxWallCorr];
     double ck_vent, c_ale;  // shape_tab_mtr.for mesh-motion factors of the wall flux
 };
-enum { RES_CELLS = 0, RES_FACES = 1 };
+enum { RES_CELLS = 0, RES_FACES = 1, RES_KCOL = 2 };
+constexpr int RES_KCOL_N = 4;
 
 struct LuArgs {
     int i0, i1, j0, j1, k0, k1;   // LU range (shift_lu)

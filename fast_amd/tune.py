@@ -2,13 +2,14 @@
 zones and keeps the fastest.  Every schedule gives the same results bit for bit, so the choice
 only affects speed.
 
-Knobs (fm_ctx_set_option): the residual schedule (six faces per cell / every face once) and
+Knobs (fm_ctx_set_option): the residual schedule (six faces per cell / every face once /
+columns of cells up k) and
 the LU-SGS schedule (block diagonals / persistent pipelined tiles)."""
 from . import _lib
 
 KNOBS = (
     ("residual", _lib.OPT_RESIDUAL_SCHEDULE, _lib.PHASE_RESIDUAL,
-     (("cells", _lib.RES_CELLS), ("faces", _lib.RES_FACES))),
+     (("cells", _lib.RES_CELLS), ("faces", _lib.RES_FACES), ("kcol", _lib.RES_KCOL))),
     ("lu", _lib.OPT_LU_SCHEDULE, _lib.PHASE_LU,
      (("blocks", _lib.LU_BLOCKS), ("pipeline", _lib.LU_PIPELINE))),
 )

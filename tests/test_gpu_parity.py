@@ -179,6 +179,27 @@ def test_roe_wall_flux_correction(ctxmod, orc, case, mode, scheme):
     assert_same(zo, zg, ghosts=True)
 
 
+@pytest.mark.parametrize("schedule", ["faces", "kcol"])
+def test_roe_wall_correction_other_schedules(ctxmod, orc, schedule):
+    """The wall correction under the other residual schedules (faces falls back to cells)."""
+    num = H.numerics("roe_kap", "bdf")
+    mk = lambda: cases.channel_zone(18, 12, 6, num)
+    zo, zg = mk(), mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    orc.substep(zo, 1, first_it=0, stop=1)
+    ctx = ctxmod.Context(0)
+    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+    ctx.substep(zid, 1, first_it=0, stop=1)
+    ctx.pull(zid, zg)
+    ctx.close()
+    assert np.array_equal(zo.interior(zo.drodm), zg.interior(zg.drodm))
+
+
 def test_norms_match_residual_blocks(ctxmod, orc):
     """GPU L2/Linf against the oracle's per-block sums (cprdu3s1.for), relative 1e-12."""
     mk = lambda: cases.tgv_zone((24, 20, 16), H.numerics("roe_min", "bdf"), perturb=0.02)
@@ -251,12 +272,16 @@ def test_cfl_monitor(ctxmod, orc, mesh, flow):
     np.testing.assert_allclose(got[2], cfl[2] / n, rtol=1e-12)
 
 
-@pytest.mark.parametrize("schedule", ["cells", "faces"])
+RES = {"cells": 0, "faces": 1, "kcol": 2}
+
+
+@pytest.mark.parametrize("schedule", ["cells", "faces", "kcol"])
 @pytest.mark.parametrize("mesh", ["cart", "full"])
 @pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
 def test_residual_schedules(ctxmod, orc, schedule, mesh, scheme):
-    """Face balance with six faces per cell and with every face once (63-cell i-chunks,
-    4-row j-chunks, 32-plane k-chunks: these sizes straddle all three chunk boundaries)."""
+    """Face balance with six faces per cell, with every face once (63-cell i-chunks, 4-row
+    j-chunks, 32-plane k-chunks) and with 4-cell k columns: the sizes straddle every chunk
+    boundary (35 and 34 planes are not multiples of 4 or 32)."""
     num = H.numerics(scheme, "bdf")
     if mesh == "cart":
         mk = lambda: cases.tgv_zone((70, 9, 35), num, perturb=0.02)
@@ -267,7 +292,7 @@ def test_residual_schedules(ctxmod, orc, schedule, mesh, scheme):
     zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
     orc.substep(zo, 1, first_it=0, stop=1)
     ctx = ctxmod.Context(0)
-    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, ctxmod.RES_CELLS if schedule == "cells" else ctxmod.RES_FACES)
+    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
