@@ -47,6 +47,7 @@ def test_autotune_picks_the_fastest_schedule():
             self.opts = {}
             self.cost = {(_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_CELLS): 2.0,
                          (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_FACES): 1.5,
+                         (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_KCOL): 1.7,
                          (_lib.OPT_LU_SCHEDULE, _lib.LU_BLOCKS): 3.0,
                          (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0}
 
