@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phase-reps", type=int, default=5)
     ap.add_argument("--lu", choices=["pipeline", "blocks"], default="blocks")
-    ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="cells")
+    ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="kcol")
     ap.add_argument("--workload", choices=["tgv256", "channel", "zones64"], default="tgv256",
                     help="tgv256: the headline line; channel: 257x129x33 curvilinear channel with a "
                          "bump (3dfull, local time step, CFL 10); zones64: 64 abutting blocks "

@@ -96,7 +96,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     fm_ctx* c = new fm_ctx();
     c->device = device;
     c->lu_schedule = LU_BLOCKS;
-    c->res_schedule = RES_CELLS;
+    c->res_schedule = RES_KCOL;
     c->lu_batch = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }

@@ -138,9 +138,9 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_OPT_LU_SCHEDULE 1
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
-/* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS (default) evaluates the six faces of every cell;
+/* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
-   FM_RES_KCOL walks a column of 4 cells up k per thread, evaluating the k faces once. */
+   FM_RES_KCOL (default) walks a column of 4 cells up k per thread, evaluating the k faces once. */
 #define FM_OPT_RESIDUAL_SCHEDULE 2
 #define FM_RES_CELLS 0
 #define FM_RES_FACES 1
