@@ -737,7 +737,7 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
             fm_ctx::LuList nl;
             nl.cart = cart == 1; nl.boxes = boxes; nl.work = nullptr;
             std::vector<LuWork> w;
-            lu_work_list(boxes, w, nl.offs);
+            lu_work_list(cart == 1, boxes, w, nl.offs);
             if (!w.empty()) {
                 HIPCHK(hipMalloc(&nl.work, w.size() * sizeof(LuWork)));
                 HIPCHK(hipMemcpy(nl.work, w.data(), w.size() * sizeof(LuWork), hipMemcpyHostToDevice));

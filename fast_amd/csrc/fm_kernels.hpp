@@ -85,7 +85,7 @@ void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
 // diagonal-batched LU-SGS (blocks schedule) over several zones of the same mesh kind
 struct LuZone { KZ z; LuArgs a; };
 struct LuWork { int zone, bi, bj, bk; };
-void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs);
+void lu_work_list(bool cart, const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs);
 void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);

@@ -19,6 +19,17 @@
 // The off-diagonal products b = B(W_nb) dW_nb are formed by the cell that owns W_nb, exactly
 // as the reference forms them for the cell that consumes them (same operands, same order).
 #include "fm_kernels.hpp"
+
+// Cells per line segment of an LU block (its i extent), per mesh kind.  With 8, a Cartesian
+// block takes 80 KB of LDS (two blocks per CU) and a curvilinear one 118 KB; 16 Cartesian
+// cells make 128-byte row segments at 148 KB (one block per CU).
+#ifndef FM_LU_CI_CART
+#define FM_LU_CI_CART 8
+#endif
+#ifndef FM_LU_CI_FULL
+#define FM_LU_CI_FULL 8
+#endif
+template <bool CART> constexpr int lu_ci() { return CART ? FM_LU_CI_CART : FM_LU_CI_FULL; }
 #include <array>
 #include <vector>
 
@@ -626,8 +637,7 @@ __global__ void k_copy_box(KZ z, const double* __restrict__ src, double* __restr
 namespace {
 template <bool CART, bool LOWER>
 void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
-    // 8 cells per line keeps a block under 80 KB of LDS: two blocks per CU
-    constexpr int CI = 8;
+    constexpr int CI = lu_ci<CART>();
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int nbi = (ni + CI - 1) / CI, nbj = (nj + 7) / 8, nbk = (nk + 7) / 8;
     int ndiag = nbi + nbj + nbk - 2;
@@ -662,11 +672,11 @@ void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
 }
 }  // namespace
 
-// Block lists of a set of zones for the diagonal-batched sweeps: for every zone (LU box of
-// boxes[n]) its 8x8x8 blocks in sweep coordinates, grouped by diagonal; offs[d]..offs[d+1]
-// are the blocks of launch d.
-void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs) {
-    constexpr int CI = 8;
+// Block lists of a set of zones of one mesh kind for the diagonal-batched sweeps: for every zone
+// (LU box of boxes[n]) its CI x 8 x 8 blocks in sweep coordinates, grouped by diagonal;
+// offs[d]..offs[d+1] are the blocks of launch d.
+void lu_work_list(bool cart, const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs) {
+    const int CI = cart ? lu_ci<true>() : lu_ci<false>();
     int nd = 0;
     std::vector<std::array<int, 3>> nb(boxes.size());
     for (size_t n = 0; n < boxes.size(); n++) {
@@ -692,17 +702,17 @@ void lu_work_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, std:
 
 void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st) {
-    constexpr int CI = 8;
+    constexpr int CC = lu_ci<true>(), CF = lu_ci<false>();
     for (size_t d = 0; d + 1 < offs.size(); d++) {
         int n = offs[d + 1] - offs[d];
         if (n <= 0) continue;
         const LuWork* w = work + offs[d];
         if (cart) {
-            if (lower) hipLaunchKernelGGL((k_lu_block_list<true, true, CI>), dim3(n), dim3(256), 0, st, zones, w);
-            else hipLaunchKernelGGL((k_lu_block_list<true, false, CI>), dim3(n), dim3(256), 0, st, zones, w);
+            if (lower) hipLaunchKernelGGL((k_lu_block_list<true, true, CC>), dim3(n), dim3(256), 0, st, zones, w);
+            else hipLaunchKernelGGL((k_lu_block_list<true, false, CC>), dim3(n), dim3(256), 0, st, zones, w);
         } else {
-            if (lower) hipLaunchKernelGGL((k_lu_block_list<false, true, CI>), dim3(n), dim3(256), 0, st, zones, w);
-            else hipLaunchKernelGGL((k_lu_block_list<false, false, CI>), dim3(n), dim3(256), 0, st, zones, w);
+            if (lower) hipLaunchKernelGGL((k_lu_block_list<false, true, CF>), dim3(n), dim3(256), 0, st, zones, w);
+            else hipLaunchKernelGGL((k_lu_block_list<false, false, CF>), dim3(n), dim3(256), 0, st, zones, w);
         }
     }
 }
