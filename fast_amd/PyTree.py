@@ -312,6 +312,14 @@ def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list
             for i, v in enumerate(VARS):
                 arr = a[i * nd:(i + 1) * nd].reshape(tuple(int(c) for c in nc), order="F").copy(order="F")
                 Internal.createUniqueChild(fs, v + sfx, "DataArray_t", arr)
+    if Adjoint:
+        # FastC/PyTree.py:782-794 (createPrimVars with Adjoint): the dJ/dW fields, zero
+        for z, hz in HOOK["zones"]:
+            fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+            shape = tuple(int(c) for c in z[1][:, 1])
+            for v in ADJ_VARS["cd"] + ADJ_VARS["cl"]:
+                if Internal.getNodeFromName1(fs, v) is None:
+                    Internal.createUniqueChild(fs, v, "DataArray_t", np.zeros(shape, order="F"))
     HOOK["first_it"] = first_it
     HOOK["nitrun_done"] = 0
     return t, tc, metrics
@@ -520,6 +528,39 @@ def createStressNodes(t, BC=None, windows=None):
                 Internal.createUniqueChild(o, "Parameter_int", "DataArray_t", pe)
             no_z += 1
     return teff
+
+
+ADJ_VARS = {c: ["dp%sp_dp%s" % (c.upper(), v) for v in ("Density", "MomentumX", "MomentumY", "MomentumZ",
+                                                         "EnergyStagDens")] for c in ("cd", "cl")}
+
+
+def _compute_dpJ_dpW(t, teff, metrics, cosAoA, sinAoA, surfinv):
+    """Gradients of the pressure drag and lift coefficients of the slip-wall windows of teff with
+    respect to the conservative state, assigned into the fields dpCDp_dp* and dpCLp_dp* of the
+    wall cells (compute_dpJ_dpW.cpp:33-261 via PyTree.py:3719-3735; the fields are created by
+    warmup(..., Adjoint=True)).  AUSM+(P) o3 Euler zones."""
+    ctx = HOOK["ctx"]
+    for ze in Internal.getZones(teff):
+        pe = np.asarray(Internal.getNodeFromName1(Internal.getNodeFromName1(ze, ".Solver#ownData"),
+                                                  "Parameter_int")[1])
+        no = int(pe[EFF_NONZ])
+        z, hz = HOOK["zones"][no]
+        fs = Internal.getNodeFromName1(z, "FlowSolution#Centers")
+        arrs = {}
+        for c in ("cd", "cl"):
+            nodes = [Internal.getNodeFromName1(fs, v) for v in ADJ_VARS[c]]
+            if any(nd_ is None for nd_ in nodes):
+                raise ValueError("zone %s has no %s fields: call warmup(..., Adjoint=True)" % (z[0], ADJ_VARS[c][0]))
+            arrs[c] = (nodes, np.concatenate([nd_[1].ravel(order="F") for nd_ in nodes]))
+        win = [int(v) for v in pe[EFF_LOOP:EFF_LOOP + 6]]
+        ctx.dpj_dpw(HOOK["ids"][no], int(pe[EFF_IDIR]), win, cosAoA, sinAoA, surfinv,
+                    arrs["cd"][1], arrs["cl"][1])
+        for c in ("cd", "cl"):
+            nodes, flat = arrs[c]
+            nd = hz.nd
+            for e, node in enumerate(nodes):
+                node[1][...] = flat[e * nd:(e + 1) * nd].reshape(node[1].shape, order="F")
+    return None
 
 
 def _computeStress(t, teff, metrics, xyz_ref=(0., 0., 0.)):

@@ -45,6 +45,8 @@ SIGNATURES = {
     "fm_transfer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fm_wall_effort": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                       _dp, _dp, _dp]),
+    "fm_dpj_dpw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
 OPT_LU_SCHEDULE = 1
@@ -175,6 +177,16 @@ class Context:
                                       p.ctypes.data_as(_dp), eff.ctypes.data_as(_dp),
                                       fl.ctypes.data_as(_dp) if fields else None))
         return (eff, fl.reshape(21, nf)) if fields else eff
+
+    def dpj_dpw(self, zone, idir, window, cosa, sina, surfinv, dcd, dcl, level=0):
+        """Assign the gradients of the pressure drag / lift coefficients of a slip-wall window
+        into ``dcd``, ``dcl`` (5 x NDIMDX float64, in place) (compute_dpJ_dpW.cpp)."""
+        w = np.ascontiguousarray(window, dtype=np.int32)
+        for a in (dcd, dcl):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("dcd, dcl must be contiguous float64 arrays")
+        check(self.lib.fm_dpj_dpw(self.h, zone, level, idir, w.ctypes.data_as(_ip), float(cosa),
+                                  float(sina), float(surfinv), dptr(dcd), dptr(dcl)))
 
     def transfer(self, level=0):
         check(self.lib.fm_transfer(self.h, level))

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastmi.so")
-SOURCES = ["fm_kernels.hip", "fm_lu.hip", "fm_api.hip"]
+SOURCES = ["fm_kernels.hip", "fm_lu.hip", "fm_adjoint.hip", "fm_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FM_OFFLOAD_ARCH", "gfx950")
 

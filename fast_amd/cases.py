@@ -23,16 +23,18 @@ def tgv_state():
     return P.reference_state(mach=0.1, reynolds=1600.0, prandtl=0.72, gamma=1.4)
 
 
-def tgv_zone(n, numerics=None, perturb=0.0, seed=0, flow="lamin", name="tgv"):
+def tgv_zone(n, numerics=None, perturb=0.0, seed=0, flow="lamin", name="tgv", bcs=None):
     """TGV zone of n^3 (or n = (ni,nj,nk)) cells; ``perturb`` adds a seeded relative noise to
-    every primitive (used by the parity tests so that no term of the fluxes vanishes)."""
+    every primitive (used by the parity tests so that no term of the fluxes vanishes); ``bcs``
+    replaces the periodic boundary conditions."""
     if np.isscalar(n):
         n = (n, n, n)
     num = P.Numerics() if numerics is None else numerics
     st = tgv_state()
     lo, hi = (0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI)
     x, y, z = M.cart_vertices(n, lo, hi)
-    zone = HostZone(name, n[0], n[1], n[2], num, st, x, y, z, bcs=ALL_PERIODIC, flow=flow)
+    zone = HostZone(name, n[0], n[1], n[2], num, st, x, y, z, bcs=ALL_PERIODIC if bcs is None else bcs,
+                    flow=flow)
     xc, yc, zc = M.cell_centers_cart(n, lo, hi)
     w = M.tgv_primitives(xc, yc, zc, st)
     if perturb > 0.0:

@@ -1,0 +1,181 @@
+// Adjoint pieces: gradients of the wall pressure-force coefficients with respect to the state
+// (compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for -> dpJ_dpW_fluausm_select.for ->
+// dpJ_dpW_fluausm_euler_o3_2d.for, stated here for 3-D zones).
+//
+// Every floating-point operation follows the order of the oracle's fo_dpj_dpw, compiled without
+// contraction, so that the two agree bit for bit.
+#include "fm_device.hpp"
+#include "fm_kernels.hpp"
+
+namespace fm {
+
+// primitive gradient g of one cell (rho, u, v, w, T) -> gradient with respect to the
+// conservative state (rho, rho u, rho v, rho w, rho E) (dpP_dpW_fluausm_euler_2d_idir_min.for)
+__device__ __forceinline__ void dpj_cons(const double q[5], double cvm, const double g[5], double out[5]) {
+    double w1 = q[0], w2 = q[0] * q[1], w3 = q[0] * q[2], w4 = q[0] * q[3];
+    double ke = 0.5 * (w2 * w2 + w3 * w3 + w4 * w4);
+    double w5 = cvm * w1 * q[4] + ke / w1;
+    double w11 = w1 * w1;
+    double t1 = 1. / cvm * (-w5 / w11 + 2. * ke / (w11 * w1));
+    out[0] = g[0] + g[1] * (-w2 / w11) + g[2] * (-w3 / w11) + g[3] * (-w4 / w11) + g[4] * t1;
+    out[1] = g[1] * (1. / w1) + g[4] * (-1. / cvm * w2 / w11);
+    out[2] = g[2] * (1. / w1) + g[4] * (-1. / cvm * w3 / w11);
+    out[3] = g[3] * (1. / w1) + g[4] * (-1. / cvm * w4 / w11);
+    out[4] = g[4] * (1. / (cvm * w1));
+}
+
+// Reverse pass of the AUSM+(P) o3 momentum flux of one face (AUSM fluFaceEuler.for, the
+// flux inviscid_flux<1> forms): q[s] = primitive states of the stencil (nm2, nm, l, np),
+// fb = weights of (F_x, F_y, F_z); wb[s] += d(fb . F)/dq[s], the sound speed c^2 = rgp gam T_l
+// included (dpJ_dpW_fluausm_euler_o3_2d.for:468-614).
+__device__ __forceinline__ void dpj_face(const FluxConst& f, const double q[4][5], const double tn[3],
+                                         double sn, const double fb[3], double wb[4][5]) {
+    double qp[5], qm[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        qm[e] = f.c4 * q[2][e] + f.c5 * q[1][e] + f.c6 * q[3][e];
+        qp[e] = f.c4 * q[1][e] + f.c6 * q[0][e] + f.c5 * q[2][e];
+    }
+    double rgp = f.rgp, r1 = qp[0], r2 = qm[0];
+    double p1 = r1 * qp[4] * rgp, p2 = r2 * qm[4] * rgp;
+    double qn1 = qp[1] * tn[0] + qp[2] * tn[1] + qp[3] * tn[2];
+    double qn2 = qm[1] * tn[0] + qm[2] * tn[1] + qm[3] * tn[2];
+    double c = rgp * f.gam * q[2][4];
+    double son = sqrt(qn1 * qn1 / c);
+    double tam = f.c3 * son + sn;
+    double tam1 = dmax(0., tam) * f.c2;
+    double u = 0.25 * (qn1 + qn2) - tam1 * (p2 - p1);
+    double tdu = dmax(fabs(u), f.c1 * sn) * f.wig_cte;
+    double ub = 0., tdub = 0., p1p2b = 0., m1b[3], m2b[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        double mom1 = r1 * qp[1 + m], mom2 = r2 * qm[1 + m];
+        ub = ub + fb[m] * (mom1 + mom2);
+        tdub = tdub + fb[m] * (mom1 - mom2);
+        m1b[m] = fb[m] * (u + tdu);
+        m2b[m] = fb[m] * (u - tdu);
+        p1p2b = p1p2b + fb[m] * tn[m];
+    }
+    double p1b = 0.5 * p1p2b, p2b = 0.5 * p1p2b;
+    if (fabs(u) >= f.c1 * sn) ub = ub + tdub * f.wig_cte * (u >= 0. ? 1. : -1.);
+    double qn1b = 0.25 * ub, qn2b = 0.25 * ub;
+    double tam1b = -ub * (p2 - p1);
+    p1b = p1b + ub * tam1;
+    p2b = p2b - ub * tam1;
+    double tamb = tam >= 0. ? tam1b * f.c2 : 0.;
+    double sonb = f.c3 * tamb;
+    double sc = sqrt(c);
+    qn1b = qn1b + sonb * (qn1 >= 0. ? 1. / sc : -1. / sc);
+    double cb = sonb * (-0.5 * sqrt(qn1 * qn1) / sqrt(c * c * c));
+    double qpb[5], qmb[5];
+    qpb[0] = p1b * qp[4] * rgp; qmb[0] = p2b * qm[4] * rgp;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        qpb[0] = qpb[0] + m1b[m] * qp[1 + m];
+        qmb[0] = qmb[0] + m2b[m] * qm[1 + m];
+        qpb[1 + m] = qn1b * tn[m] + m1b[m] * r1;
+        qmb[1 + m] = qn2b * tn[m] + m2b[m] * r2;
+    }
+    qpb[4] = p1b * r1 * rgp; qmb[4] = p2b * r2 * rgp;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        wb[0][e] = wb[0][e] + f.c6 * qpb[e];
+        wb[1][e] = wb[1][e] + (f.c4 * qpb[e] + f.c5 * qmb[e]);
+        wb[2][e] = wb[2][e] + (f.c5 * qpb[e] + f.c4 * qmb[e]);
+        wb[3][e] = wb[3][e] + f.c6 * qmb[e];
+    }
+    wb[2][4] = wb[2][4] + cb * (rgp * f.gam);
+}
+
+// ghost gradient g carried onto the interior cell it mirrors (bvbs_wall_inviscid.for: density
+// and temperature copied, velocity reflected about the plane of unit normal n): x += M^T g
+__device__ __forceinline__ void dpj_mirror(const double n[3], const double g[5], double x[5]) {
+    x[0] = x[0] + g[0];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double s = 0.;
+#pragma unroll
+        for (int b = 0; b < 3; b++) s = s + g[1 + b] * ((a == b ? 1. : 0.) - 2. * n[b] * n[a]);
+        x[1 + a] = x[1 + a] + s;
+    }
+    x[4] = x[4] + g[4];
+}
+
+// One thread per face of the window: J_D = s (cos a F_x + sin a F_y), J_L = s (cos a F_y -
+// sin a F_x), s = sens * surfinv (dpJ_dpW_fluausm_euler_o3_2d.for:359-366, 855-890); the two
+// interior cells of the stencil are assigned their conservative gradients (:958-978).
+template <bool CART, int D>
+__global__ void k_dpj_dpw(KZ z, DpjArgs a) {
+    const int n0 = a.w[1] - a.w[0] + 1, n1 = a.w[3] - a.w[2] + 1, n2 = a.w[5] - a.w[4] + 1;
+    const long nf = (long)n0 * n1 * n2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const int i = a.w[0] + (int)(t % n0), j = a.w[2] + (int)((t / n0) % n1), k = a.w[4] + (int)(t / ((long)n0 * n1));
+    const FluxConst f = flux_const(z);
+    const long l = IDX(z, i, j, k), nd = z.nd, ndm = z.ndm;
+    const long inc = D == 0 ? 1 : D == 1 ? z.incj : z.inck;
+    const bool up = (a.idir % 2) == 0;
+    const double norm = 2. / (z.roinf * (z.vinf * z.vinf));
+    const double sens = up ? norm : -norm;
+    const double cc = sens * a.surfinv * a.cosa, cs = sens * a.surfinv * a.sina;
+    const double cvm = f.rgp / (f.gam - 1.);
+    const long cell[4] = {l - 2 * inc, l - inc, l, l + inc};
+    double tn[3] = {0., 0., 0.}, sn;
+    if (CART) {
+        tn[D] = f.tc[D]; sn = f.s[D];
+    } else {
+        const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
+        const long lt = MTR(z, i, j, k);
+        tn[0] = tm[lt]; tn[1] = tm[lt + ndm]; tn[2] = tm[lt + 2 * ndm];
+        sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+    }
+    const double nrm = dmax(sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]), 1e-30);
+    const double n[3] = {tn[0] / nrm, tn[1] / nrm, tn[2] / nrm};
+    double q[4][5];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) q[s][e] = a.rop[cell[s] + e * nd];
+    const int A = up ? 1 : 2, B = up ? 0 : 3, gA = up ? 2 : 1, gB = up ? 3 : 0;
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+        const double fb[3] = {o == 0 ? cc : -cs, o == 0 ? cs : cc, 0.};
+        double wb[4][5];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int e = 0; e < 5; e++) wb[s][e] = 0.;
+        dpj_face(f, q, tn, sn, fb, wb);
+        dpj_mirror(n, wb[gA], wb[A]);
+        dpj_mirror(n, wb[gB], wb[B]);
+        double ga[5], gb[5];
+        dpj_cons(q[A], cvm, wb[A], ga);
+        dpj_cons(q[B], cvm, wb[B], gb);
+        double* out = o == 0 ? a.dcd : a.dcl;
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+            out[cell[A] + e * nd] = ga[e];
+            out[cell[B] + e * nd] = gb[e];
+        }
+    }
+}
+
+template <bool CART>
+static void dpj_d(const KZ& z, const DpjArgs& a, dim3 g, hipStream_t st) {
+    int d = (a.idir - 1) / 2;
+    if (d == 0) hipLaunchKernelGGL((k_dpj_dpw<CART, 0>), g, dim3(256), 0, st, z, a);
+    else if (d == 1) hipLaunchKernelGGL((k_dpj_dpw<CART, 1>), g, dim3(256), 0, st, z, a);
+    else hipLaunchKernelGGL((k_dpj_dpw<CART, 2>), g, dim3(256), 0, st, z, a);
+}
+
+int launch_dpj_dpw(const KZ& z, const DpjArgs& a, hipStream_t st) {
+    if (z.kflu != 1 || z.slope != 2 || z.iflow != 1) return -1;     // option 213 only
+    long nf = (long)(a.w[1] - a.w[0] + 1) * (a.w[3] - a.w[2] + 1) * (a.w[5] - a.w[4] + 1);
+    if (nf <= 0) return 0;
+    dim3 g((unsigned)((nf + 255) / 256));
+    if (z.typ == 2) dpj_d<true>(z, a, g, st);
+    else dpj_d<false>(z, a, g, st);
+    return 0;
+}
+
+}  // namespace fm

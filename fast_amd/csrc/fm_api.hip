@@ -652,6 +652,43 @@ extern "C" int fm_wall_effort(fm_ctx* c, int zone, int level, int idir, const in
     return FM_OK;
 }
 
+extern "C" int fm_dpj_dpw(fm_ctx* c, int zone, int level, int idir, const int* window, double cosa,
+                          double sina, double surfinv, double* dcd, double* dcl) {
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2 || idir < 1 || idir > 6 || !window || !dcd || !dcl)
+        return fail(FM_E_ARG, "fm_dpj_dpw: bad arguments");
+    const KZ& k = z->k;
+    if (k.kflu != 1 || k.slope != 2 || k.iflow != 1)
+        return fail(FM_E_UNSUPPORTED, "fm_dpj_dpw: AUSM+(P) o3 Euler zones only (the reference's option 213)");
+    int d = (idir - 1) / 2;
+    int lo[3] = {1, 1, 1}, hi[3] = {k.ni, k.nj, k.nk};
+    hi[d] = hi[d] + 1;
+    for (int a = 0; a < 3; a++)
+        if (window[2 * a] < lo[a] || window[2 * a + 1] > hi[a] || window[2 * a] > window[2 * a + 1])
+            return fail(FM_E_ARG, "fm_dpj_dpw: window outside the zone");
+    if (window[2 * d] != window[2 * d + 1]) return fail(FM_E_ARG, "fm_dpj_dpw: window is not one face plane");
+    int face = idir % 2 == 1 ? 1 : (d == 0 ? k.ni : d == 1 ? k.nj : k.nk) + 1;
+    if (window[2 * d] != face) return fail(FM_E_ARG, "fm_dpj_dpw: window is not on the zone face of idir %d", idir);
+    HIPCHK(hipSetDevice(c->device));
+    size_t bytes = sizeof(double) * 5 * (size_t)k.nd;
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, 2 * bytes));
+    hipError_t e = hipMemcpyAsync(buf, dcd, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf + 5 * k.nd, dcl, bytes, hipMemcpyHostToDevice, c->stream);
+    DpjArgs a;
+    a.idir = idir;
+    for (int q = 0; q < 6; q++) a.w[q] = window[q];
+    a.cosa = cosa; a.sina = sina; a.surfinv = surfinv;
+    a.rop = z->ro[level]; a.dcd = buf; a.dcl = buf + 5 * k.nd;
+    if (e == hipSuccess) { launch_dpj_dpw(k, a, c->stream); e = hipGetLastError(); }
+    if (e == hipSuccess) e = hipMemcpyAsync(dcd, buf, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dcl, buf + 5 * k.nd, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(FM_E_HIP, "fm_dpj_dpw: %s", hipGetErrorString(e));
+    return FM_OK;
+}
+
 extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int* no_d,
                                 const int64_t* nb_pts, const int64_t* rcv_pts, const int64_t* dnr_pts) {
     if (!c || nrac < 0 || (nrac > 0 && (!no_r || !no_d || !nb_pts || !rcv_pts || !dnr_pts)))

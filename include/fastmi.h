@@ -16,6 +16,8 @@
  *   fm_zone_substep      gsdr3 for one sub-iteration (Compute/gsdr3.cpp:57-484)
  *   fm_apply_bc          fasts._applyBC   (BC/applyBC.cpp:33-255, PyTree.py:1155)
  *   fm_wall_effort       fasts.compute_effort (BC/compute_effort.cpp:34-289, bceffort.for)
+ *   fm_dpj_dpw           fasts.compute_dpJ_dpW (ADJOINT/compute_dpJ_dpW.cpp:33-261,
+ *                        PyTree.py:3719-3735)
  *   fm_transfers_set     the flattened tc raccords (param_int_tc) that FastC builds from the
  *                        connectivity tree (Fast/FastC/FastC/PyTree.py, used by
  *                        Fast/FastC/FastC/Com/setInterpTransfersFast.cpp:640-760)
@@ -88,6 +90,20 @@ int fm_apply_bc(fm_ctx* ctx, int zone, int level, int nstep);
    Synchronous. */
 int fm_wall_effort(fm_ctx* ctx, int zone, int level, int idir, const int* window,
                    const double* pos, double* effort, double* fields);
+
+/* Gradients of the pressure drag and lift coefficients of one slip-wall face window with
+   respect to the conservative state (rho, rho u, rho v, rho w, rho E) of the wall cells
+   (replaces fasts.compute_dpJ_dpW, ADJOINT/compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for
+   -> ADJOINT/AUSM/dpJ_dpW_fluausm_select.for -> dpJ_dpW_fluausm_euler_o3_2d.for, whose 2-D
+   statement is extended here to 3-D zones; FastS.PyTree._compute_dpJ_dpW, PyTree.py:3719).
+   Per face, J_D = s (cos_a F_x + sin_a F_y) and J_L = s (cos_a F_y - sin_a F_x), F the AUSM+(P)
+   o3 momentum flux of the face, s = -+2 surfinv/(rho_inf V_inf^2) on min / max faces.  The
+   two ghost cells of each face follow the slip-wall mirror.  window: as fm_wall_effort, on the
+   zone face of idir.  dcd, dcl: 5 x NDIMDX doubles each (the fields dpCDp_dp*, dpCLp_dp*),
+   assigned at the two interior cells of each face's stencil and left as they were elsewhere.
+   Zones: AUSM+(P), o3, Euler (FM_E_UNSUPPORTED otherwise).  Synchronous. */
+int fm_dpj_dpw(fm_ctx* ctx, int zone, int level, int idir, const int* window, double cosa,
+               double sina, double surfinv, double* dcd, double* dcl);
 
 /* Injection raccords between zones (1-to-1 matches), replacing any set before.  Raccord r
    fills nb_pts[r] receiver cells of zone no_r[r] from donor cells of zone no_d[r]; rcv_pts and

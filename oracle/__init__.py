@@ -95,6 +95,30 @@ class _Backend:
             raise RuntimeError("%s effort failed (%d)" % (self.prefix, rc))
         return (out, flu.reshape(21, nf)) if fields else out
 
+    def dpj_dpw(self, z, idir, ind, cosa, sina, surfinv, rop=None, dcd=None, dcl=None):
+        """Gradients of the pressure drag and lift coefficients with respect to the
+        conservative state of the wall cells of the slip-wall window ``ind`` (face index in the
+        normal direction idir), assigned into ``dcd``, ``dcl`` (5 x NDIMDX, zero elsewhere when
+        created here).  Oracle: the 3-D statement; Reference: the reference's 2-D routine run
+        plane by plane (zero w, face vectors in the (x, y) plane required)."""
+        rop = z.ro if rop is None else rop
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        dcd = np.zeros(5 * z.nd) if dcd is None else dcd
+        dcl = np.zeros(5 * z.nd) if dcl is None else dcl
+        f = getattr(self.lib, self.prefix + "dpj_dpw")
+        f.restype = ctypes.c_int
+        c = [ctypes.c_double(v) for v in (cosa, sina, surfinv)]
+        if self.prefix == "fo_":
+            rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(idir), _i(ind), _d(rop), _d(z.ti),
+                   _d(z.tj), _d(z.tk), c[0], c[1], c[2], _d(dcd), _d(dcl))
+        else:
+            rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(idir), _i(ind), _d(rop), _d(z.wig),
+                   _d(z.x), _d(z.y), _d(z.z), _d(z.ti), _d(z.tj), _d(z.tk), _d(z.vol), c[0], c[1],
+                   c[2], _d(dcd), _d(dcl))
+        if rc != 0:
+            raise RuntimeError("%s dpj_dpw failed (%d)" % (self.prefix, rc))
+        return dcd, dcl
+
     def apply_bc(self, z, rop, nstep=1):
         f = getattr(self.lib, self.prefix + "apply_bc")
         rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nstep), _d(rop), _d(z.ti),

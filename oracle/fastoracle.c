@@ -1058,6 +1058,165 @@ int fo_effort(const int* p, const double* pr, int idir, const int* ind, const do
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* Gradients of the pressure drag and lift coefficients at a slip wall:
+   compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for:55-70 -> dpJ_dpW_fluausm_select.for:60-170
+   -> dpJ_dpW_fluausm_euler_o3_2d.for (option 213: AUSM+(P), o3, Euler; the reference codes
+   the 2-D zone only and leaves every 3-D option empty, so this is its 3-D statement).
+   Per face of the window, with F the AUSM+(P) o3 momentum flux of the face (AUSM
+   fluFaceEuler.for, the flux inviscid_flux forms) and s = sens * surfinv, sens = -+2/(rho_inf
+   V_inf^2) on min / max faces (:359-366):
+       J_D = s (cos a F_x + sin a F_y),   J_L = s (cos a F_y - sin a F_x)      (:855-890)
+   differentiated (reverse mode here, a forward chain of partials in the reference,
+   :468-850) with respect to the primitive states of the four-cell stencil, the sound speed
+   c^2 = rgp gam T_l included (:468, :587-614).  The two ghost cells of the stencil follow the
+   slip-wall mirror of bvbs_wall_inviscid.for (density and temperature copied, velocity
+   reflected about the face plane, :620-700), so their gradients are carried onto the interior
+   cells they mirror, and the primitive gradients of the two interior cells are mapped to the
+   conservative state (rho, rho u, rho v, rho w, rho E) through dpP_dpW_fluausm_euler_2d_
+   idir_{min,max}.for.  The values are assigned, not accumulated, at the two interior cells
+   (:958-978), every other cell is left as it was.                                        */
+static void dpj_cons(const double q[5], double cvm, const double g[5], double out[5])
+{
+    double w1 = q[0], w2 = q[0] * q[1], w3 = q[0] * q[2], w4 = q[0] * q[3];
+    double ke = 0.5 * (w2 * w2 + w3 * w3 + w4 * w4);
+    double w5 = cvm * w1 * q[4] + ke / w1;
+    double w11 = w1 * w1;
+    double t1 = 1. / cvm * (-w5 / w11 + 2. * ke / (w11 * w1));
+    out[0] = g[0] + g[1] * (-w2 / w11) + g[2] * (-w3 / w11) + g[3] * (-w4 / w11) + g[4] * t1;
+    out[1] = g[1] * (1. / w1) + g[4] * (-1. / cvm * w2 / w11);
+    out[2] = g[2] * (1. / w1) + g[4] * (-1. / cvm * w3 / w11);
+    out[3] = g[3] * (1. / w1) + g[4] * (-1. / cvm * w4 / w11);
+    out[4] = g[4] * (1. / (cvm * w1));
+}
+
+/* reverse pass of one face: q[s] = primitive states of nm2, nm, l, np; fb = weights of
+   (F_x, F_y, F_z); wb[s] += d(fb . F)/d q[s] */
+static void dpj_face(const FC* f, const double q[4][5], const double tn[3], double sn,
+                     const double fb[3], double wb[4][5])
+{
+    double qp[5], qm[5];
+    for (int e = 0; e < 5; e++) {
+        qm[e] = f->c4 * q[2][e] + f->c5 * q[1][e] + f->c6 * q[3][e];
+        qp[e] = f->c4 * q[1][e] + f->c6 * q[0][e] + f->c5 * q[2][e];
+    }
+    double rgp = f->rgp, r1 = qp[0], r2 = qm[0];
+    double p1 = r1 * qp[4] * rgp, p2 = r2 * qm[4] * rgp;
+    double qn1 = qp[1] * tn[0] + qp[2] * tn[1] + qp[3] * tn[2];
+    double qn2 = qm[1] * tn[0] + qm[2] * tn[1] + qm[3] * tn[2];
+    double c = rgp * f->gam * q[2][4];
+    double son = sqrt(qn1 * qn1 / c);
+    double tam = f->c3 * son + sn;
+    double tam1 = fmax2(0., tam) * f->c2;
+    double u = 0.25 * (qn1 + qn2) - tam1 * (p2 - p1);
+    double tdu = fmax2(fabs(u), f->c1 * sn) * f->wig_cte;
+    double ub = 0., tdub = 0., p1p2b = 0., m1b[3], m2b[3];
+    for (int m = 0; m < 3; m++) {
+        double mom1 = r1 * qp[1 + m], mom2 = r2 * qm[1 + m];
+        ub = ub + fb[m] * (mom1 + mom2);
+        tdub = tdub + fb[m] * (mom1 - mom2);
+        m1b[m] = fb[m] * (u + tdu);
+        m2b[m] = fb[m] * (u - tdu);
+        p1p2b = p1p2b + fb[m] * tn[m];
+    }
+    double p1b = 0.5 * p1p2b, p2b = 0.5 * p1p2b;
+    if (fabs(u) >= f->c1 * sn) ub = ub + tdub * f->wig_cte * (u >= 0. ? 1. : -1.);
+    double qn1b = 0.25 * ub, qn2b = 0.25 * ub;
+    double tam1b = -ub * (p2 - p1);
+    p1b = p1b + ub * tam1;
+    p2b = p2b - ub * tam1;
+    double tamb = tam >= 0. ? tam1b * f->c2 : 0.;
+    double sonb = f->c3 * tamb;
+    double sc = sqrt(c);
+    qn1b = qn1b + sonb * (qn1 >= 0. ? 1. / sc : -1. / sc);
+    double cb = sonb * (-0.5 * sqrt(qn1 * qn1) / sqrt(c * c * c));
+    double qpb[5], qmb[5];
+    qpb[0] = p1b * qp[4] * rgp; qmb[0] = p2b * qm[4] * rgp;
+    for (int m = 0; m < 3; m++) {
+        qpb[0] = qpb[0] + m1b[m] * qp[1 + m];
+        qmb[0] = qmb[0] + m2b[m] * qm[1 + m];
+        qpb[1 + m] = qn1b * tn[m] + m1b[m] * r1;
+        qmb[1 + m] = qn2b * tn[m] + m2b[m] * r2;
+    }
+    qpb[4] = p1b * r1 * rgp; qmb[4] = p2b * r2 * rgp;
+    for (int e = 0; e < 5; e++) {
+        wb[0][e] = wb[0][e] + f->c6 * qpb[e];
+        wb[1][e] = wb[1][e] + (f->c4 * qpb[e] + f->c5 * qmb[e]);
+        wb[2][e] = wb[2][e] + (f->c5 * qpb[e] + f->c4 * qmb[e]);
+        wb[3][e] = wb[3][e] + f->c6 * qmb[e];
+    }
+    wb[2][4] = wb[2][4] + cb * (rgp * f->gam);
+}
+
+/* ghost gradient g carried onto the interior cell it mirrors: x += M^T g, M = the slip-wall
+   mirror (velocity reflected about the plane of unit normal n) */
+static void dpj_mirror(const double n[3], const double g[5], double x[5])
+{
+    x[0] = x[0] + g[0];
+    for (int a = 0; a < 3; a++) {
+        double s = 0.;
+        for (int b = 0; b < 3; b++) s = s + g[1 + b] * ((a == b ? 1. : 0.) - 2. * n[b] * n[a]);
+        x[1 + a] = x[1 + a] + s;
+    }
+    x[4] = x[4] + g[4];
+}
+
+int fo_dpj_dpw(const int* p, const double* pr, int idir, const int* ind, const double* rop,
+               const double* ti, const double* tj, const double* tk, double cosa, double sina,
+               double surfinv, double* dcd, double* dcl)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    FC f;
+    double vdum = 1.;                /* fc_init reads a Cartesian zone's volume */
+    if (fc_init(&f, Z, p, pr, rop, NULL, ti, tj, tk, &vdum, NULL)) return -1;
+    if (f.kflu != 1 || f.slope != 2 || p[IFLOW] != 1) return -1;    /* option 213 only */
+    int d = (idir - 1) / 2, up = idir % 2 == 0;
+    long nd = Z->nd, ndm = Z->ndm;
+    long inc = d == 0 ? 1 : d == 1 ? Z->incj : Z->inck;
+    double norm = 2. / (pr[ROINF] * (pr[VINF] * pr[VINF]));
+    double sens = up ? norm : -norm;
+    double cc = sens * surfinv * cosa, cs = sens * surfinv * sina;
+    double fbd[3] = {cc, cs, 0.}, fbl[3] = {-cs, cc, 0.};
+    double cvm = f.rgp / (f.gam - 1.);
+    for (long k = ind[4]; k <= ind[5]; k++)
+        for (long j = ind[2]; j <= ind[3]; j++)
+            for (long i = ind[0]; i <= ind[1]; i++) {
+                long l = IDX(Z, i, j, k);
+                long cell[4] = {l - 2 * inc, l - inc, l, l + inc};
+                double tn[3] = {0., 0., 0.}, sn;
+                if (f.cart) { tn[d] = f.tc[d]; sn = f.s[d]; }
+                else {
+                    const double* t = d == 0 ? ti : d == 1 ? tj : tk;
+                    long lt = MTR(Z, i, j, k);
+                    tn[0] = t[lt]; tn[1] = t[lt + ndm]; tn[2] = t[lt + 2 * ndm];
+                    sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+                }
+                double nrm = fmax2(sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]), 1e-30);
+                double n[3] = {tn[0] / nrm, tn[1] / nrm, tn[2] / nrm};
+                double q[4][5];
+                for (int s = 0; s < 4; s++)
+                    for (int e = 0; e < 5; e++) q[s][e] = rop[cell[s] + e * nd];
+                /* interior cells (A nearest the wall) and the ghosts mirroring them */
+                int A = up ? 1 : 2, B = up ? 0 : 3, gA = up ? 2 : 1, gB = up ? 3 : 0;
+                for (int o = 0; o < 2; o++) {
+                    double wb[4][5];
+                    memset(wb, 0, sizeof wb);
+                    dpj_face(&f, q, tn, sn, o == 0 ? fbd : fbl, wb);
+                    dpj_mirror(n, wb[gA], wb[A]);
+                    dpj_mirror(n, wb[gB], wb[B]);
+                    double ga[5], gb[5];
+                    dpj_cons(q[A], cvm, wb[A], ga);
+                    dpj_cons(q[B], cvm, wb[B], gb);
+                    double* out = o == 0 ? dcd : dcl;
+                    for (int e = 0; e < 5; e++) {
+                        out[cell[A] + e * nd] = ga[e];
+                        out[cell[B] + e * nd] = gb[e];
+                    }
+                }
+            }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* core3as2.for (neq = 5, non-2d): Newton/BDF residual                                   */
 static void fo_core3as2(const int* p, const double* pr, int nitcfg, int first_it,
                         const int* ind, const double* rop, const double* rop_n,

@@ -53,6 +53,12 @@ int fo_effort(const int* p, const double* pr, int idir, const int* ind, const do
               const double* y, const double* z, const double* ti, const double* tj,
               const double* tk, const double* vol, double* eff, double* flu);
 
+/* gradients of the pressure drag / lift coefficients with respect to the conservative state
+   of the wall cells of one slip-wall window (dpJ_dpW_fluausm_euler_o3_2d.for in 3-D) */
+int fo_dpj_dpw(const int* p, const double* pr, int idir, const int* ind, const double* rop,
+               const double* ti, const double* tj, const double* tk, double cosa, double sina,
+               double surfinv, double* dcd, double* dcl);
+
 #ifdef __cplusplus
 }
 #endif

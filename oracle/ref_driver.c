@@ -424,3 +424,34 @@ int ref_effort(I* p, F* pr, I idir, I* ind, F* pos, F* rop, F* xmut, F* wig, F* 
     free(flu);
     return 0;
 }
+
+/* compute_dpJ_dpW.cpp:220-230 -> dpj_dpw_calc_meth -> dpJ_dpW_fluausm_select (option 213),
+   called here at its 2-D routine directly: the reference codes no 3-D option.  The routine
+   is run on 3-D arrays whose k planes are independent 2-D problems (zero w, face vectors in
+   the (x, y) plane); NEQ_IJ is given as 2, the 2-D zone's metric count, so that shape_tab_mtr
+   drops the z component of the wall normal (ck_mtr = 0), as for a 2-D zone. */
+void dpj_dpw_fluausm_euler_o3_2d_(I* ndom, I* ithread, I* p, F* pr, I* pe, I* ind_loop, F* pos,
+                                  F* rop, F* wig, F* x, F* y, F* z, F* dcd, F* dcl, F* venti,
+                                  F* ventj, F* ventk, F* ti, F* tj, F* tk, F* vol, F* xmut,
+                                  F* cosa, F* sina, F* surfinv);
+
+int ref_dpj_dpw(I* p, F* pr, I idir, I* ind, F* rop, F* wig, F* x, F* y, F* z, F* ti, F* tj,
+                F* tk, F* vol, F cosa, F sina, F surfinv, F* dcd, F* dcl)
+{
+    I nd = 0, ithread = 1;
+    I pe[64], p2[137];               /* the header slots of Parameter_int */
+    memset(pe, 0, sizeof pe);
+    memcpy(p2, p, sizeof p2);
+    p2[NEQ_IJ] = 2;
+    I n0 = ind[1] - ind[0] + 1, n1 = ind[3] - ind[2] + 1, n2 = ind[5] - ind[4] + 1;
+    pe[NEQ] = 21; pe[NDIMDX] = n0 * n1 * n2;
+    for (int q = 0; q < 6; q++) pe[23 + q] = ind[q];
+    pe[29] = ind[0]; pe[30] = ind[2]; pe[31] = ind[4]; pe[32] = n0; pe[33] = n0 * n1;
+    pe[34] = 0; pe[35] = idir;
+    I loop[6];
+    memcpy(loop, ind, sizeof loop);
+    F vent[3] = {0., 0., 0.}, pos[3] = {0., 0., 0.};
+    dpj_dpw_fluausm_euler_o3_2d_(&nd, &ithread, p2, pr, pe, loop, pos, rop, wig, x, y, z, dcd, dcl,
+                                 vent, vent, vent, ti, tj, tk, vol, rop, &cosa, &sina, &surfinv);
+    return 0;
+}

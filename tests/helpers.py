@@ -71,3 +71,30 @@ def inflow_outpres(n, p_exit=None, seed=5):
     return [("BCInflow", "imin", None), ("BCOutpres", "imax", [p_exit]),
             ("BCWallViscous", "jmin", None), ("BCFarfield", "jmax", None),
             ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
+
+
+def extruded_box(n, num, seed=7, amp=0.05, bcs=None, perturb=0.02, flow="euler"):
+    """A 2-D jittered (x, y) mesh extruded along z with unequal k steps (a 3dfull zone whose
+    k planes are independent 2-D problems: i and j face vectors lie in the (x, y) plane), with
+    the TGV state perturbed and w = 0."""
+    st = cases.tgv_state()
+    x, y, z = M.cart_vertices(n, (0, 0, 0), (2 * np.pi,) * 3)
+    rng = np.random.default_rng(seed)
+    nv = (n[0] + 5, n[1] + 5, n[2] + 5)
+    X = x.reshape(nv, order="F"); Y = y.reshape(nv, order="F"); Z = z.reshape(nv, order="F")
+    h = 2 * np.pi / np.array(n)
+    s = (slice(1, -1), slice(1, -1))
+    dx = amp * h[0] * rng.uniform(-1, 1, X[s + (0,)].shape)
+    dy = amp * h[1] * rng.uniform(-1, 1, Y[s + (0,)].shape)
+    X[s] += dx[:, :, None]; Y[s] += dy[:, :, None]
+    zk = Z[0, 0, :].copy()
+    zk[1:-1] += amp * h[2] * rng.uniform(-1, 1, zk.size - 2)
+    Z[:] = zk[None, None, :]
+    x, y, z = X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")
+    bcs = INVISCID_J if bcs is None else bcs
+    zz = HostZone("ebox", n[0], n[1], n[2], num, st, x, y, z, bcs=bcs, flow=flow)
+    xc, yc, zc = M.cell_centers_from_vertices(x, y, z, nv)
+    w = M.tgv_primitives(xc, yc, zc, st)
+    w = w + perturb * rng.uniform(-1, 1, w.size) * np.repeat([1., 1., 1., 0., 1.], w.size // 5)
+    zz.ro[:] = w; zz.ro_m1[:] = w; zz.ro_p1[:] = w
+    return zz
