@@ -47,6 +47,8 @@ SIGNATURES = {
                                       _dp, _dp, _dp]),
     "fm_dpj_dpw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
+    "fm_flux_balance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
+    "fm_residual_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
 OPT_LU_SCHEDULE = 1
@@ -133,6 +135,10 @@ class Context:
         self.zones.append(hz)
         return zid.value
 
+    def zone_nd(self, zone):
+        """NDIMDX of a zone added through add_zone."""
+        return int(self.zones[zone].nd)
+
     def set(self, zone, field, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
         check(self.lib.fm_zone_set_field(self.h, zone, field, dptr(a), a.size))
@@ -187,6 +193,22 @@ class Context:
                 raise ValueError("dcd, dcl must be contiguous float64 arrays")
         check(self.lib.fm_dpj_dpw(self.h, zone, level, idir, w.ctypes.data_as(_ip), float(cosa),
                                   float(sina), float(surfinv), dptr(dcd), dptr(dcl)))
+
+    def flux_balance(self, zone, level=0):
+        """The face balance of the interior cells alone (5 x NDIMDX)."""
+        nd = self.zone_nd(zone)
+        out = np.zeros(5 * nd)
+        check(self.lib.fm_flux_balance(self.h, zone, level, dptr(out)))
+        return out
+
+    def residual_vjp(self, zone, lam, level=0):
+        """(dR/dW)^T lam for the AUSM+(P) o3 Euler face balance (5 x NDIMDX)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros_like(lam)
+        if lam.size != 5 * self.zone_nd(zone):
+            raise ValueError("lam must hold 5 x NDIMDX values")
+        check(self.lib.fm_residual_vjp(self.h, zone, level, dptr(lam), dptr(out)))
+        return out
 
     def transfer(self, level=0):
         check(self.lib.fm_transfer(self.h, level))

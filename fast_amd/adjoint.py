@@ -1,0 +1,39 @@
+"""The transposed residual Jacobian behind ``torch.autograd``.
+
+``flux_balance(W, ctx, zone)`` is the AUSM+(P) o3 Euler face balance R(W) of a zone's interior
+cells (the fluxes the reference's RHS kernels accumulate into drodm), W the primitive state of
+every cell of the zone (5 x NDIMDX, ghost cells included, float64).  Its backward is the
+transposed Jacobian product of ``fm_residual_vjp`` (the reverse mode the reference holds for 2-D
+zones as ADJOINT/AUSM/2d/fluausm_euler_o3_2d_b.for).  Both run on the GPU through the C ABI; the
+zone's time level ``level`` (default 0) is the working state and is overwritten with W.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class FluxBalance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, W, fmctx, zone, level):
+        if W.dtype != torch.float64:
+            raise TypeError("W must be float64")
+        w = W.detach().cpu().numpy()
+        fmctx.set(zone, (_lib.FIELD_RO, _lib.FIELD_RO_M1, _lib.FIELD_RO_P1)[level], w)
+        R = fmctx.flux_balance(zone, level)
+        ctx.fm = (fmctx, zone, level)
+        ctx.save_for_backward(W)
+        return torch.from_numpy(R).to(W.device)
+
+    @staticmethod
+    def backward(ctx, gR):
+        (W,) = ctx.saved_tensors
+        fmctx, zone, level = ctx.fm
+        fmctx.set(zone, (_lib.FIELD_RO, _lib.FIELD_RO_M1, _lib.FIELD_RO_P1)[level], W.detach().cpu().numpy())
+        g = fmctx.residual_vjp(zone, np.ascontiguousarray(gR.detach().cpu().numpy()), level)
+        return torch.from_numpy(g).to(W.device), None, None, None
+
+
+def flux_balance(W, fmctx, zone, level=0):
+    """R(W) with the transposed Jacobian product as its backward."""
+    return FluxBalance.apply(W, fmctx, zone, level)

@@ -1,9 +1,12 @@
-// Adjoint pieces: gradients of the wall pressure-force coefficients with respect to the state
-// (compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for -> dpJ_dpW_fluausm_select.for ->
-// dpJ_dpW_fluausm_euler_o3_2d.for, stated here for 3-D zones).
+// Adjoint pieces, for 3-D zones:
+// * gradients of the wall pressure-force coefficients with respect to the state
+//   (compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for -> dpJ_dpW_fluausm_select.for ->
+//   dpJ_dpW_fluausm_euler_o3_2d.for);
+// * the transposed Jacobian product of the AUSM+(P) o3 Euler face balance (the reverse mode of
+//   fluausm_euler_o3_2d_b.for).
 //
-// Every floating-point operation follows the order of the oracle's fo_dpj_dpw, compiled without
-// contraction, so that the two agree bit for bit.
+// Every floating-point operation follows the order of the oracle's fo_dpj_dpw and
+// fo_residual_vjp, compiled without contraction, so that the two agree bit for bit.
 #include "fm_device.hpp"
 #include "fm_kernels.hpp"
 
@@ -175,6 +178,202 @@ int launch_dpj_dpw(const KZ& z, const DpjArgs& a, hipStream_t st) {
     dim3 g((unsigned)((nf + 255) / 256));
     if (z.typ == 2) dpj_d<true>(z, a, g, st);
     else dpj_d<false>(z, a, g, st);
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Transposed residual Jacobian product (the reverse mode Tapenade gives fluausm_euler_o3_2d in
+// ADJOINT/AUSM/2d/fluausm_euler_o3_2d_b.for: the gradient of the face balance drodm with
+// respect to rop), for 3-D zones: out = (dR/dW)^T lambda, R the AUSM+(P) o3 Euler face balance
+// of the interior cells (R_l = sum_d F_d(l) - F_d(l + inc_d)), W the primitive state of every
+// cell, ghost cells included.
+//
+// Pass 1, one thread per face of direction D (face index 1..n_D + 1, interior across): the
+// weight of the face is fb = lambda_l - lambda_{l - inc} (each term only where its cell is
+// interior) and the face's reverse pass gives its contribution to the four stencil cells,
+// stored face-major.  Pass 2, one thread per cell of the slab (index -1..n_D + 2 along D,
+// interior across): the contributions of the (at most four) faces whose stencil holds the cell,
+// in stencil-slot order, are added to out.  Directions run i, j, k.
+
+// reverse pass of the whole AUSM+(P) o3 face flux (AUSM fluFaceEuler.for): fb = weights of the
+// five flux components; wb[s] = d(fb . F)/dq[s] for the stencil (nm2, nm, l, np)
+__device__ __forceinline__ void ausm_face_rev(const FluxConst& f, const double q[4][5], const double tn[3],
+                                              double sn, const double fb[5], double wb[4][5]) {
+    double qp[5], qm[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        qm[e] = f.c4 * q[2][e] + f.c5 * q[1][e] + f.c6 * q[3][e];
+        qp[e] = f.c4 * q[1][e] + f.c6 * q[0][e] + f.c5 * q[2][e];
+    }
+    double rgp = f.rgp, gam1 = f.gam1, r1 = qp[0], r2 = qm[0];
+    double mom1[3], mom2[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) { mom1[m] = r1 * qp[1 + m]; mom2[m] = r2 * qm[1 + m]; }
+    double p1 = r1 * qp[4] * rgp, p2 = r2 * qm[4] * rgp;
+    double h1 = gam1 * p1 + .5 * (mom1[0] * qp[1] + mom1[1] * qp[2] + mom1[2] * qp[3]);
+    double h2 = gam1 * p2 + .5 * (mom2[0] * qm[1] + mom2[1] * qm[2] + mom2[2] * qm[3]);
+    double qn1 = qp[1] * tn[0] + qp[2] * tn[1] + qp[3] * tn[2];
+    double qn2 = qm[1] * tn[0] + qm[2] * tn[1] + qm[3] * tn[2];
+    double c = rgp * f.gam * q[2][4];
+    double son = sqrt(qn1 * qn1 / c);
+    double tam = f.c3 * son + sn;
+    double tam1 = dmax(0., tam) * f.c2;
+    double u = 0.25 * (qn1 + qn2) - tam1 * (p2 - p1);
+    double tdu = dmax(fabs(u), f.c1 * sn) * f.wig_cte;
+    // F0 = u (r1 + r2) - tdu (r2 - r1); F_{1+m} = u (mom1 + mom2) - tdu (mom2 - mom1) + tn_m p1p2;
+    // F4 = u (h1 + h2) - tdu (h2 - h1)
+    double ub = fb[0] * (r1 + r2), tdub = -(fb[0] * (r2 - r1)), p1p2b = 0.;
+    double m1b[3], m2b[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        ub = ub + fb[1 + m] * (mom1[m] + mom2[m]);
+        tdub = tdub - fb[1 + m] * (mom2[m] - mom1[m]);
+        m1b[m] = fb[1 + m] * (u + tdu);
+        m2b[m] = fb[1 + m] * (u - tdu);
+        p1p2b = p1p2b + fb[1 + m] * tn[m];
+    }
+    ub = ub + fb[4] * (h1 + h2);
+    tdub = tdub - fb[4] * (h2 - h1);
+    double r1b = fb[0] * (u + tdu), r2b = fb[0] * (u - tdu);
+    double h1b = fb[4] * (u + tdu), h2b = fb[4] * (u - tdu);
+    double qpb[5], qmb[5];
+    // h = gam1 p + 1/2 mom . q
+    double p1b = gam1 * h1b + 0.5 * p1p2b, p2b = gam1 * h2b + 0.5 * p1p2b;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        m1b[m] = m1b[m] + .5 * h1b * qp[1 + m];
+        m2b[m] = m2b[m] + .5 * h2b * qm[1 + m];
+        qpb[1 + m] = .5 * h1b * mom1[m];
+        qmb[1 + m] = .5 * h2b * mom2[m];
+    }
+    if (fabs(u) >= f.c1 * sn) ub = ub + tdub * f.wig_cte * (u >= 0. ? 1. : -1.);
+    double qn1b = 0.25 * ub, qn2b = 0.25 * ub;
+    double tam1b = -ub * (p2 - p1);
+    p1b = p1b + ub * tam1;
+    p2b = p2b - ub * tam1;
+    double tamb = tam >= 0. ? tam1b * f.c2 : 0.;
+    double sonb = f.c3 * tamb;
+    double sc = sqrt(c);
+    qn1b = qn1b + sonb * (qn1 >= 0. ? 1. / sc : -1. / sc);
+    double cb = sonb * (-0.5 * sqrt(qn1 * qn1) / sqrt(c * c * c));
+    r1b = r1b + p1b * qp[4] * rgp;
+    r2b = r2b + p2b * qm[4] * rgp;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        r1b = r1b + m1b[m] * qp[1 + m];
+        r2b = r2b + m2b[m] * qm[1 + m];
+        qpb[1 + m] = qpb[1 + m] + qn1b * tn[m] + m1b[m] * r1;
+        qmb[1 + m] = qmb[1 + m] + qn2b * tn[m] + m2b[m] * r2;
+    }
+    qpb[0] = r1b; qmb[0] = r2b;
+    qpb[4] = p1b * r1 * rgp; qmb[4] = p2b * r2 * rgp;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        wb[0][e] = f.c6 * qpb[e];
+        wb[1][e] = f.c4 * qpb[e] + f.c5 * qmb[e];
+        wb[2][e] = f.c5 * qpb[e] + f.c4 * qmb[e];
+        wb[3][e] = f.c6 * qmb[e];
+    }
+    wb[2][4] = wb[2][4] + cb * (rgp * f.gam);
+}
+
+template <int D>
+__device__ __forceinline__ void face_extent(const KZ& z, int& n0, int& n1, int& n2) {
+    n0 = z.ni + (D == 0); n1 = z.nj + (D == 1); n2 = z.nk + (D == 2);
+}
+
+template <bool CART, int D>
+__global__ void k_vjp_faces(KZ z, VjpArgs a) {
+    int n0, n1, n2;
+    face_extent<D>(z, n0, n1, n2);
+    const long nf = (long)n0 * n1 * n2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const int i = 1 + (int)(t % n0), j = 1 + (int)((t / n0) % n1), k = 1 + (int)(t / ((long)n0 * n1));
+    const FluxConst f = flux_const(z);
+    const long l = IDX(z, i, j, k), nd = z.nd, ndm = z.ndm;
+    const long inc = D == 0 ? 1 : D == 1 ? z.incj : z.inck;
+    const int fi = D == 0 ? i : D == 1 ? j : k;
+    const int nD = D == 0 ? z.ni : D == 1 ? z.nj : z.nk;
+    double fb[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        double p = fi <= nD ? a.lam[l + e * nd] : 0.;
+        double m = fi >= 2 ? a.lam[l - inc + e * nd] : 0.;
+        fb[e] = p - m;
+    }
+    double tn[3] = {0., 0., 0.}, sn;
+    if (CART) {
+        tn[D] = f.tc[D]; sn = f.s[D];
+    } else {
+        const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
+        const long lt = MTR(z, i, j, k);
+        tn[0] = tm[lt]; tn[1] = tm[lt + ndm]; tn[2] = tm[lt + 2 * ndm];
+        sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+    }
+    double q[4][5];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) q[s][e] = a.rop[l + (s - 2) * inc + e * nd];
+    double wb[4][5];
+    ausm_face_rev(f, q, tn, sn, fb, wb);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) a.buf[(long)(s * 5 + e) * nf + t] = wb[s][e];
+}
+
+template <int D>
+__global__ void k_vjp_gather(KZ z, VjpArgs a) {
+    int n0, n1, n2;
+    face_extent<D>(z, n0, n1, n2);
+    const long nf = (long)n0 * n1 * n2;
+    // cells: index -1 .. n_D + 2 along D, interior across
+    const int c0 = D == 0 ? z.ni + 4 : z.ni, c1 = D == 1 ? z.nj + 4 : z.nj;
+    const int c2 = D == 2 ? z.nk + 4 : z.nk;
+    const long nc = (long)c0 * c1 * c2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    int i = (int)(t % c0), j = (int)((t / c0) % c1), k = (int)(t / ((long)c0 * c1));
+    i += D == 0 ? -1 : 1; j += D == 1 ? -1 : 1; k += D == 2 ? -1 : 1;
+    const int c = D == 0 ? i : D == 1 ? j : k;
+    const int nD = D == 0 ? z.ni : D == 1 ? z.nj : z.nk;
+    const long l = IDX(z, i, j, k), nd = z.nd;
+    double acc[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int fidx = c + 2 - s;               // the face whose stencil slot s is this cell
+        if (fidx < 1 || fidx > nD + 1) continue;
+        const int fi = D == 0 ? fidx : i, fj = D == 1 ? fidx : j, fk = D == 2 ? fidx : k;
+        const long tf = (long)(fi - 1) + (long)(fj - 1) * n0 + (long)(fk - 1) * n0 * n1;
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] + a.buf[(long)(s * 5 + e) * nf + tf];
+    }
+#pragma unroll
+    for (int e = 0; e < 5; e++) a.out[l + e * nd] = a.out[l + e * nd] + acc[e];
+}
+
+template <bool CART, int D>
+static void vjp_dir(const KZ& z, const VjpArgs& a, hipStream_t st) {
+    int n0, n1, n2;
+    n0 = z.ni + (D == 0); n1 = z.nj + (D == 1); n2 = z.nk + (D == 2);
+    long nf = (long)n0 * n1 * n2;
+    hipLaunchKernelGGL((k_vjp_faces<CART, D>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, z, a);
+    long nc = (long)(z.ni + (D == 0 ? 4 : 0)) * (z.nj + (D == 1 ? 4 : 0)) * (z.nk + (D == 2 ? 4 : 0));
+    hipLaunchKernelGGL((k_vjp_gather<D>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, z, a);
+}
+
+long vjp_buffer_doubles(const KZ& z) {
+    long a = (long)(z.ni + 1) * z.nj * z.nk, b = (long)z.ni * (z.nj + 1) * z.nk, c = (long)z.ni * z.nj * (z.nk + 1);
+    long m = a > b ? a : b;
+    return 20 * (m > c ? m : c);
+}
+
+int launch_residual_vjp(const KZ& z, const VjpArgs& a, hipStream_t st) {
+    if (z.kflu != 1 || z.slope != 2 || z.iflow != 1) return -1;
+    if (z.typ == 2) { vjp_dir<true, 0>(z, a, st); vjp_dir<true, 1>(z, a, st); vjp_dir<true, 2>(z, a, st); }
+    else { vjp_dir<false, 0>(z, a, st); vjp_dir<false, 1>(z, a, st); vjp_dir<false, 2>(z, a, st); }
     return 0;
 }
 

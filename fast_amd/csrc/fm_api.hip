@@ -52,6 +52,7 @@ struct Zone {
     double* lu_dws;          // dW* between the pipelined LU sweeps
     int* lu_sync;            // tile counter and error word of the pipelined LU sweeps
     double* lu_xch;          // tile-boundary exchange of the pipelined LU sweeps (on first use)
+    double* adj_buf;         // face contributions of the transposed product (on first use)
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
     long n_metric;
@@ -115,6 +116,7 @@ static void free_zone(Zone* z) {
     if (z->lu_dws) (void)hipFree(z->lu_dws);
     if (z->lu_sync) (void)hipFree(z->lu_sync);
     if (z->lu_xch) (void)hipFree(z->lu_xch);
+    if (z->adj_buf) (void)hipFree(z->adj_buf);
     delete z;
 }
 
@@ -185,7 +187,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->p.assign(p, p + n_int);
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
-    z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr;
+    z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
     z->norm_part = z->norm_out = nullptr; z->have_norms = 0; z->have_cfl = 0;
     KZ& k = z->k;
     k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
@@ -686,6 +688,68 @@ extern "C" int fm_dpj_dpw(fm_ctx* c, int zone, int level, int idir, const int* w
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(FM_E_HIP, "fm_dpj_dpw: %s", hipGetErrorString(e));
+    return FM_OK;
+}
+
+// RHS arguments that make the residual kernels write the face balance alone into drodm: the
+// explicit path with coefW = 0 (its stage state goes to the scratch level and is discarded)
+static int balance_args(Zone* z, const double* rop, RhsArgs& a, int schedule) {
+    memset(&a, 0, sizeof a);
+    a.schedule = schedule;
+    if (z->k.kflu == 5) {
+        int rc = wall_corrections(z, a);
+        if (rc) return rc;
+    }
+    a.rop = rop; a.rop_n = rop; a.rop_n1 = rop;
+    a.implicit = 0; a.nitcfg = 1; a.zero_init = 1; a.coefH = 0.; a.coefW = 0.;
+    a.rk_in = rop; a.rk_out = z->ro_scratch;
+    return FM_OK;
+}
+
+extern "C" int fm_flux_balance(fm_ctx* c, int zone, int level, double* drodm) {
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2 || !drodm) return fail(FM_E_ARG, "fm_flux_balance: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    KZ& k = z->k;
+    hipStream_t st = c->stream;
+    const double* rop = z->ro[level];
+    // viscosity (and the coe array the update reads) of this state, then the SENSEUR sensor
+    Box b = {0, k.ni + 1, 0, k.nj + 1, 0, k.nk + 1};
+    launch_coe(k, rop, 0, 0, b, k.iflow == 2, st);
+    if (k.kflu == 2) launch_wiggle(k, rop, st);
+    RhsArgs a;
+    if (int rc = balance_args(z, rop, a, c->res_schedule)) return rc;
+    HIPCHK(hipMemsetAsync(k.drodm, 0, sizeof(double) * 5 * (size_t)k.nd, st));
+    if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(drodm, k.drodm, sizeof(double) * 5 * (size_t)k.nd, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return FM_OK;
+}
+
+extern "C" int fm_residual_vjp(fm_ctx* c, int zone, int level, const double* lam, double* out) {
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2 || !lam || !out) return fail(FM_E_ARG, "fm_residual_vjp: bad arguments");
+    KZ& k = z->k;
+    if (k.kflu != 1 || k.slope != 2 || k.iflow != 1)
+        return fail(FM_E_UNSUPPORTED, "fm_residual_vjp: AUSM+(P) o3 Euler zones only");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    if (!z->adj_buf) {
+        if (int rc = dmalloc(&z->adj_buf, vjp_buffer_doubles(k))) return rc;
+    }
+    size_t bytes = sizeof(double) * 5 * (size_t)k.nd;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * bytes));
+    VjpArgs a;
+    a.rop = z->ro[level]; a.lam = d; a.out = d + 5 * k.nd; a.buf = z->adj_buf;
+    hipError_t e = hipMemcpyAsync(d, lam, bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d + 5 * k.nd, 0, bytes, st);
+    if (e == hipSuccess) { launch_residual_vjp(k, a, st); e = hipGetLastError(); }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 5 * k.nd, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(FM_E_HIP, "fm_residual_vjp: %s", hipGetErrorString(e));
     return FM_OK;
 }
 

@@ -79,6 +79,11 @@ int launch_effort(const KZ& z, const EffArgs& a, hipStream_t st);
 // gradients of the pressure drag / lift coefficients at a slip-wall window (fm_adjoint.hip)
 struct DpjArgs { int idir, w[6]; double cosa, sina, surfinv; const double* rop; double* dcd; double* dcl; };
 int launch_dpj_dpw(const KZ& z, const DpjArgs& a, hipStream_t st);
+// out += (dR/dW)^T lam for the AUSM+(P) o3 Euler face balance (fm_adjoint.hip); buf holds
+// vjp_buffer_doubles(z) doubles
+struct VjpArgs { const double* rop; const double* lam; double* out; double* buf; };
+long vjp_buffer_doubles(const KZ& z);
+int launch_residual_vjp(const KZ& z, const VjpArgs& a, hipStream_t st);
 
 void launch_metric(const KZ& z, hipStream_t st);
 void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, int lamin, hipStream_t st);

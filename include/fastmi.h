@@ -105,6 +105,19 @@ int fm_wall_effort(fm_ctx* ctx, int zone, int level, int idir, const int* window
 int fm_dpj_dpw(fm_ctx* ctx, int zone, int level, int idir, const int* window, double cosa,
                double sina, double surfinv, double* dcd, double* dcl);
 
+/* The face balance of the interior cells alone, R_l = sum_d F_d(l) - F_d(l + inc_d) (the fluxes
+   the RHS kernels of Compute/navier_stokes_struct.for accumulate into drodm before the time
+   terms; the function fluausm_euler_o3_2d_b.for differentiates), from the state of a time
+   level.  drodm: 5 x NDIMDX doubles, zero outside the interior.  Synchronous. */
+int fm_flux_balance(fm_ctx* ctx, int zone, int level, double* drodm);
+
+/* Transposed Jacobian product of that balance: out = (dR/dW)^T lam, W the primitive state of
+   every cell of the level (ghost cells included), lam read on the interior cells (replaces the
+   Tapenade reverse routine ADJOINT/AUSM/2d/fluausm_euler_o3_2d_b.for, which exists for 2-D zones
+   only and is not called by the reference; stated here for 3-D zones).  lam, out: 5 x NDIMDX
+   doubles.  Zones: AUSM+(P), o3, Euler (FM_E_UNSUPPORTED otherwise).  Synchronous. */
+int fm_residual_vjp(fm_ctx* ctx, int zone, int level, const double* lam, double* out);
+
 /* Injection raccords between zones (1-to-1 matches), replacing any set before.  Raccord r
    fills nb_pts[r] receiver cells of zone no_r[r] from donor cells of zone no_d[r]; rcv_pts and
    dnr_pts hold the points of all raccords one after the other, as 0-based linear indices into

@@ -119,6 +119,31 @@ class _Backend:
             raise RuntimeError("%s dpj_dpw failed (%d)" % (self.prefix, rc))
         return dcd, dcl
 
+    def flux_balance(self, z, rop=None):
+        """The AUSM/Roe/SENSEUR face balance of the interior cells alone (5 x NDIMDX)."""
+        rop = z.ro if rop is None else rop
+        out = np.zeros(5 * z.nd)
+        f = getattr(self.lib, self.prefix + "flux_balance")
+        f.restype = ctypes.c_int
+        rc = f(_i(z.param_int), _d(z.param_real), _d(rop), _d(z.wig), _d(z.ti), _d(z.tj), _d(z.tk),
+               _d(z.vol), _d(z.xmut), _d(out))
+        if rc != 0:
+            raise RuntimeError("%s flux_balance failed (%d)" % (self.prefix, rc))
+        return out
+
+    def residual_vjp(self, z, lam, rop=None):
+        """(dR/dW)^T lam for the AUSM+(P) o3 Euler face balance R, W the primitive state of
+        every cell (5 x NDIMDX)."""
+        rop = z.ro if rop is None else rop
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros(5 * z.nd)
+        f = getattr(self.lib, self.prefix + "residual_vjp")
+        f.restype = ctypes.c_int
+        rc = f(_i(z.param_int), _d(z.param_real), _d(rop), _d(z.ti), _d(z.tj), _d(z.tk), _d(lam), _d(out))
+        if rc != 0:
+            raise RuntimeError("%s residual_vjp failed (%d)" % (self.prefix, rc))
+        return out
+
     def apply_bc(self, z, rop, nstep=1):
         f = getattr(self.lib, self.prefix + "apply_bc")
         rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nstep), _d(rop), _d(z.ti),

@@ -1217,6 +1217,167 @@ int fo_dpj_dpw(const int* p, const double* pr, int idir, const int* ind, const d
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* Transposed Jacobian product of the AUSM+(P) o3 Euler face balance: the reverse mode the
+   reference has for 2-D zones as Tapenade output (ADJOINT/AUSM/2d/fluausm_euler_o3_2d_b.for:
+   the gradient of drodm with respect to rop), stated here for 3-D zones.  out = (dR/dW)^T lam,
+   R_l = sum_d F_d(l) - F_d(l + inc_d) over the interior cells (the balance fo_flux forms),
+   W the primitive state of every cell, ghost cells included.  Per direction i, j, k: every
+   face (index 1..n_d + 1, interior across) is weighted by lam_l - lam_{l - inc} (each term only
+   where its cell is interior) and its reverse pass gives the contributions to the four stencil
+   cells; each cell then adds, in stencil-slot order, the contributions of the faces whose
+   stencil holds it.                                                                       */
+static void ausm_face_rev(const FC* f, const double q[4][5], const double tn[3], double sn,
+                          const double fb[5], double wb[4][5])
+{
+    double qp[5], qm[5];
+    for (int e = 0; e < 5; e++) {
+        qm[e] = f->c4 * q[2][e] + f->c5 * q[1][e] + f->c6 * q[3][e];
+        qp[e] = f->c4 * q[1][e] + f->c6 * q[0][e] + f->c5 * q[2][e];
+    }
+    double rgp = f->rgp, gam1 = f->gam1, r1 = qp[0], r2 = qm[0];
+    double mom1[3], mom2[3];
+    for (int m = 0; m < 3; m++) { mom1[m] = r1 * qp[1 + m]; mom2[m] = r2 * qm[1 + m]; }
+    double p1 = r1 * qp[4] * rgp, p2 = r2 * qm[4] * rgp;
+    double h1 = gam1 * p1 + .5 * (mom1[0] * qp[1] + mom1[1] * qp[2] + mom1[2] * qp[3]);
+    double h2 = gam1 * p2 + .5 * (mom2[0] * qm[1] + mom2[1] * qm[2] + mom2[2] * qm[3]);
+    double qn1 = qp[1] * tn[0] + qp[2] * tn[1] + qp[3] * tn[2];
+    double qn2 = qm[1] * tn[0] + qm[2] * tn[1] + qm[3] * tn[2];
+    double c = rgp * f->gam * q[2][4];
+    double son = sqrt(qn1 * qn1 / c);
+    double tam = f->c3 * son + sn;
+    double tam1 = fmax2(0., tam) * f->c2;
+    double u = 0.25 * (qn1 + qn2) - tam1 * (p2 - p1);
+    double tdu = fmax2(fabs(u), f->c1 * sn) * f->wig_cte;
+    double ub = fb[0] * (r1 + r2), tdub = -(fb[0] * (r2 - r1)), p1p2b = 0.;
+    double m1b[3], m2b[3];
+    for (int m = 0; m < 3; m++) {
+        ub = ub + fb[1 + m] * (mom1[m] + mom2[m]);
+        tdub = tdub - fb[1 + m] * (mom2[m] - mom1[m]);
+        m1b[m] = fb[1 + m] * (u + tdu);
+        m2b[m] = fb[1 + m] * (u - tdu);
+        p1p2b = p1p2b + fb[1 + m] * tn[m];
+    }
+    ub = ub + fb[4] * (h1 + h2);
+    tdub = tdub - fb[4] * (h2 - h1);
+    double r1b = fb[0] * (u + tdu), r2b = fb[0] * (u - tdu);
+    double h1b = fb[4] * (u + tdu), h2b = fb[4] * (u - tdu);
+    double qpb[5], qmb[5];
+    double p1b = gam1 * h1b + 0.5 * p1p2b, p2b = gam1 * h2b + 0.5 * p1p2b;
+    for (int m = 0; m < 3; m++) {
+        m1b[m] = m1b[m] + .5 * h1b * qp[1 + m];
+        m2b[m] = m2b[m] + .5 * h2b * qm[1 + m];
+        qpb[1 + m] = .5 * h1b * mom1[m];
+        qmb[1 + m] = .5 * h2b * mom2[m];
+    }
+    if (fabs(u) >= f->c1 * sn) ub = ub + tdub * f->wig_cte * (u >= 0. ? 1. : -1.);
+    double qn1b = 0.25 * ub, qn2b = 0.25 * ub;
+    double tam1b = -ub * (p2 - p1);
+    p1b = p1b + ub * tam1;
+    p2b = p2b - ub * tam1;
+    double tamb = tam >= 0. ? tam1b * f->c2 : 0.;
+    double sonb = f->c3 * tamb;
+    double sc = sqrt(c);
+    qn1b = qn1b + sonb * (qn1 >= 0. ? 1. / sc : -1. / sc);
+    double cb = sonb * (-0.5 * sqrt(qn1 * qn1) / sqrt(c * c * c));
+    r1b = r1b + p1b * qp[4] * rgp;
+    r2b = r2b + p2b * qm[4] * rgp;
+    for (int m = 0; m < 3; m++) {
+        r1b = r1b + m1b[m] * qp[1 + m];
+        r2b = r2b + m2b[m] * qm[1 + m];
+        qpb[1 + m] = qpb[1 + m] + qn1b * tn[m] + m1b[m] * r1;
+        qmb[1 + m] = qmb[1 + m] + qn2b * tn[m] + m2b[m] * r2;
+    }
+    qpb[0] = r1b; qmb[0] = r2b;
+    qpb[4] = p1b * r1 * rgp; qmb[4] = p2b * r2 * rgp;
+    for (int e = 0; e < 5; e++) {
+        wb[0][e] = f->c6 * qpb[e];
+        wb[1][e] = f->c4 * qpb[e] + f->c5 * qmb[e];
+        wb[2][e] = f->c5 * qpb[e] + f->c4 * qmb[e];
+        wb[3][e] = f->c6 * qmb[e];
+    }
+    wb[2][4] = wb[2][4] + cb * (rgp * f->gam);
+}
+
+int fo_residual_vjp(const int* p, const double* pr, const double* rop, const double* ti,
+                    const double* tj, const double* tk, const double* lam, double* out)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    FC f;
+    double vdum = 1.;
+    if (fc_init(&f, Z, p, pr, rop, NULL, ti, tj, tk, &vdum, NULL)) return -1;
+    if (f.kflu != 1 || f.slope != 2 || p[IFLOW] != 1) return -1;
+    long nd = Z->nd, ndm = Z->ndm;
+    int n[3] = {p[IJKV], p[IJKV + 1], p[IJKV + 2]};
+    for (long q = 0; q < 5 * nd; q++) out[q] = 0.;
+    for (int d = 0; d < 3; d++) {
+        long inc = d == 0 ? 1 : d == 1 ? Z->incj : Z->inck;
+        int e0 = n[0] + (d == 0), e1 = n[1] + (d == 1), e2 = n[2] + (d == 2);
+        long nf = (long)e0 * e1 * e2;
+        double* buf = (double*)malloc(sizeof(double) * 20 * (size_t)nf);
+        if (!buf) return -2;
+        long t = 0;
+        for (int k = 1; k <= e2; k++)
+            for (int j = 1; j <= e1; j++)
+                for (int i = 1; i <= e0; i++, t++) {
+                    long l = IDX(Z, i, j, k);
+                    int fi = d == 0 ? i : d == 1 ? j : k;
+                    double fb[5];
+                    for (int e = 0; e < 5; e++) {
+                        double a = fi <= n[d] ? lam[l + e * nd] : 0.;
+                        double b = fi >= 2 ? lam[l - inc + e * nd] : 0.;
+                        fb[e] = a - b;
+                    }
+                    double tn[3] = {0., 0., 0.}, sn;
+                    if (f.cart) { tn[d] = f.tc[d]; sn = f.s[d]; }
+                    else {
+                        const double* tm = d == 0 ? ti : d == 1 ? tj : tk;
+                        long lt = MTR(Z, i, j, k);
+                        tn[0] = tm[lt]; tn[1] = tm[lt + ndm]; tn[2] = tm[lt + 2 * ndm];
+                        sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+                    }
+                    double q[4][5], wb[4][5];
+                    for (int s = 0; s < 4; s++)
+                        for (int e = 0; e < 5; e++) q[s][e] = rop[l + (s - 2) * inc + e * nd];
+                    ausm_face_rev(&f, q, tn, sn, fb, wb);
+                    for (int s = 0; s < 4; s++)
+                        for (int e = 0; e < 5; e++) buf[(s * 5 + e) * nf + t] = wb[s][e];
+                }
+        /* cells -1 .. n_d + 2 along d, interior across */
+        int lo[3] = {1, 1, 1}, hi[3] = {n[0], n[1], n[2]};
+        lo[d] = -1; hi[d] = n[d] + 2;
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    int c = d == 0 ? i : d == 1 ? j : k;
+                    long l = IDX(Z, i, j, k);
+                    double acc[5] = {0., 0., 0., 0., 0.};
+                    for (int s = 0; s < 4; s++) {
+                        int fidx = c + 2 - s;
+                        if (fidx < 1 || fidx > n[d] + 1) continue;
+                        int fi = d == 0 ? fidx : i, fj = d == 1 ? fidx : j, fk = d == 2 ? fidx : k;
+                        long tf = (long)(fi - 1) + (long)(fj - 1) * e0 + (long)(fk - 1) * e0 * e1;
+                        for (int e = 0; e < 5; e++) acc[e] = acc[e] + buf[(s * 5 + e) * nf + tf];
+                    }
+                    for (int e = 0; e < 5; e++) out[l + e * nd] = out[l + e * nd] + acc[e];
+                }
+        free(buf);
+    }
+    return 0;
+}
+
+/* The face balance alone over the interior cells (the part of the RHS the transposed product
+   above differentiates): drodm = 0 then fo_flux. */
+int fo_flux_balance(const int* p, const double* pr, const double* rop, const double* wig,
+                    const double* ti, const double* tj, const double* tk, const double* vol,
+                    const double* xmut, double* drodm)
+{
+    Ix zz = mkix(p);
+    for (long q = 0; q < 5 * zz.nd; q++) drodm[q] = 0.;
+    int ind[6] = {1, p[IJKV], 1, p[IJKV + 1], 1, p[IJKV + 2]};
+    return fo_flux(p, pr, ind, rop, drodm, wig, ti, tj, tk, vol, xmut);
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* core3as2.for (neq = 5, non-2d): Newton/BDF residual                                   */
 static void fo_core3as2(const int* p, const double* pr, int nitcfg, int first_it,
                         const int* ind, const double* rop, const double* rop_n,

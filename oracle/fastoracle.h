@@ -59,6 +59,15 @@ int fo_dpj_dpw(const int* p, const double* pr, int idir, const int* ind, const d
                const double* ti, const double* tj, const double* tk, double cosa, double sina,
                double surfinv, double* dcd, double* dcl);
 
+/* out = (dR/dW)^T lam for the AUSM+(P) o3 Euler face balance R of the interior cells, W the
+   primitive state of every cell (5 x NDIMDX each) */
+int fo_residual_vjp(const int* p, const double* pr, const double* rop, const double* ti,
+                    const double* tj, const double* tk, const double* lam, double* out);
+/* the face balance alone (drodm zeroed first) */
+int fo_flux_balance(const int* p, const double* pr, const double* rop, const double* wig,
+                    const double* ti, const double* tj, const double* tk, const double* vol,
+                    const double* xmut, double* drodm);
+
 #ifdef __cplusplus
 }
 #endif

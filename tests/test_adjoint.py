@@ -195,3 +195,135 @@ def test_compute_dpj_dpw_pytree(orc):
             got = Internal.getNodeFromName1(fs, v)[1].ravel(order="F")
             assert np.array_equal(got, ref[e * zo.nd:(e + 1) * zo.nd]), v
     assert np.count_nonzero(dcd) > 0
+
+
+# ------------------------------------------------------------------------------------------
+# transposed Jacobian product of the AUSM+(P) o3 Euler face balance
+VJP_CASES = {
+    "jittered": lambda num: H.jittered_box((6, 5, 4), num, flow="euler"),
+    "cart": lambda num: cases.tgv_zone((6, 5, 4), num, perturb=0.05, flow="euler"),
+    "walls": lambda num: H.jittered_box((5, 6, 4), num, bcs=SLIP_ALL, flow="euler"),
+}
+
+
+def vjp_zone(orc, case):
+    z = VJP_CASES[case](H.numerics("ausmpred", "bdf"))
+    orc.init_metric(z)
+    orc.apply_bc(z, z.ro)
+    return z
+
+
+def scaled_direction(z, rng):
+    scale = np.abs(z.ro.reshape(5, -1)).max(axis=1) * np.array([1e-1, 1e-2, 1e-2, 1e-2, 1e-1])
+    return rng.standard_normal(5 * z.nd) * np.repeat(scale, z.nd)
+
+
+@pytest.mark.parametrize("case", sorted(VJP_CASES))
+def test_residual_vjp_dot_product(orc, case):
+    """<lam, J v> over the interior (J v by central differences of the face balance) equals
+    <J^T lam, v> for random directions v over every cell, ghosts included; the difference error
+    falls with the step (1e-4 -> 1e-5) to the 1e-9 level."""
+    z = vjp_zone(orc, case)
+    rng = np.random.default_rng(1)
+    lam = rng.standard_normal(5 * z.nd)
+    g = orc.residual_vjp(z, lam)
+    for _ in range(3):
+        v = scaled_direction(z, rng)
+        errs = []
+        for h in (1e-4, 1e-5):
+            jv = (orc.flux_balance(z, z.ro + h * v) - orc.flux_balance(z, z.ro - h * v)) / (2 * h)
+            lhs = np.dot(z.interior(lam).ravel(), z.interior(jv).ravel())
+            rhs = np.dot(g, v)
+            errs.append(abs(lhs - rhs) / abs(rhs))
+        assert errs[1] < 1e-9, errs
+
+
+def test_residual_vjp_entries(orc):
+    """Single entries of J^T lam, interior and ghost cells, against central differences of
+    <lam, R(W + h e_m)>; cells outside every stencil (edge ghosts) get exactly zero."""
+    z = vjp_zone(orc, "jittered")
+    nd = z.nd
+    rng = np.random.default_rng(2)
+    lam = rng.standard_normal(5 * nd)
+    g = orc.residual_vjp(z, lam)
+    ni, nj, nk = z.dims
+    idx = lambda i, j, k: (i + 1) + (j + 1) * (ni + 4) + (k + 1) * (ni + 4) * (nj + 4)
+    cells = [idx(1, 1, 1), idx(3, 2, 2), idx(ni, nj, nk), idx(0, 2, 2), idx(-1, 3, 1),
+             idx(ni + 2, 1, 3), idx(2, nj + 1, 2), idx(3, 3, -1)]
+    for c in cells:
+        for e in range(5):
+            m = c + e * nd
+            h = 1e-5 * max(abs(z.ro[m]), 1e-2)
+            w = z.ro.copy(); w[m] += h
+            rp = orc.flux_balance(z, w)
+            w[m] -= 2 * h
+            rm = orc.flux_balance(z, w)
+            fd = np.dot(z.interior(lam).ravel(), z.interior((rp - rm) / (2 * h)).ravel())
+            assert abs(fd - g[m]) <= 1e-7 * max(abs(g[m]), np.abs(g).max() * 1e-3), (c, e, fd, g[m])
+    assert g[idx(0, 0, 1)] == 0. and g[idx(-1, nj + 2, 3)] == 0.
+
+
+def test_residual_vjp_ignores_ghost_lambda(orc):
+    """lam is read on the interior cells only (R is defined there)."""
+    z = vjp_zone(orc, "cart")
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(5 * z.nd)
+    lam2 = rng.standard_normal(5 * z.nd)
+    z.interior(lam2)[...] = z.interior(lam)
+    assert np.array_equal(orc.residual_vjp(z, lam), orc.residual_vjp(z, lam2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(VJP_CASES))
+def test_residual_vjp_gpu(orc, case):
+    """fm_residual_vjp and fm_flux_balance equal the oracle bit for bit."""
+    from fast_amd import _lib
+    zo = vjp_zone(orc, case)
+    zg = VJP_CASES[case](H.numerics("ausmpred", "bdf"))
+    ctx = _lib.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    lam = np.random.default_rng(4).standard_normal(5 * zo.nd)
+    assert np.array_equal(ctx.residual_vjp(zid, lam), orc.residual_vjp(zo, lam))
+    rg, ro_ = ctx.flux_balance(zid), orc.flux_balance(zo)
+    assert np.array_equal(zo.interior(rg), zo.interior(ro_))
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["ausmpred", "senseur", "roe_min"])
+def test_flux_balance_gpu(orc, scheme):
+    """fm_flux_balance (the RHS kernels with the time terms left out) equals the oracle's face
+    balance on the interior, bit for bit, for every flux family (laminar: viscous fluxes too)."""
+    from fast_amd import _lib
+    num = H.numerics(scheme, "bdf")
+    mk = lambda: H.jittered_box((7, 6, 5), num, bcs=H.WALL_J)
+    zo, zg = mk(), mk()
+    orc.init_metric(zo)
+    orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    orc.substep(zo, 1, first_it=0, stop=1)        # viscosity and SENSEUR sensor of this state
+    ctx = _lib.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    assert np.array_equal(zo.interior(ctx.flux_balance(zid)), zo.interior(orc.flux_balance(zo)))
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_flux_balance_autograd(orc):
+    """torch.autograd through fast_amd.adjoint.flux_balance: the gradient of <lam, R(W)> is the
+    oracle's transposed product, bit for bit."""
+    import torch
+    from fast_amd import _lib, adjoint
+    zo = vjp_zone(orc, "walls")
+    zg = VJP_CASES["walls"](H.numerics("ausmpred", "bdf"))
+    ctx = _lib.Context(0)
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    W = torch.from_numpy(zo.ro.copy()).to("cuda").requires_grad_(True)
+    lam = np.random.default_rng(5).standard_normal(5 * zo.nd)
+    R = adjoint.flux_balance(W, ctx, zid)
+    (R * torch.from_numpy(lam).to("cuda")).sum().backward()
+    assert np.array_equal(W.grad.cpu().numpy(), orc.residual_vjp(zo, lam))
+    ctx.close()
