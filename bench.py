@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--scheme", default="ausmpred")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal timing")
     ap.add_argument("--phase-reps", type=int, default=5)
     ap.add_argument("--lu", choices=["pipeline", "blocks"], default="blocks")
     ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="kcol")
@@ -129,6 +130,26 @@ def cpu_baseline(n, scheme, iters):
 # normals and volume to the residual and to each LU sweep; local time stepping reads no BDF levels)
 BYTES_FULL_LOCAL = {"coe": 168.0, "residual": 160.0, "lu": 400.0, "bc": 0.0, "iteration": 560.0}
 BYTES_CART_LOCAL = {"coe": 88.0, "residual": 80.0, "lu": 240.0, "bc": 0.0, "iteration": 320.0}
+
+
+def adjoint_ratio(n, scheme, reps):
+    """Time of the transposed residual product (fm_residual_vjp) over the primal residual, both
+    on the Euler variant of the workload (the transposed product covers the AUSM+(P) o3 Euler
+    face balance)."""
+    from fast_amd import _lib, cases, params as P
+    num = P.Numerics(scheme=scheme, slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="global", time_step=1.0)
+    z = cases.tgv_zone(n, num, flow="euler")
+    z.param_real[P.DTC] = cases.tgv_dt(z, cfl=2.0)
+    ctx = _lib.Context(0)
+    zid = ctx.add_zone(z)
+    del z
+    ctx.apply_bc(zid, 0)
+    res = ctx.time_phase(zid, _lib.PHASE_RESIDUAL, reps=reps)
+    vjp = ctx.time_phase(zid, _lib.PHASE_VJP, reps=reps)
+    ctx.close()
+    return {"workload": "TGV %d^3 Euler, %s o3: residual vs transposed product (dR/dW)^T lam" % (n, scheme),
+            "residual_ms": round(res, 4), "vjp_ms": round(vjp, 4), "ratio": round(vjp / res, 3)}
 
 
 def zones64_case():
@@ -280,6 +301,8 @@ def main():
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "event_ms_per_step": round(ms_ev / a.steps, 3),
            "roofline": roofline}
+    if not a.no_adjoint and a.scheme == "ausmpred":
+        out["adjoint"] = adjoint_ratio(a.n, a.scheme, a.phase_reps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.n, a.scheme, a.cpu_iters)
     print(json.dumps(out))

@@ -114,7 +114,9 @@ def newZoneFromHost(hz, base, bcs=None):
     zbc = Internal.createChild(z, "ZoneBC", "ZoneBC_t")
     pt = int(p[P.PT_BC])
     nb = int(p[pt]) if pt > 0 else 0
-    inv = {v: k for k, v in P.BC_TAGS.items()}
+    inv = {}
+    for k, v in P.BC_TAGS.items():          # the first name of each type (BCWallInviscid, not its alias)
+        inv.setdefault(v, k)
     for n in range(nb):
         q = int(p[pt + 1 + n])
         typ, idir = int(p[q]), int(p[q + 1])

@@ -50,7 +50,7 @@ SIGNATURES = {
     "fm_flux_balance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
     "fm_residual_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
 }
-PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC = range(4)
+PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC, PHASE_VJP = range(5)
 OPT_LU_SCHEDULE = 1
 LU_BLOCKS, LU_PIPELINE = 0, 1
 OPT_RESIDUAL_SCHEDULE = 2

@@ -5,7 +5,9 @@ cells (the fluxes the reference's RHS kernels accumulate into drodm), W the prim
 every cell of the zone (5 x NDIMDX, ghost cells included, float64).  Its backward is the
 transposed Jacobian product of ``fm_residual_vjp`` (the reverse mode the reference holds for 2-D
 zones as ADJOINT/AUSM/2d/fluausm_euler_o3_2d_b.for).  Both run on the GPU through the C ABI; the
-zone's time level ``level`` (default 0) is the working state and is overwritten with W.
+zone's time level ``level`` (default 0) is the working state and is overwritten with W.  W and
+the gradients are host (CPU) tensors: the library brings its own HIP runtime, so torch's CUDA
+device is not used in the same process.
 """
 import numpy as np
 import torch

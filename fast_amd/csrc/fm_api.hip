@@ -1132,6 +1132,9 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
+    if (phase == FM_PHASE_VJP && !z->adj_buf) {
+        if (int rc = dmalloc(&z->adj_buf, vjp_buffer_doubles(k))) return rc;
+    }
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipEventRecord(c->e0, st));
     for (int r = 0; r < reps; r++) {
@@ -1145,6 +1148,11 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
             int shift[6], sdm[6] = {1, k.ni, 1, k.nj, 1, k.nk};
             bczone(c, z, 0, z->ro_scratch, shift, nullptr);
             coins(c, z, z->ro_scratch, sdm);
+        } else if (phase == FM_PHASE_VJP) {
+            // the transposed product with lam = the current drodm, into the scratch level
+            VjpArgs va;
+            va.rop = ro; va.lam = k.drodm; va.out = z->ro_scratch; va.buf = z->adj_buf;
+            if (launch_residual_vjp(k, va, st)) return fail(FM_E_UNSUPPORTED, "no transposed product for this scheme");
         } else return fail(FM_E_ARG, "unknown phase %d", phase);
     }
     HIPCHK(hipEventRecord(c->e1, st));

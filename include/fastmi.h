@@ -158,7 +158,8 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    zone `reps` times on the context stream between two HIP events and returns the mean
    milliseconds per run.  phase: FM_PHASE_COE (viscosity + time step / LU coefficients),
    FM_PHASE_RESIDUAL (fused fluxes + balance + BDF update), FM_PHASE_LU (lower + upper sweeps
-   with the Newton update), FM_PHASE_BC (ghost-cell fill).  No counterpart in the reference. */
+   with the Newton update), FM_PHASE_BC (ghost-cell fill), FM_PHASE_VJP (the transposed product
+   of fm_residual_vjp, both passes and three directions).  No counterpart in the reference. */
 /* Context options.  FM_OPT_LU_SCHEDULE selects how the LU-SGS sweeps are scheduled on the
    GPU (both give results bit-identical to the reference's lexicographic sweeps):
    FM_LU_BLOCKS (default) one launch per diagonal of 8x8x8 blocks;
@@ -189,6 +190,7 @@ int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
 #define FM_PHASE_RESIDUAL 1
 #define FM_PHASE_LU 2
 #define FM_PHASE_BC 3
+#define FM_PHASE_VJP 4
 int fm_time_phase(fm_ctx* ctx, int zone, int phase, int reps, float* ms_per_run);
 
 #ifdef __cplusplus
