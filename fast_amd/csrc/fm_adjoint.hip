@@ -188,12 +188,15 @@ int launch_dpj_dpw(const KZ& z, const DpjArgs& a, hipStream_t st) {
 // of the interior cells (R_l = sum_d F_d(l) - F_d(l + inc_d)), W the primitive state of every
 // cell, ghost cells included.
 //
-// Pass 1, one thread per face of direction D (face index 1..n_D + 1, interior across): the
-// weight of the face is fb = lambda_l - lambda_{l - inc} (each term only where its cell is
-// interior) and the face's reverse pass gives its contribution to the four stencil cells,
-// stored face-major.  Pass 2, one thread per cell of the slab (index -1..n_D + 2 along D,
-// interior across): the contributions of the (at most four) faces whose stencil holds the cell,
-// in stencil-slot order, are added to out.  Directions run i, j, k.
+// Every face of direction D (face index 1..n_D + 1, interior across) is weighted by
+// fb = lambda_l - lambda_{l - inc} (each term only where its cell is interior); its reverse pass
+// gives its contributions to the four stencil cells, and each cell (index -1..n_D + 2 along D)
+// adds those of the faces c - 1, c, c + 1, c + 2 in that order, then adds the sum to out.
+// Directions run i, j, k.  Kernels:
+// * i: one workgroup per (j, k) row, the row's face contributions staged in LDS;
+// * j, k: one thread per line, marching up the line with the four-cell stencil and the four
+//   partial sums in registers (a cell is complete, and written, once its last face is done);
+// * rows too long for LDS: face contributions through a global buffer, then a gather pass.
 
 // reverse pass of the whole AUSM+(P) o3 face flux (AUSM fluFaceEuler.for): fb = weights of the
 // five flux components; wb[s] = d(fb . F)/dq[s] for the stencil (nm2, nm, l, np)
@@ -342,7 +345,7 @@ __global__ void k_vjp_gather(KZ z, VjpArgs a) {
     const long l = IDX(z, i, j, k), nd = z.nd;
     double acc[5] = {0., 0., 0., 0., 0.};
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+    for (int s = 3; s >= 0; s--) {                // faces c - 1, c, c + 1, c + 2
         const int fidx = c + 2 - s;               // the face whose stencil slot s is this cell
         if (fidx < 1 || fidx > nD + 1) continue;
         const int fi = D == 0 ? fidx : i, fj = D == 1 ? fidx : j, fk = D == 2 ? fidx : k;
@@ -354,8 +357,156 @@ __global__ void k_vjp_gather(KZ z, VjpArgs a) {
     for (int e = 0; e < 5; e++) a.out[l + e * nd] = a.out[l + e * nd] + acc[e];
 }
 
+// weight of face fi (index along the line) from the adjoint of its two cells
+__device__ __forceinline__ void face_weight(const double lp[5], const double lm[5], int fi, int n, double fb[5]) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        double p = fi <= n ? lp[e] : 0.;
+        double m = fi >= 2 ? lm[e] : 0.;
+        fb[e] = p - m;
+    }
+}
+
+template <bool CART, int D>
+__device__ __forceinline__ void face_normal(const KZ& z, const FluxConst& f, int i, int j, int k,
+                                            double tn[3], double& sn) {
+    tn[0] = tn[1] = tn[2] = 0.;
+    if (CART) {
+        tn[D] = f.tc[D]; sn = f.s[D];
+    } else {
+        const double* tm = D == 0 ? z.ti : D == 1 ? z.tj : z.tk;
+        const long lt = MTR(z, i, j, k), ndm = z.ndm;
+        tn[0] = tm[lt]; tn[1] = tm[lt + ndm]; tn[2] = tm[lt + 2 * ndm];
+        sn = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+    }
+}
+
+// i direction: one workgroup per (j, k) row; LDS holds the row's 20 x (ni + 1) contributions
+template <bool CART>
+__global__ void __launch_bounds__(256) k_vjp_row(KZ z, VjpArgs a) {
+    extern __shared__ double sh[];
+    const int j = 1 + (int)(blockIdx.x % z.nj), k = 1 + (int)(blockIdx.x / z.nj);
+    const int n = z.ni, nf = n + 1;
+    const long nd = z.nd;
+    const FluxConst f = flux_const(z);
+    for (int fi = 1 + (int)threadIdx.x; fi <= nf; fi += blockDim.x) {
+        const long l = IDX(z, fi, j, k);
+        double lp[5], lm[5], fb[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+            lp[e] = fi <= n ? a.lam[l + e * nd] : 0.;
+            lm[e] = fi >= 2 ? a.lam[l - 1 + e * nd] : 0.;
+        }
+        face_weight(lp, lm, fi, n, fb);
+        double tn[3], sn;
+        face_normal<CART, 0>(z, f, fi, j, k, tn, sn);
+        double q[4][5], wb[4][5];
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+            for (int e = 0; e < 5; e++) q[s2][e] = a.rop[l + (s2 - 2) + e * nd];
+        ausm_face_rev(f, q, tn, sn, fb, wb);
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+            for (int e = 0; e < 5; e++) sh[(s2 * 5 + e) * nf + (fi - 1)] = wb[s2][e];
+    }
+    __syncthreads();
+    for (int c = -1 + (int)threadIdx.x; c <= n + 2; c += blockDim.x) {
+        const long l = IDX(z, c, j, k);
+        double acc[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int s2 = 3; s2 >= 0; s2--) {
+            const int fidx = c + 2 - s2;
+            if (fidx < 1 || fidx > nf) continue;
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] + sh[(s2 * 5 + e) * nf + (fidx - 1)];
+        }
+#pragma unroll
+        for (int e = 0; e < 5; e++) a.out[l + e * nd] = a.out[l + e * nd] + acc[e];
+    }
+}
+
+// j or k direction: one thread per line (i fastest across threads), marching the faces of the line
+template <bool CART, int D>
+__global__ void __launch_bounds__(256) k_vjp_march(KZ z, VjpArgs a) {
+    const int no = D == 1 ? z.nk : z.nj;
+    const long nl = (long)z.ni * no;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nl) return;
+    const int i = 1 + (int)(t % z.ni), o = 1 + (int)(t / z.ni);
+    const int n = D == 1 ? z.nj : z.nk;
+    const long nd = z.nd, inc = D == 1 ? z.incj : z.inck;
+    const FluxConst f = flux_const(z);
+    auto cell = [&](int c) -> long { return D == 1 ? IDX(z, i, c, o) : IDX(z, i, o, c); };
+    double q[4][5], acc[4][5], lm[5];
+    const long l0 = cell(1);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) { q[s2][e] = a.rop[l0 + (s2 - 2) * inc + e * nd]; acc[s2][e] = 0.; }
+#pragma unroll
+    for (int e = 0; e < 5; e++) lm[e] = 0.;
+    for (int fi = 1; fi <= n + 1; fi++) {
+        const long l = l0 + (long)(fi - 1) * inc;
+        double lp[5], fb[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) lp[e] = fi <= n ? a.lam[l + e * nd] : 0.;
+        face_weight(lp, lm, fi, n, fb);
+        double tn[3], sn;
+        if (D == 1) face_normal<CART, 1>(z, f, i, fi, o, tn, sn);
+        else face_normal<CART, 2>(z, f, i, o, fi, tn, sn);
+        double wb[4][5];
+        ausm_face_rev(f, q, tn, sn, fb, wb);
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[s2][e] = acc[s2][e] + wb[s2][e];
+        // cell fi - 2 has had its last face
+        const long lc = l - 2 * inc;
+#pragma unroll
+        for (int e = 0; e < 5; e++) a.out[lc + e * nd] = a.out[lc + e * nd] + acc[0][e];
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+            acc[0][e] = acc[1][e]; acc[1][e] = acc[2][e]; acc[2][e] = acc[3][e]; acc[3][e] = 0.;
+            q[0][e] = q[1][e]; q[1][e] = q[2][e]; q[2][e] = q[3][e];
+            lm[e] = lp[e];
+        }
+        if (fi <= n) {
+#pragma unroll
+            for (int e = 0; e < 5; e++) q[3][e] = a.rop[l + 2 * inc + e * nd];
+        }
+    }
+    // cells n, n + 1, n + 2
+    const long ln = cell(n);
+#pragma unroll
+    for (int s2 = 0; s2 < 3; s2++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) a.out[ln + s2 * inc + e * nd] = a.out[ln + s2 * inc + e * nd] + acc[s2][e];
+}
+
+constexpr int kVjpRowMax = 400;    // longest i row staged in LDS (20 x (ni + 1) doubles <= 64 KiB)
+
+template <bool CART, int D>
+static void vjp_two_pass(const KZ& z, const VjpArgs& a, hipStream_t st);
+
 template <bool CART, int D>
 static void vjp_dir(const KZ& z, const VjpArgs& a, hipStream_t st) {
+    if (D == 0 && z.ni <= kVjpRowMax) {
+        size_t lds = sizeof(double) * 20 * (size_t)(z.ni + 1);
+        hipLaunchKernelGGL((k_vjp_row<CART>), dim3((unsigned)((long)z.nj * z.nk)), dim3(256), lds, st, z, a);
+        return;
+    }
+    if (D == 1 || D == 2) {
+        long nl = (long)z.ni * (D == 1 ? z.nk : z.nj);
+        hipLaunchKernelGGL((k_vjp_march<CART, D>), dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, z, a);
+        return;
+    }
+    vjp_two_pass<CART, D>(z, a, st);
+}
+
+template <bool CART, int D>
+static void vjp_two_pass(const KZ& z, const VjpArgs& a, hipStream_t st) {
     int n0, n1, n2;
     n0 = z.ni + (D == 0); n1 = z.nj + (D == 1); n2 = z.nk + (D == 2);
     long nf = (long)n0 * n1 * n2;
@@ -365,9 +516,8 @@ static void vjp_dir(const KZ& z, const VjpArgs& a, hipStream_t st) {
 }
 
 long vjp_buffer_doubles(const KZ& z) {
-    long a = (long)(z.ni + 1) * z.nj * z.nk, b = (long)z.ni * (z.nj + 1) * z.nk, c = (long)z.ni * z.nj * (z.nk + 1);
-    long m = a > b ? a : b;
-    return 20 * (m > c ? m : c);
+    // only the i direction of rows too long for LDS goes through the buffer
+    return z.ni <= kVjpRowMax ? 0 : 20 * (long)(z.ni + 1) * z.nj * z.nk;
 }
 
 int launch_residual_vjp(const KZ& z, const VjpArgs& a, hipStream_t st) {

@@ -1224,8 +1224,8 @@ int fo_dpj_dpw(const int* p, const double* pr, int idir, const int* ind, const d
    W the primitive state of every cell, ghost cells included.  Per direction i, j, k: every
    face (index 1..n_d + 1, interior across) is weighted by lam_l - lam_{l - inc} (each term only
    where its cell is interior) and its reverse pass gives the contributions to the four stencil
-   cells; each cell then adds, in stencil-slot order, the contributions of the faces whose
-   stencil holds it.                                                                       */
+   cells; each cell then adds the contributions of the faces whose stencil holds it, in face
+   order (faces c - 1, c, c + 1, c + 2 of cell c), and adds that sum to out.               */
 static void ausm_face_rev(const FC* f, const double q[4][5], const double tn[3], double sn,
                           const double fb[5], double wb[4][5])
 {
@@ -1351,7 +1351,7 @@ int fo_residual_vjp(const int* p, const double* pr, const double* rop, const dou
                     int c = d == 0 ? i : d == 1 ? j : k;
                     long l = IDX(Z, i, j, k);
                     double acc[5] = {0., 0., 0., 0., 0.};
-                    for (int s = 0; s < 4; s++) {
+                    for (int s = 3; s >= 0; s--) {       /* faces c - 1, c, c + 1, c + 2 */
                         int fidx = c + 2 - s;
                         if (fidx < 1 || fidx > n[d] + 1) continue;
                         int fi = d == 0 ? fidx : i, fj = d == 1 ? fidx : j, fk = d == 2 ? fidx : k;

@@ -203,6 +203,8 @@ VJP_CASES = {
     "jittered": lambda num: H.jittered_box((6, 5, 4), num, flow="euler"),
     "cart": lambda num: cases.tgv_zone((6, 5, 4), num, perturb=0.05, flow="euler"),
     "walls": lambda num: H.jittered_box((5, 6, 4), num, bcs=SLIP_ALL, flow="euler"),
+    # i rows longer than the LDS-staged kernel takes: the two-pass path
+    "long_i": lambda num: H.jittered_box((405, 3, 2), num, flow="euler"),
 }
 
 
