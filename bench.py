@@ -149,7 +149,9 @@ def adjoint_ratio(n, scheme, reps):
     vjp = ctx.time_phase(zid, _lib.PHASE_VJP, reps=reps)
     ctx.close()
     return {"workload": "TGV %d^3 Euler, %s o3: residual vs transposed product (dR/dW)^T lam" % (n, scheme),
-            "residual_ms": round(res, 4), "vjp_ms": round(vjp, 4), "ratio": round(vjp / res, 3)}
+            "residual_ms": round(res, 4), "vjp_ms": round(vjp, 4), "ratio": round(vjp / res, 3),
+            "vjp_gbs_algorithmic": round(480.0 * n ** 3 / (vjp * 1e-3) / 1e9, 1),
+            "vjp_traffic": traffic_from_profiles("adjoint/vjp")}
 
 
 def zones64_case():

@@ -156,7 +156,7 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 
 /* Per-phase timing for the roofline report: runs one phase of the first sub-iteration of a
    zone `reps` times on the context stream between two HIP events and returns the mean
-   milliseconds per run.  phase: FM_PHASE_COE (viscosity + time step / LU coefficients),
+   milliseconds per run (after one untimed run).  phase: FM_PHASE_COE (viscosity + time step / LU coefficients),
    FM_PHASE_RESIDUAL (fused fluxes + balance + BDF update), FM_PHASE_LU (lower + upper sweeps
    with the Newton update), FM_PHASE_BC (ghost-cell fill), FM_PHASE_VJP (the transposed product
    of fm_residual_vjp, both passes and three directions).  No counterpart in the reference. */
