@@ -146,6 +146,8 @@ __device__ __forceinline__ void lu_prod_global(const KZ& z, const LUC& c, const 
 // the sweep reads (one cell per lane, lanes on different i) hit 16 distinct 8-byte banks.
 template <bool CART, int CI>
 struct LuLds {
+    // W 0-4, coe1..4 times the sweep sign 5-8, 1/coe5 9, right-hand side 10-14 (upper sweep:
+    // times coe5), [3dfull: normals 15-23]
     static constexpr int NV = CART ? 15 : 24;
     double st[NV][CI][64];
     double hI[5][64];
@@ -189,12 +191,16 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         if (oj >= nj || ok >= nk || ii >= ci) continue;
         const int i = I(oi0 + ii), j = J(oj), k = K(ok);
         const long l = IDX(z, i, j, k);
+        // the sweep's divisor, sign products and scaled right-hand side, formed here off the
+        // wavefront's critical path (the same operations as in the solve)
+        const double cf4 = z.coe[l + 4 * nd];
 #pragma unroll
         for (int v = 0; v < 5; v++) {
             S.st[v][ii][L] = rop[l + v * nd];
-            S.st[5 + v][ii][L] = z.coe[l + v * nd];
-            S.st[10 + v][ii][L] = dw[l + v * nd];
+            if (v < 4) S.st[5 + v][ii][L] = z.coe[l + v * nd] * signe;
+            S.st[10 + v][ii][L] = LOWER ? dw[l + v * nd] : dw[l + v * nd] * cf4;
         }
+        S.st[9][ii][L] = 1. / cf4;
         if (!CART) {
             const long lt = MTR(z, i, j, k);
             double tx, ty, tz;
@@ -246,19 +252,35 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
     // them in registers across the stores to dw
     const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
     const int nsteps = ci + 14;
+    // Newton update of the cell solved at a step: before the first barrier when the k products
+    // read the updated state (in-place sub-iterations), else after it, beside the product waves
+    const bool newton = !LOWER && a.newton;
+    auto newton_update = [&](long l, const double x[5], double r0, double u0, double v0, double w0,
+                             double t0) {
+        double* __restrict__ O = a.out;
+        const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
+        const double r1 = r0 + x[0];
+        const double r_1 = 1. / r1;
+        const double u1 = (r0 * u0 + x[1]) * r_1, v1 = (r0 * v0 + x[2]) * r_1, w1 = (r0 * w0 + x[3]) * r_1;
+        const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
+        const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+        O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
+        if (kalias) { S.wk[0][L] = u1; S.wk[1][L] = v1; S.wk[2][L] = w1; S.wk[3][L] = t1; }
+    };
     for (int s = 0; s < nsteps; s++) {
         const int ii = s - jj - kk;
         const bool act = lane_ok && ii >= 0 && ii < ci;
+        double x[5], r0 = 0., u0 = 0., v0 = 0., w0 = 0., t0 = 0.;
+        long l = 0;
         if (wv == 0 && act) {
             const int oi = oi0 + ii;
-            const long l = IDX(z, I(oi), j, k);
-            const double r0 = S.st[0][ii][L], u0 = S.st[1][ii][L], v0 = S.st[2][ii][L],
-                         w0 = S.st[3][ii][L], t0 = S.st[4][ii][L];
-            const double cf0 = S.st[5][ii][L], cf4 = S.st[9][ii][L];
-            const double xal = cf0 * signe;
-            double x[5], b[5];
+            l = IDX(z, I(oi), j, k);
+            r0 = S.st[0][ii][L]; u0 = S.st[1][ii][L]; v0 = S.st[2][ii][L];
+            w0 = S.st[3][ii][L]; t0 = S.st[4][ii][L];
+            const double xal = S.st[5][ii][L];
+            double b[5];
 #pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = LOWER ? S.st[10 + e][ii][L] : S.st[10 + e][ii][L] * cf4;
+            for (int e = 0; e < 5; e++) x[e] = S.st[10 + e][ii][L];
             // neighbour terms in the reference order: lower 3dcart J,K,I; lower 3dfull and
             // upper K,J,I
 #pragma unroll
@@ -279,22 +301,13 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
             }
-            const double diag = 1. / cf4;
+            const double diag = S.st[9][ii][L];
 #pragma unroll
             for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; S.xs[e][L] = x[e]; }
-            if (!LOWER && a.newton) {
-                double* __restrict__ O = a.out;
-                const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
-                const double r1 = r0 + x[0];
-                const double r_1 = 1. / r1;
-                const double u1 = (r0 * u0 + x[1]) * r_1, v1 = (r0 * v0 + x[2]) * r_1, w1 = (r0 * w0 + x[3]) * r_1;
-                const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
-                const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
-                O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
-                if (kalias) { S.wk[0][L] = u1; S.wk[1][L] = v1; S.wk[2][L] = w1; S.wk[3][L] = t1; }
-            }
+            if (newton && kalias) newton_update(l, x, r0, u0, v0, w0, t0);
         }
         __syncthreads();
+        if (wv == 0 && act && newton && !kalias) newton_update(l, x, r0, u0, v0, w0, t0);
         if (wv > 0 && act) {
             // products of the cell just solved for its three consumers
             double x[5], b[5];
@@ -305,16 +318,16 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             if (wv == 1) {
                 if (CART) { tx = tci; ty = tz = 0.; }
                 else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
-                lu_prod<CART, 0>(c, uu, vv, ww, tt, S.st[6][ii][L] * signe, tx, ty, tz, x, b);
+                lu_prod<CART, 0>(c, uu, vv, ww, tt, S.st[6][ii][L], tx, ty, tz, x, b);
             } else if (wv == 2) {
                 if (CART) { tx = tcj; ty = tz = 0.; }
                 else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
-                lu_prod<CART, 1>(c, uu, vv, ww, tt, S.st[7][ii][L] * signe, tx, ty, tz, x, b);
+                lu_prod<CART, 1>(c, uu, vv, ww, tt, S.st[7][ii][L], tx, ty, tz, x, b);
             } else {
                 if (kalias) { uu = S.wk[0][L]; vv = S.wk[1][L]; ww = S.wk[2][L]; tt = S.wk[3][L]; }
                 if (CART) { tx = tck; ty = tz = 0.; }
                 else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
-                lu_prod<CART, 2>(c, uu, vv, ww, tt, S.st[8][ii][L] * signe, tx, ty, tz, x, b);
+                lu_prod<CART, 2>(c, uu, vv, ww, tt, S.st[8][ii][L], tx, ty, tz, x, b);
             }
 #pragma unroll
             for (int e = 0; e < 5; e++) S.px[wv - 1][e][L] = b[e];
