@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--phase-reps", type=int, default=5)
     ap.add_argument("--lu", choices=["pipeline", "blocks"], default="blocks")
     ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="kcol")
+    ap.add_argument("--lu-precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fp32: mixed precision, LU-SGS sweep arithmetic in FP32 (state, residual FP64)")
     ap.add_argument("--workload", choices=["tgv256", "channel", "zones64"], default="tgv256",
                     help="tgv256: the headline line; channel: 257x129x33 curvilinear channel with a "
                          "bump (3dfull, local time step, CFL 10); zones64: 64 abutting blocks "
@@ -245,6 +247,7 @@ def main():
     ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
                    {"cells": _lib.RES_CELLS, "faces": _lib.RES_FACES, "kcol": _lib.RES_KCOL}[a.residual])
     ctx.set_option(_lib.OPT_GRAPH, a.graph)
+    ctx.set_option(_lib.OPT_LU_PRECISION, _lib.LU_FP32 if a.lu_precision == "fp32" else _lib.LU_FP64)
     zid = ctx.add_zone(z)
     ctx.apply_bc(zid, 0)
     buf = np.zeros_like(z.ro)
@@ -293,10 +296,12 @@ def main():
     out = {"metric": "Mcell-iter/s NS residual+LU-SGS at 256^3", "value": round(value, 2),
            "unit": "Mcell-iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(step_ms, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic (Taylor-Green vortex initial field)",
+           "vs_baseline": None, "dtype": "f64" if a.lu_precision == "fp64" else "f64 (LU-SGS sweeps f32)",
+           "data": "synthetic (Taylor-Green vortex initial field)",
            "config": {"workload": "TGV %d^3 implicit BDF residual (%s o3 + laminar viscous) + LU-SGS, FP64"
                       % (a.n, a.scheme), "cells": ncell, "zones": 1, "mesh": "3dcart periodic",
                       "lu_schedule": a.lu, "residual_schedule": a.residual, "graph": a.graph,
+                      "lu_precision": a.lu_precision,
                       "autotuned": None if tuned is None else {k: {n: round(t, 4) for n, t in v[1].items()}
                                                                for k, v in tuned.items()}},
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),

@@ -51,6 +51,8 @@ SIGNATURES = {
     "fm_residual_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC, PHASE_VJP = range(5)
+OPT_LU_PRECISION = 5
+LU_FP64, LU_FP32 = 0, 1
 OPT_LU_SCHEDULE = 1
 LU_BLOCKS, LU_PIPELINE = 0, 1
 OPT_RESIDUAL_SCHEDULE = 2

@@ -49,7 +49,7 @@ def tgv_zone(n, numerics=None, perturb=0.0, seed=0, flow="lamin", name="tgv", bc
 
 
 def tgv_dt(zone, cfl=0.5):
-    """Explicit time step dt = cfl * h / max(|u| + c) from the field."""
+    """Explicit time step dt = cfl * h / max(|u| + c) from the field (the TGV 64^3 explicit case)."""
     st = zone.state
     nd = zone.nd
     ro = zone.interior(zone.ro)

@@ -75,6 +75,7 @@ struct fm_ctx {
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
     int lu_batch;            // batch the LU of several zones (default on)
+    int lu_fp32;             // mixed precision: LU-SGS sweep arithmetic in FP32 (default off)
     // time steps captured as HIP graphs (FM_OPT_GRAPH): one per distinct configuration
     struct GraphEntry {
         std::vector<const void*> ptrs;  // every zone's three level pointers
@@ -99,6 +100,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     c->lu_schedule = LU_BLOCKS;
     c->res_schedule = RES_KCOL;
     c->lu_batch = 1;
+    c->lu_fp32 = 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
     HIPCHK(hipEventCreate(&c->e0));
@@ -564,7 +566,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.i0 = shift_lu[0]; la.i1 = shift_lu[1]; la.j0 = shift_lu[2]; la.j1 = shift_lu[3];
             la.k0 = shift_lu[4]; la.k1 = shift_lu[5];
             la.dw = k.drodm; la.newton = 1;
-            la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
+            la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
             if (int rc = ensure_exchange(c, z)) return rc;
             la.xch = z->lu_xch;
             la.rop = ro_ssiter;
@@ -858,8 +860,8 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
                 ge->tabs.push_back({key, d});
             }
             if (!capture) HIPCHK(hipMemcpy(d, tab.data(), tab.size() * sizeof(LuZone), hipMemcpyHostToDevice));
-            launch_lu_list(cart == 1, true, d, L->work, L->offs, st);
-            launch_lu_list(cart == 1, false, d, L->work, L->offs, st);
+            launch_lu_list(cart == 1, true, c->lu_fp32, d, L->work, L->offs, st);
+            launch_lu_list(cart == 1, false, c->lu_fp32, d, L->work, L->offs, st);
             HIPCHK(hipGetLastError());
             continue;
         }
@@ -880,8 +882,8 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         memcpy(c->lu_host[q], tab.data(), tab.size() * sizeof(LuZone));
         HIPCHK(hipMemcpyAsync(c->lu_tab, c->lu_host[q], tab.size() * sizeof(LuZone), hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(c->lu_ev[q], st));
-        launch_lu_list(cart == 1, true, c->lu_tab, L->work, L->offs, st);
-        launch_lu_list(cart == 1, false, c->lu_tab, L->work, L->offs, st);
+        launch_lu_list(cart == 1, true, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
+        launch_lu_list(cart == 1, false, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
         HIPCHK(hipGetLastError());
     }
     for (size_t n = 0; n < zs.size(); n++) {
@@ -1065,6 +1067,12 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
         c->graph_epoch++;
         return FM_OK;
     }
+    if (option == FM_OPT_LU_PRECISION) {
+        if (value != FM_LU_FP64 && value != FM_LU_FP32) return fail(FM_E_ARG, "LU precision %d", value);
+        c->lu_fp32 = value == FM_LU_FP32;
+        c->graph_epoch++;
+        return FM_OK;
+    }
     if (option == FM_OPT_GRAPH) {
         if (value != 0 && value != 1) return fail(FM_E_ARG, "graph %d", value);
         c->graph_on = value;
@@ -1128,7 +1136,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     LuArgs la;
     la.i0 = 1; la.i1 = k.ni; la.j0 = 1; la.j1 = k.nj; la.k0 = 1; la.k1 = k.nk;
     la.dw = k.drodm; la.rop = ro; la.rop_k = ro; la.out = z->ro_scratch; la.newton = 1;
-    la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule;
+    la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};

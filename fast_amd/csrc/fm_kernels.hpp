@@ -45,6 +45,8 @@ struct LuArgs {
     int* sync;
     double* xch;                  // pipelined schedule: tile-boundary exchange buffer
     int schedule;                 // LU_BLOCKS or LU_PIPELINE
+    int fp32;                     // blocks schedule: sweep arithmetic in FP32 (state, right-hand
+                                  // side, results and Newton update stay FP64)
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1 };
 
@@ -94,7 +96,7 @@ void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
 struct LuZone { KZ z; LuArgs a; };
 struct LuWork { int zone, bi, bj, bk; };
 void lu_work_list(bool cart, const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs);
-void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
+void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);
 // doubles of the tile-boundary exchange buffer the pipelined sweeps of an LU box need

@@ -35,54 +35,58 @@ template <bool CART> constexpr int lu_ci() { return CART ? FM_LU_CI_CART : FM_LU
 
 namespace fm {
 
-struct LUC { double gam1, gamm1, cp, gam2; };
+// T: the arithmetic type of the sweeps (double; float for the mixed-precision option, whose
+// state, right-hand side, Newton update and results stay double)
+template <typename T = double>
+struct LUC { T gam1, gamm1, cp, gam2; };
 
-__device__ __forceinline__ LUC lu_const(const KZ& z) {
-    LUC c;
-    c.gam1 = z.gam; c.gamm1 = c.gam1 - 1.; c.cp = z.gam * z.cv; c.gam2 = c.gamm1 - 1.;
+template <typename T = double>
+__device__ __forceinline__ LUC<T> lu_const(const KZ& z) {
+    double gam1 = z.gam, gamm1 = gam1 - 1., cp = z.gam * z.cv, gam2 = gamm1 - 1.;
+    LUC<T> c;
+    c.gam1 = (T)gam1; c.gamm1 = (T)gamm1; c.cp = (T)cp; c.gam2 = (T)gam2;
     return c;
 }
 
 // b = B_D(u, v, w, t; cs, face normal) d   (invlu_l.for / invlu_u.for neighbour term)
-template <bool CART, int D>
-__device__ __forceinline__ void lu_prod(const LUC& c, double u, double v, double w, double t,
-                                        double cs, double tcx, double tcy, double tcz,
-                                        const double d[5], double b[5]) {
-    double q2 = .5 * (u * u + v * v + w * w);
-    double h = c.cp * t + q2;
-    double ph2 = c.gamm1 * q2;
-    double gam1 = c.gam1, gamm1 = c.gamm1, gam2 = c.gam2;
-    double d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3], d5 = d[4];
+template <bool CART, int D, typename T = double>
+__device__ __forceinline__ void lu_prod(const LUC<T>& c, T u, T v, T w, T t, T cs, T tcx, T tcy, T tcz,
+                                        const T d[5], T b[5]) {
+    T q2 = (T).5 * (u * u + v * v + w * w);
+    T h = c.cp * t + q2;
+    T ph2 = c.gamm1 * q2;
+    T gam1 = c.gam1, gamm1 = c.gamm1, gam2 = c.gam2;
+    T d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3], d5 = d[4];
     if (CART) {
-        double tc = tcx;
+        T tc = tcx;
         if (D == 0) {
-            double qn = tc * u;
-            double b21 = (tc * ph2 - u * qn), b22 = (qn - tc * gam2 * u) + cs, b23 = (-tc * gamm1 * v);
-            double b24 = (-tc * gamm1 * w), b25 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = (tc * h - gamm1 * u * qn), b53 = (-gamm1 * v * qn);
-            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+            T qn = tc * u;
+            T b21 = (tc * ph2 - u * qn), b22 = (qn - tc * gam2 * u) + cs, b23 = (-tc * gamm1 * v);
+            T b24 = (-tc * gamm1 * w), b25 = tc * gamm1;
+            T b51 = qn * (ph2 - h), b52 = (tc * h - gamm1 * u * qn), b53 = (-gamm1 * v * qn);
+            T b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
             b[0] = cs * d1 + tc * d2;
             b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
             b[2] = (-v * qn) * d1 + (tc * v) * d2 + ((qn) + cs) * d3;
             b[3] = (-w * qn) * d1 + (tc * w) * d2 + ((qn) + cs) * d4;
             b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
         } else if (D == 1) {
-            double qn = tc * v;
-            double b31 = (tc * ph2 - v * qn), b32 = (-tc * gamm1 * u), b33 = (qn - tc * gam2 * v) + cs;
-            double b34 = (-tc * gamm1 * w), b35 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = (-gamm1 * u * qn), b53 = (tc * h - gamm1 * v * qn);
-            double b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+            T qn = tc * v;
+            T b31 = (tc * ph2 - v * qn), b32 = (-tc * gamm1 * u), b33 = (qn - tc * gam2 * v) + cs;
+            T b34 = (-tc * gamm1 * w), b35 = tc * gamm1;
+            T b51 = qn * (ph2 - h), b52 = (-gamm1 * u * qn), b53 = (tc * h - gamm1 * v * qn);
+            T b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
             b[0] = cs * d1 + tc * d3;
             b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d3;
             b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
             b[3] = (-w * qn) * d1 + (tc * w) * d3 + ((qn) + cs) * d4;
             b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
         } else {
-            double qn = tc * w;
-            double b41 = tc * ph2 - w * qn, b42 = -tc * gamm1 * u, b43 = -tc * gamm1 * v;
-            double b44 = (qn - tc * gam2 * w) + cs, b45 = tc * gamm1;
-            double b51 = qn * (ph2 - h), b52 = -gamm1 * u * qn, b53 = -gamm1 * v * qn;
-            double b54 = tc * h - gamm1 * w * qn, b55 = gam1 * qn + cs;
+            T qn = tc * w;
+            T b41 = tc * ph2 - w * qn, b42 = -tc * gamm1 * u, b43 = -tc * gamm1 * v;
+            T b44 = (qn - tc * gam2 * w) + cs, b45 = tc * gamm1;
+            T b51 = qn * (ph2 - h), b52 = -gamm1 * u * qn, b53 = -gamm1 * v * qn;
+            T b54 = tc * h - gamm1 * w * qn, b55 = gam1 * qn + cs;
             b[0] = cs * d1 + tc * d4;
             b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d4;
             b[2] = (-v * qn) * d1 + (qn + cs) * d3 + (tc * v) * d4;
@@ -91,15 +95,15 @@ __device__ __forceinline__ void lu_prod(const LUC& c, double u, double v, double
         }
         return;
     }
-    double qn = tcx * u + tcy * v + tcz * w;
-    double b21 = (tcx * ph2 - u * qn), b22 = (qn - tcx * gam2 * u) + cs;
-    double b23 = (tcy * u - tcx * gamm1 * v), b24 = (tcz * u - tcx * gamm1 * w), b25 = tcx * gamm1;
-    double b31 = (tcy * ph2 - v * qn), b32 = (tcx * v - tcy * gamm1 * u);
-    double b33 = (qn - tcy * gam2 * v) + cs, b34 = (tcz * v - tcy * gamm1 * w), b35 = tcy * gamm1;
-    double b41 = (tcz * ph2 - w * qn), b42 = (tcx * w - tcz * gamm1 * u);
-    double b43 = (tcy * w - tcz * gamm1 * v), b44 = (qn - tcz * gam2 * w) + cs, b45 = tcz * gamm1;
-    double b51 = qn * (ph2 - h), b52 = (tcx * h - gamm1 * u * qn), b53 = (tcy * h - gamm1 * v * qn);
-    double b54 = (tcz * h - gamm1 * w * qn), b55 = (gam1 * qn) + cs;
+    T qn = tcx * u + tcy * v + tcz * w;
+    T b21 = (tcx * ph2 - u * qn), b22 = (qn - tcx * gam2 * u) + cs;
+    T b23 = (tcy * u - tcx * gamm1 * v), b24 = (tcz * u - tcx * gamm1 * w), b25 = tcx * gamm1;
+    T b31 = (tcy * ph2 - v * qn), b32 = (tcx * v - tcy * gamm1 * u);
+    T b33 = (qn - tcy * gam2 * v) + cs, b34 = (tcz * v - tcy * gamm1 * w), b35 = tcy * gamm1;
+    T b41 = (tcz * ph2 - w * qn), b42 = (tcx * w - tcz * gamm1 * u);
+    T b43 = (tcy * w - tcz * gamm1 * v), b44 = (qn - tcz * gam2 * w) + cs, b45 = tcz * gamm1;
+    T b51 = qn * (ph2 - h), b52 = (tcx * h - gamm1 * u * qn), b53 = (tcy * h - gamm1 * v * qn);
+    T b54 = (tcz * h - gamm1 * w * qn), b55 = (gam1 * qn) + cs;
     b[0] = cs * d1 + tcx * d2 + tcy * d3 + tcz * d4;
     b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
     b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
@@ -125,37 +129,37 @@ __device__ __forceinline__ void lu_normal(const KZ& z, long lt, double& tcx, dou
 }
 
 // Product of a neighbour finished by an earlier launch, read back from global memory.
-template <bool CART, int D>
-__device__ __forceinline__ void lu_prod_global(const KZ& z, const LUC& c, const double* __restrict__ rop,
+template <bool CART, int D, typename T = double>
+__device__ __forceinline__ void lu_prod_global(const KZ& z, const LUC<T>& c, const double* __restrict__ rop,
                                                const double* __restrict__ dw, int i, int j, int k,
-                                               double signe, double b[5]) {
+                                               double signe, T b[5]) {
     long nd = z.nd, l1 = IDX(z, i, j, k);
     double tcx, tcy, tcz;
     lu_normal<CART, D>(z, CART ? 0 : MTR(z, i, j, k), tcx, tcy, tcz);
-    double d[5];
+    T d[5];
 #pragma unroll
-    for (int e = 0; e < 5; e++) d[e] = dw[l1 + e * nd];
+    for (int e = 0; e < 5; e++) d[e] = (T)dw[l1 + e * nd];
     double cs = z.coe[l1 + (1 + D) * nd] * signe;
-    lu_prod<CART, D>(c, rop[l1 + nd], rop[l1 + 2 * nd], rop[l1 + 3 * nd], rop[l1 + 4 * nd], cs,
-                     tcx, tcy, tcz, d, b);
+    lu_prod<CART, D, T>(c, (T)rop[l1 + nd], (T)rop[l1 + 2 * nd], (T)rop[l1 + 3 * nd], (T)rop[l1 + 4 * nd],
+                        (T)cs, (T)tcx, (T)tcy, (T)tcz, d, b);
 }
 
 // LDS image of one block: the block's own fields (state, coefficients, right-hand side and,
 // for 3dfull, the three mean face normals), the products of the halo cells finished by earlier
 // launches, and the exchange between the solving wave and the product waves.  Rows are indexed by lane so that
 // the sweep reads (one cell per lane, lanes on different i) hit 16 distinct 8-byte banks.
-template <bool CART, int CI>
+template <bool CART, int CI, typename T = double>
 struct LuLds {
     // W 0-4, coe1..4 times the sweep sign 5-8, 1/coe5 9, right-hand side 10-14 (upper sweep:
     // times coe5), [3dfull: normals 15-23]
     static constexpr int NV = CART ? 15 : 24;
-    double st[NV][CI][64];
-    double hI[5][64];
-    double hJ[5][CI][8];
-    double hK[5][CI][8];
-    double xs[5][64];        // result of the cell each lane solved at this step
-    double wk[4][64];        // its Newton-updated state (upper sweep in place)
-    double px[3][5][64];     // its i, j, k products
+    T st[NV][CI][64];
+    T hI[5][64];
+    T hJ[5][CI][8];
+    T hK[5][CI][8];
+    T xs[5][64];             // result of the cell each lane solved at this step
+    T wk[4][64];             // its Newton-updated state (upper sweep in place)
+    T px[3][5][64];          // its i, j, k products
 };
 
 // One block of CI x 8 x 8 cells (i x j x k, in sweep order) per launch diagonal.
@@ -164,17 +168,18 @@ struct LuLds {
 //  2. the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles cell
 //     ii = s - jj - kk at step s; wave 0 solves, waves 1-3 form the products; results go
 //     straight to global memory.
-template <bool CART, bool LOWER, int CI>
+template <bool CART, bool LOWER, int CI, typename T = double>
 __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, int bj, int bk,
-                                         LuLds<CART, CI>& S) {
-    constexpr int NV = LuLds<CART, CI>::NV;
+                                         LuLds<CART, CI, T>& S) {
+    constexpr int NV = LuLds<CART, CI, T>::NV;
+    constexpr bool F64 = sizeof(T) == sizeof(double);
     const int tid = threadIdx.x;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     const int oi0 = bi * CI;
     const int ci = ni - oi0 < CI ? ni - oi0 : CI;
     const int oj0 = bj * 8, ok0 = bk * 8;
     const long nd = z.nd;
-    const LUC c = lu_const(z);
+    const LUC<T> c = lu_const<T>(z);
     const double signe = LOWER ? 0.5 : -0.5;
     const double* __restrict__ rop = a.rop;
     const double* __restrict__ ropk = LOWER ? a.rop : a.rop_k;
@@ -196,41 +201,41 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         const double cf4 = z.coe[l + 4 * nd];
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            S.st[v][ii][L] = rop[l + v * nd];
-            if (v < 4) S.st[5 + v][ii][L] = z.coe[l + v * nd] * signe;
-            S.st[10 + v][ii][L] = LOWER ? dw[l + v * nd] : dw[l + v * nd] * cf4;
+            S.st[v][ii][L] = (T)rop[l + v * nd];
+            if (v < 4) S.st[5 + v][ii][L] = (T)(z.coe[l + v * nd] * signe);
+            S.st[10 + v][ii][L] = (T)(LOWER ? dw[l + v * nd] : dw[l + v * nd] * cf4);
         }
-        S.st[9][ii][L] = 1. / cf4;
+        S.st[9][ii][L] = (T)(1. / cf4);
         if (!CART) {
             const long lt = MTR(z, i, j, k);
             double tx, ty, tz;
             lu_normal<CART, 0>(z, lt, tx, ty, tz);
-            S.st[15][ii][L] = tx; S.st[16][ii][L] = ty; S.st[17][ii][L] = tz;
+            S.st[15][ii][L] = (T)tx; S.st[16][ii][L] = (T)ty; S.st[17][ii][L] = (T)tz;
             lu_normal<CART, 1>(z, lt, tx, ty, tz);
-            S.st[18][ii][L] = tx; S.st[19][ii][L] = ty; S.st[20][ii][L] = tz;
+            S.st[18][ii][L] = (T)tx; S.st[19][ii][L] = (T)ty; S.st[20][ii][L] = (T)tz;
             lu_normal<CART, 2>(z, lt, tx, ty, tz);
-            S.st[21][ii][L] = tx; S.st[22][ii][L] = ty; S.st[23][ii][L] = tz;
+            S.st[21][ii][L] = (T)tx; S.st[22][ii][L] = (T)ty; S.st[23][ii][L] = (T)tz;
         }
     }
     // 1b. products of the halo cells (finished by earlier launches)
     for (int e = tid; e < 64 + 16 * CI; e += 256) {
-        double b[5];
+        T b[5];
         if (e < 64) {
             const int L = e, oj = oj0 + (L & 7), ok = ok0 + (L >> 3);
             if (oi0 == 0 || oj >= nj || ok >= nk) continue;
-            lu_prod_global<CART, 0>(z, c, rop, dw, I(oi0 - 1), J(oj), K(ok), signe, b);
+            lu_prod_global<CART, 0, T>(z, c, rop, dw, I(oi0 - 1), J(oj), K(ok), signe, b);
 #pragma unroll
             for (int v = 0; v < 5; v++) S.hI[v][L] = b[v];
         } else if (e < 64 + 8 * CI) {
             const int q = e - 64, ii = q / 8, kk = q % 8, ok = ok0 + kk;
             if (oj0 == 0 || ii >= ci || ok >= nk) continue;
-            lu_prod_global<CART, 1>(z, c, rop, dw, I(oi0 + ii), J(oj0 - 1), K(ok), signe, b);
+            lu_prod_global<CART, 1, T>(z, c, rop, dw, I(oi0 + ii), J(oj0 - 1), K(ok), signe, b);
 #pragma unroll
             for (int v = 0; v < 5; v++) S.hJ[v][ii][kk] = b[v];
         } else {
             const int q = e - 64 - 8 * CI, ii = q / 8, jj = q % 8, oj = oj0 + jj;
             if (ok0 == 0 || ii >= ci || oj >= nj) continue;
-            lu_prod_global<CART, 2>(z, c, ropk, dw, I(oi0 + ii), J(oj), K(ok0 - 1), signe, b);
+            lu_prod_global<CART, 2, T>(z, c, ropk, dw, I(oi0 + ii), J(oj), K(ok0 - 1), signe, b);
 #pragma unroll
             for (int v = 0; v < 5; v++) S.hK[v][ii][jj] = b[v];
         }
@@ -250,7 +255,7 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
     const bool kalias = !LOWER && a.rop_k != a.rop;
     // the Cartesian face normals are read once: inside the loop the compiler could not keep
     // them in registers across the stores to dw
-    const double tci = CART ? z.ti[0] : 0., tcj = CART ? z.tj[0] : 0., tck = CART ? z.tk[0] : 0.;
+    const T tci = (T)(CART ? z.ti[0] : 0.), tcj = (T)(CART ? z.tj[0] : 0.), tck = (T)(CART ? z.tk[0] : 0.);
     const int nsteps = ci + 14;
     // Newton update of the cell solved at a step: before the first barrier when the k products
     // read the updated state (in-place sub-iterations), else after it, beside the product waves
@@ -265,20 +270,29 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         const double roe_old = r0 * (z.cv * t0 + 0.5 * (u0 * u0 + v0 * v0 + w0 * w0));
         const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
         O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
-        if (kalias) { S.wk[0][L] = u1; S.wk[1][L] = v1; S.wk[2][L] = w1; S.wk[3][L] = t1; }
+        if (kalias) { S.wk[0][L] = (T)u1; S.wk[1][L] = (T)v1; S.wk[2][L] = (T)w1; S.wk[3][L] = (T)t1; }
     };
     for (int s = 0; s < nsteps; s++) {
         const int ii = s - jj - kk;
         const bool act = lane_ok && ii >= 0 && ii < ci;
-        double x[5], r0 = 0., u0 = 0., v0 = 0., w0 = 0., t0 = 0.;
+        T x[5];
+        double xd[5], r0 = 0., u0 = 0., v0 = 0., w0 = 0., t0 = 0.;
         long l = 0;
         if (wv == 0 && act) {
             const int oi = oi0 + ii;
             l = IDX(z, I(oi), j, k);
-            r0 = S.st[0][ii][L]; u0 = S.st[1][ii][L]; v0 = S.st[2][ii][L];
-            w0 = S.st[3][ii][L]; t0 = S.st[4][ii][L];
-            const double xal = S.st[5][ii][L];
-            double b[5];
+            if (newton) {
+                // the Newton update works on the double state: the staged copy (FP64 sweeps)
+                // or global memory (mixed precision)
+                if (F64) {
+                    r0 = (double)S.st[0][ii][L]; u0 = (double)S.st[1][ii][L]; v0 = (double)S.st[2][ii][L];
+                    w0 = (double)S.st[3][ii][L]; t0 = (double)S.st[4][ii][L];
+                } else {
+                    r0 = rop[l]; u0 = rop[l + nd]; v0 = rop[l + 2 * nd]; w0 = rop[l + 3 * nd]; t0 = rop[l + 4 * nd];
+                }
+            }
+            const T xal = S.st[5][ii][L];
+            T b[5];
 #pragma unroll
             for (int e = 0; e < 5; e++) x[e] = S.st[10 + e][ii][L];
             // neighbour terms in the reference order: lower 3dcart J,K,I; lower 3dfull and
@@ -301,33 +315,35 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
             }
-            const double diag = S.st[9][ii][L];
+            const T diag = S.st[9][ii][L];
 #pragma unroll
-            for (int e = 0; e < 5; e++) { x[e] = x[e] * diag; dw[l + e * nd] = x[e]; S.xs[e][L] = x[e]; }
-            if (newton && kalias) newton_update(l, x, r0, u0, v0, w0, t0);
+            for (int e = 0; e < 5; e++) {
+                x[e] = x[e] * diag; xd[e] = (double)x[e]; dw[l + e * nd] = xd[e]; S.xs[e][L] = x[e];
+            }
+            if (newton && kalias) newton_update(l, xd, r0, u0, v0, w0, t0);
         }
         __syncthreads();
-        if (wv == 0 && act && newton && !kalias) newton_update(l, x, r0, u0, v0, w0, t0);
+        if (wv == 0 && act && newton && !kalias) newton_update(l, xd, r0, u0, v0, w0, t0);
         if (wv > 0 && act) {
             // products of the cell just solved for its three consumers
-            double x[5], b[5];
+            T b[5];
 #pragma unroll
             for (int e = 0; e < 5; e++) x[e] = S.xs[e][L];
-            double uu = S.st[1][ii][L], vv = S.st[2][ii][L], ww = S.st[3][ii][L], tt = S.st[4][ii][L];
-            double tx, ty, tz;
+            T uu = S.st[1][ii][L], vv = S.st[2][ii][L], ww = S.st[3][ii][L], tt = S.st[4][ii][L];
+            T tx, ty, tz;
             if (wv == 1) {
-                if (CART) { tx = tci; ty = tz = 0.; }
+                if (CART) { tx = tci; ty = tz = (T)0.; }
                 else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
-                lu_prod<CART, 0>(c, uu, vv, ww, tt, S.st[6][ii][L], tx, ty, tz, x, b);
+                lu_prod<CART, 0, T>(c, uu, vv, ww, tt, S.st[6][ii][L], tx, ty, tz, x, b);
             } else if (wv == 2) {
-                if (CART) { tx = tcj; ty = tz = 0.; }
+                if (CART) { tx = tcj; ty = tz = (T)0.; }
                 else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
-                lu_prod<CART, 1>(c, uu, vv, ww, tt, S.st[7][ii][L], tx, ty, tz, x, b);
+                lu_prod<CART, 1, T>(c, uu, vv, ww, tt, S.st[7][ii][L], tx, ty, tz, x, b);
             } else {
                 if (kalias) { uu = S.wk[0][L]; vv = S.wk[1][L]; ww = S.wk[2][L]; tt = S.wk[3][L]; }
-                if (CART) { tx = tck; ty = tz = 0.; }
+                if (CART) { tx = tck; ty = tz = (T)0.; }
                 else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
-                lu_prod<CART, 2>(c, uu, vv, ww, tt, S.st[8][ii][L], tx, ty, tz, x, b);
+                lu_prod<CART, 2, T>(c, uu, vv, ww, tt, S.st[8][ii][L], tx, ty, tz, x, b);
             }
 #pragma unroll
             for (int e = 0; e < 5; e++) S.px[wv - 1][e][L] = b[e];
@@ -337,25 +353,25 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
     (void)NV;
 }
 
-template <bool CART, bool LOWER, int CI>
+template <bool CART, bool LOWER, int CI, typename T = double>
 __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int bk_lo, int nbi, int nbj) {
     const int bj = blockIdx.x, bk = bk_lo + blockIdx.y, bi = diag - bj - bk;
     if (bi < 0 || bi >= nbi || bj >= nbj) return;
-    __shared__ LuLds<CART, CI> S;
-    lu_block<CART, LOWER, CI>(z, a, bi, bj, bk, S);
+    __shared__ LuLds<CART, CI, T> S;
+    lu_block<CART, LOWER, CI, T>(z, a, bi, bj, bk, S);
 }
 
 // Several zones in one launch per block diagonal: block b of the launch takes its zone and its
 // block coordinates from the work list (zones are independent: each keeps its own diagonal
 // order, diagonal d of every zone goes into launch d).
-template <bool CART, bool LOWER, int CI>
+template <bool CART, bool LOWER, int CI, typename T = double>
 __global__ __launch_bounds__(256) void k_lu_block_list(const LuZone* __restrict__ zones,
                                                        const LuWork* __restrict__ work) {
     const LuWork w = work[blockIdx.x];
-    __shared__ LuLds<CART, CI> S;
+    __shared__ LuLds<CART, CI, T> S;
     const KZ z = zones[w.zone].z;
     const LuArgs a = zones[w.zone].a;
-    lu_block<CART, LOWER, CI>(z, a, w.bi, w.bj, w.bk, S);
+    lu_block<CART, LOWER, CI, T>(z, a, w.bi, w.bj, w.bk, S);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -457,7 +473,7 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
     const int j = LOWER ? a.j0 + oj : a.j1 - oj, k = LOWER ? a.k0 + ok : a.k1 - ok;
     const int sgn = LOWER ? -1 : 1;
     const long nd = z.nd, ndm = z.ndm;
-    const LUC c = lu_const(z);
+    const LUC<> c = lu_const(z);
     const double signe = LOWER ? 0.5 : -0.5;
     const double* __restrict__ rop = a.rop;
     const bool kalias = !LOWER && a.rop_k != a.rop;
@@ -660,7 +676,8 @@ void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
         int hi = d < nbk - 1 ? d : nbk - 1;
         int jhi = d < nbj - 1 ? d : nbj - 1;
         dim3 g(jhi + 1, hi - lo + 1);
-        hipLaunchKernelGGL((k_lu_block<CART, LOWER, CI>), g, dim3(256), 0, st, z, a, d, lo, nbi, nbj);
+        if (a.fp32) hipLaunchKernelGGL((k_lu_block<CART, LOWER, CI, float>), g, dim3(256), 0, st, z, a, d, lo, nbi, nbj);
+        else hipLaunchKernelGGL((k_lu_block<CART, LOWER, CI>), g, dim3(256), 0, st, z, a, d, lo, nbi, nbj);
     }
 }
 }  // namespace
@@ -713,14 +730,22 @@ void lu_work_list(bool cart, const std::vector<Box>& boxes, std::vector<LuWork>&
     }
 }
 
-void launch_lu_list(bool cart, bool lower, const LuZone* zones, const LuWork* work,
+void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st) {
     constexpr int CC = lu_ci<true>(), CF = lu_ci<false>();
     for (size_t d = 0; d + 1 < offs.size(); d++) {
         int n = offs[d + 1] - offs[d];
         if (n <= 0) continue;
         const LuWork* w = work + offs[d];
-        if (cart) {
+        if (fp32) {
+            if (cart) {
+                if (lower) hipLaunchKernelGGL((k_lu_block_list<true, true, CC, float>), dim3(n), dim3(256), 0, st, zones, w);
+                else hipLaunchKernelGGL((k_lu_block_list<true, false, CC, float>), dim3(n), dim3(256), 0, st, zones, w);
+            } else {
+                if (lower) hipLaunchKernelGGL((k_lu_block_list<false, true, CF, float>), dim3(n), dim3(256), 0, st, zones, w);
+                else hipLaunchKernelGGL((k_lu_block_list<false, false, CF, float>), dim3(n), dim3(256), 0, st, zones, w);
+            }
+        } else if (cart) {
             if (lower) hipLaunchKernelGGL((k_lu_block_list<true, true, CC>), dim3(n), dim3(256), 0, st, zones, w);
             else hipLaunchKernelGGL((k_lu_block_list<true, false, CC>), dim3(n), dim3(256), 0, st, zones, w);
         } else {

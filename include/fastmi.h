@@ -184,6 +184,15 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    afterwards (blocks LU schedule); 0 (default) launches every kernel.  Same results.  Changing
    options, param_real, zones or transfers drops the captured graphs. */
 #define FM_OPT_GRAPH 4
+/* FM_OPT_LU_PRECISION: FM_LU_FP64 (default) runs the LU-SGS sweeps in FP64, bit-identical to the
+   reference; FM_LU_FP32 runs the sweep arithmetic (products, solve, LDS image) in FP32 while the
+   state, the right-hand side (Newton residual), the increments written back and the Newton
+   update stay FP64 (mixed precision; blocks schedule only).  LU-SGS is the approximate factor
+   of a defect correction, so the converged Newton residual is still FP64; the increments
+   differ from the FP64 sweeps at the FP32 rounding level. */
+#define FM_OPT_LU_PRECISION 5
+#define FM_LU_FP64 0
+#define FM_LU_FP32 1
 int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
 
 #define FM_PHASE_COE 0
