@@ -175,6 +175,7 @@ def run_other(a):
     ctx = _lib.Context(0)
     ctx.set_option(_lib.OPT_ZONE_BATCH, a.zone_batch)
     ctx.set_option(_lib.OPT_GRAPH, a.graph)
+    ctx.set_option(_lib.OPT_LU_PRECISION, _lib.LU_FP32 if a.lu_precision == "fp32" else _lib.LU_FP64)
     if a.workload == "channel":
         zones, racs = [cases.channel_zone(257, 129, 33)], []
         desc = ("channel with a bump 257x129x33 (3dfull, jittered), laminar AUSM+ o3, implicit LU-SGS, "
@@ -209,7 +210,8 @@ def run_other(a):
         raise RuntimeError("non-finite state after the benchmark")
     out = {"metric": "Mcell-iter/s NS residual+LU-SGS", "value": round(value, 2), "unit": "Mcell-iter/s",
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if a.lu_precision == "fp64" else "f64 (LU-SGS sweeps f32)",
            "data": "synthetic", "config": {"workload": desc, "cells": ncell, "zones": len(ids),
                                            "zone_batch": a.zone_batch, "graph": a.graph},
            "event_ms_per_step": round(ms_ev / a.steps, 3),

@@ -30,7 +30,10 @@ struct RhsArgs {
     double ck_vent, c_ale;  // shape_tab_mtr.for mesh-motion factors of the wall flux
 };
 enum { RES_CELLS = 0, RES_FACES = 1, RES_KCOL = 2 };
-constexpr int RES_KCOL_N = 4;
+#ifndef FM_RES_KCOL_N
+#define FM_RES_KCOL_N 4
+#endif
+constexpr int RES_KCOL_N = FM_RES_KCOL_N;
 
 struct LuArgs {
     int i0, i1, j0, j1, k0, k1;   // LU range (shift_lu)
