@@ -664,6 +664,7 @@ extern "C" int fm_dpj_dpw(fm_ctx* c, int zone, int level, int idir, const int* w
     const KZ& k = z->k;
     if (k.kflu != 1 || k.slope != 2 || k.iflow != 1)
         return fail(FM_E_UNSUPPORTED, "fm_dpj_dpw: AUSM+(P) o3 Euler zones only (the reference's option 213)");
+    if (k.nf < 2 || k.nfk < 2) return fail(FM_E_UNSUPPORTED, "fm_dpj_dpw: the face stencil needs 2 ghost layers");
     int d = (idir - 1) / 2;
     int lo[3] = {1, 1, 1}, hi[3] = {k.ni, k.nj, k.nk};
     hi[d] = hi[d] + 1;
@@ -735,6 +736,7 @@ extern "C" int fm_residual_vjp(fm_ctx* c, int zone, int level, const double* lam
     KZ& k = z->k;
     if (k.kflu != 1 || k.slope != 2 || k.iflow != 1)
         return fail(FM_E_UNSUPPORTED, "fm_residual_vjp: AUSM+(P) o3 Euler zones only");
+    if (k.nf < 2 || k.nfk < 2) return fail(FM_E_UNSUPPORTED, "fm_residual_vjp: the face stencil needs 2 ghost layers");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     if (!z->adj_buf) {
@@ -1140,6 +1142,8 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
+    if (phase == FM_PHASE_VJP && (k.nf < 2 || k.nfk < 2))
+        return fail(FM_E_UNSUPPORTED, "transposed product: the face stencil needs 2 ghost layers");
     if (phase == FM_PHASE_VJP && !z->adj_buf) {
         if (int rc = dmalloc(&z->adj_buf, vjp_buffer_doubles(k))) return rc;
     }
