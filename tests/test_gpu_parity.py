@@ -372,3 +372,18 @@ def test_c0_tgv64_roe_minmod_rk3(ctxmod, orc):
         return z
     zo, zg, _, _ = run_pair(ctxmod, orc, make, 3, nsteps=2)
     assert_same(zo, zg, fields=("ro", "ro_m1", "ro_p1"), ghosts=True)
+
+
+def test_tgv256_one_iteration_matches_oracle(ctxmod, orc):
+    """The benchmark's own size and case (TGV 256^3, laminar AUSM+(P) o3, implicit BDF, global
+    dt at CFL 2, one Newton sub-iteration): one full iteration on the GPU equals the oracle's bit
+    for bit (about 2 s of oracle time, 9 GB of host memory for the two copies)."""
+    num = P.Numerics(scheme="ausmpred", slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="global", time_step=1.0)
+
+    def make():
+        z = cases.tgv_zone(256, num)
+        z.param_real[P.DTC] = cases.tgv_dt(z, cfl=2.0)
+        return z
+    zo, zg, _, _ = run_pair(ctxmod, orc, make, 1)
+    assert_same(zo, zg)
