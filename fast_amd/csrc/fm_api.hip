@@ -53,6 +53,8 @@ struct Zone {
     int* lu_sync;            // tile counter and error word of the pipelined LU sweeps
     double* lu_xch;          // tile-boundary exchange of the pipelined LU sweeps (on first use)
     double* adj_buf;         // face contributions of the transposed product (on first use)
+    double* lu_fx;           // dataflow LU schedule: face exchange and block order (on first use)
+    LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
     long n_metric;
@@ -119,6 +121,8 @@ static void free_zone(Zone* z) {
     if (z->lu_sync) (void)hipFree(z->lu_sync);
     if (z->lu_xch) (void)hipFree(z->lu_xch);
     if (z->adj_buf) (void)hipFree(z->adj_buf);
+    if (z->lu_fx) (void)hipFree(z->lu_fx);
+    if (z->flow_work) (void)hipFree(z->flow_work);
     delete z;
 }
 
@@ -190,6 +194,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
+    z->lu_fx = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = nullptr; z->have_norms = 0; z->have_cfl = 0;
     KZ& k = z->k;
     k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
@@ -445,6 +450,33 @@ static int ensure_exchange(fm_ctx* c, Zone* z) {
     return FM_OK;
 }
 
+// Dataflow LU schedule: the face exchange (sized for the largest LU box, interior plus a wall
+// layer per side) and the block order of the LU box b (rebuilt when the box changes).
+static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
+    if (c->lu_schedule != LU_DATAFLOW) return FM_OK;
+    const KZ& k = z->k;
+    if (!z->lu_fx) {
+        long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMalloc(&z->lu_fx, sizeof(double) * n));
+    }
+    const Box& f = z->flow_box;
+    if (!z->flow_work || f.i0 != b.i0 || f.i1 != b.i1 || f.j0 != b.j0 || f.j1 != b.j1 || f.k0 != b.k0 || f.k1 != b.k1) {
+        std::vector<LuWork> work;
+        std::vector<int> offs;
+        lu_work_list(k.typ == 2, std::vector<Box>{b}, work, offs);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (z->flow_work) { HIPCHK(hipFree(z->flow_work)); z->flow_work = nullptr; }
+        HIPCHK(hipMalloc(&z->flow_work, sizeof(LuWork) * (work.empty() ? 1 : work.size())));
+        if (!work.empty())
+            HIPCHK(hipMemcpy(z->flow_work, work.data(), sizeof(LuWork) * work.size(), hipMemcpyHostToDevice));
+        z->flow_n = (int)work.size();
+        z->flow_box = b;
+    }
+    la.fx = z->lu_fx; la.flow_work = z->flow_work; la.flow_n = z->flow_n;
+    return FM_OK;
+}
+
 // Physical boundary conditions then edges and corners of one zone (gsdr3.cpp:436-439, bcs.cpp).
 static void fill_ghosts(fm_ctx* c, Zone* z, double* rop) {
     int shift[6], sdm[6] = {1, z->k.ni, 1, z->k.nj, 1, z->k.nk};
@@ -569,6 +601,11 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
             if (int rc = ensure_exchange(c, z)) return rc;
             la.xch = z->lu_xch;
+            la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0;
+            {
+                Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+                if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
+            }
             la.rop = ro_ssiter;
             if (ro_ssiter == ro_p1) {
                 // in place (sub-iterations >= 2): the K terms of the upper sweep see the state
@@ -601,7 +638,7 @@ static int check_lu(fm_ctx* c) {
         la.schedule = LU_PIPELINE; la.sync = z->lu_sync;
         int f = lu_pipeline_failed(la, c->stream);
         if (f < 0) return fail(FM_E_HIP, "reading the LU error word failed");
-        if (f > 0) return fail(FM_E_HIP, "pipelined LU-SGS sweep timed out waiting for a neighbour tile");
+        if (f > 0) return fail(FM_E_HIP, "LU-SGS sweep timed out waiting for a neighbour tile or block");
     }
     return FM_OK;
 }
@@ -1051,8 +1088,9 @@ extern "C" int fm_cfl(fm_ctx* c, int zone, double* out) {
 extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
     if (!c) return fail(FM_E_ARG, "null context");
     if (option == FM_OPT_LU_SCHEDULE) {
-        if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE) return fail(FM_E_ARG, "LU schedule %d", value);
-        c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : LU_BLOCKS;
+        if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE && value != FM_LU_DATAFLOW)
+            return fail(FM_E_ARG, "LU schedule %d", value);
+        c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : value == FM_LU_DATAFLOW ? LU_DATAFLOW : LU_BLOCKS;
         c->graph_epoch++;
         return FM_OK;
     }
@@ -1141,6 +1179,11 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
+    la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0;
+    {
+        Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+        if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
+    }
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
     if (phase == FM_PHASE_VJP && (k.nf < 2 || k.nfk < 2))
         return fail(FM_E_UNSUPPORTED, "transposed product: the face stencil needs 2 ghost layers");

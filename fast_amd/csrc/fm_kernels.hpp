@@ -35,6 +35,7 @@ enum { RES_CELLS = 0, RES_FACES = 1, RES_KCOL = 2 };
 #endif
 constexpr int RES_KCOL_N = FM_RES_KCOL_N;
 
+struct LuWork;
 struct LuArgs {
     int i0, i1, j0, j1, k0, k1;   // LU range (shift_lu)
     double* dw;                   // drodm in, dW out
@@ -47,11 +48,18 @@ struct LuArgs {
     double* dws;
     int* sync;
     double* xch;                  // pipelined schedule: tile-boundary exchange buffer
-    int schedule;                 // LU_BLOCKS or LU_PIPELINE
+    int schedule;                 // LU_BLOCKS, LU_PIPELINE or LU_DATAFLOW
     int fp32;                     // blocks schedule: sweep arithmetic in FP32 (state, right-hand
                                   // side, results and Newton update stay FP64)
+    // dataflow schedule: the blocks in diagonal order, and the products each block hands to
+    // the blocks after it (face exchange, pre-filled with the pending pattern)
+    const LuWork* flow_work;
+    int flow_n;
+    double* fx;
 };
-enum { LU_BLOCKS = 0, LU_PIPELINE = 1 };
+enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2 };
+// doubles of the dataflow face exchange of an LU box of ni x nj x nk cells
+long lu_flow_exchange_size(bool cart, int ni, int nj, int nk);
 
 struct BcArgs {
     int w[6];     // ghost window (1-based cell indices)

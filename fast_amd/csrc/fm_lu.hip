@@ -168,9 +168,54 @@ struct LuLds {
 //  2. the skewed wavefront from LDS: lane (jj, kk) owns line (j, k) and handles cell
 //     ii = s - jj - kk at step s; wave 0 solves, waves 1-3 form the products; results go
 //     straight to global memory.
-template <bool CART, bool LOWER, int CI, typename T = double>
+// Values handed between concurrently running blocks or tiles validate themselves: the buffer
+// is pre-filled with a signalling-NaN pattern that arithmetic never produces, producers store
+// with agent-scope stores (seen by every XCD) and consumers re-read with agent-scope loads until
+// the pattern is gone.  Every wait is bounded: past the bound, or once any wave has given up,
+// the error word is set and the kernel runs to its end, so the grid always drains.
+constexpr long long kLuPending = 0x7FF4C0DEC0DE0001ll;
+constexpr int kLuSpinMax = 1 << 22;
+
+__device__ __forceinline__ double lu_ldc(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lu_stc(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool lu_any_pending(const double* v, int n) {
+    bool p = false;
+#pragma unroll
+    for (int e = 0; e < 5; e++)
+        if (e < n) p = p || __double_as_longlong(v[e]) == kLuPending;
+    return p;
+}
+// re-read n values (stride nd) until none is the pending pattern; bounded
+__device__ __forceinline__ void lu_settle(double* v, int n, const double* base, long nd, int* err) {
+    int spins = 0;
+    while (lu_any_pending(v, n)) {
+        if (++spins > kLuSpinMax || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int e = 0; e < 5; e++)
+            if (e < n) v[e] = lu_ldc(base + e * nd);
+    }
+}
+
+// Dataflow face exchange of a block (sweep-order block coordinates bi, bj, bk): value e of the
+// products it hands to block bi + 1 (lane (jj, kk)), bj + 1 ((ii, kk)) and bk + 1 ((ii, jj)).
+template <int CI> struct LuFace {
+    static constexpr int NI = 64, NJ = 8 * CI, NK = 8 * CI, N = NI + NJ + NK;
+    __device__ static long at(long blk, int e, int face, int f) {
+        return (blk * 5 + e) * N + (face == 0 ? 0 : face == 1 ? NI : NI + NJ) + f;
+    }
+};
+
+template <bool CART, bool LOWER, int CI, typename T = double, bool FLOW = false>
 __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, int bj, int bk,
-                                         LuLds<CART, CI, T>& S) {
+                                         LuLds<CART, CI, T>& S, int nbi = 0, int nbj = 0, int nbk = 0) {
     constexpr int NV = LuLds<CART, CI, T>::NV;
     constexpr bool F64 = sizeof(T) == sizeof(double);
     const int tid = threadIdx.x;
@@ -217,9 +262,43 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             S.st[21][ii][L] = (T)tx; S.st[22][ii][L] = (T)ty; S.st[23][ii][L] = (T)tz;
         }
     }
-    // 1b. products of the halo cells (finished by earlier launches)
+    // 1b. products of the halo cells (finished by earlier launches; dataflow: handed over by the
+    //     neighbour blocks through the face exchange, read until they are no longer pending)
     for (int e = tid; e < 64 + 16 * CI; e += 256) {
         T b[5];
+        if (FLOW) {
+            double v[5];
+            long base;
+            if (e < 64) {
+                const int L = e, oj = oj0 + (L & 7), ok = ok0 + (L >> 3);
+                if (oi0 == 0 || oj >= nj || ok >= nk) continue;
+                base = LuFace<CI>::at(bi - 1 + (long)nbi * (bj + (long)nbj * bk), 0, 0, L);
+            } else if (e < 64 + 8 * CI) {
+                const int q = e - 64, ii = q / 8, kk = q % 8, ok = ok0 + kk;
+                if (oj0 == 0 || ii >= ci || ok >= nk) continue;
+                base = LuFace<CI>::at(bi + (long)nbi * (bj - 1 + (long)nbj * bk), 0, 1, ii * 8 + kk);
+            } else {
+                const int q = e - 64 - 8 * CI, ii = q / 8, jj = q % 8, oj = oj0 + jj;
+                if (ok0 == 0 || ii >= ci || oj >= nj) continue;
+                base = LuFace<CI>::at(bi + (long)nbi * (bj + (long)nbj * (bk - 1)), 0, 2, ii * 8 + jj);
+            }
+#pragma unroll
+            for (int v2 = 0; v2 < 5; v2++) v[v2] = lu_ldc(a.fx + base + (long)v2 * LuFace<CI>::N);
+            lu_settle(v, 5, a.fx + base, LuFace<CI>::N, a.sync + 1);
+            if (e < 64) {
+#pragma unroll
+                for (int v2 = 0; v2 < 5; v2++) S.hI[v2][e] = (T)v[v2];
+            } else if (e < 64 + 8 * CI) {
+                const int q = e - 64;
+#pragma unroll
+                for (int v2 = 0; v2 < 5; v2++) S.hJ[v2][q / 8][q % 8] = (T)v[v2];
+            } else {
+                const int q = e - 64 - 8 * CI;
+#pragma unroll
+                for (int v2 = 0; v2 < 5; v2++) S.hK[v2][q / 8][q % 8] = (T)v[v2];
+            }
+            continue;
+        }
         if (e < 64) {
             const int L = e, oj = oj0 + (L & 7), ok = ok0 + (L >> 3);
             if (oi0 == 0 || oj >= nj || ok >= nk) continue;
@@ -347,6 +426,18 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             }
 #pragma unroll
             for (int e = 0; e < 5; e++) S.px[wv - 1][e][L] = b[e];
+            if (FLOW) {
+                // products whose consumer is in the next block: hand them over
+                long blk = -1;
+                int face = wv - 1, f = 0;
+                if (wv == 1 && ii == ci - 1 && bi + 1 < nbi) { blk = bi + (long)nbi * (bj + (long)nbj * bk); f = L; }
+                if (wv == 2 && jj == 7 && bj + 1 < nbj) { blk = bi + (long)nbi * (bj + (long)nbj * bk); f = ii * 8 + kk; }
+                if (wv == 3 && kk == 7 && bk + 1 < nbk) { blk = bi + (long)nbi * (bj + (long)nbj * bk); f = ii * 8 + jj; }
+                if (blk >= 0) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(a.fx + LuFace<CI>::at(blk, e, face, f), (double)b[e]);
+                }
+            }
         }
         __syncthreads();
     }
@@ -359,6 +450,27 @@ __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int 
     if (bi < 0 || bi >= nbi || bj >= nbj) return;
     __shared__ LuLds<CART, CI, T> S;
     lu_block<CART, LOWER, CI, T>(z, a, bi, bj, bk, S);
+}
+
+// Dataflow schedule: one launch per sweep.  Workgroups take blocks from a counter in diagonal
+// order (a block only waits on blocks handed out before it, so there is no deadlock whatever the
+// dispatch order), stage their own fields at once, read the products of their halo cells from
+// the face exchange as the neighbour blocks hand them over, and hand their own face products on.
+// Launch diagonals are no longer barriers: a block starts as soon as its three neighbours are done.
+template <bool CART, bool LOWER, int CI>
+__global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nbj, int nbk) {
+    __shared__ LuLds<CART, CI> S;
+    __shared__ int tk;
+    for (;;) {
+        if (threadIdx.x == 0) tk = atomicAdd(a.sync, 1);
+        __syncthreads();
+        const int t = tk;
+        __syncthreads();
+        if (t >= a.flow_n) break;
+        const LuWork w = a.flow_work[t];
+        lu_block<CART, LOWER, CI, double, true>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
+        __syncthreads();
+    }
 }
 
 // Several zones in one launch per block diagonal: block b of the launch takes its zone and its
@@ -391,15 +503,7 @@ __global__ __launch_bounds__(256) void k_lu_block_list(const LuZone* __restrict_
 // with whatever it read, so the grid always drains; the host then reports the failure.
 // Tiles are taken from a counter in dependency order (by j-tile + k-tile), so a tile only
 // waits on tiles that started before it: no deadlock whatever the dispatch order.
-constexpr long long kLuPending = 0x7FF4C0DEC0DE0001ll;
-constexpr int kLuSpinMax = 1 << 22;
 
-__device__ __forceinline__ double lu_ldc(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void lu_stc(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // ticket -> tile, enumerating the tiles by diagonal jt + kt
 __device__ __forceinline__ void lu_tile_of(int t, int ntj, int ntk, int& jt, int& kt) {
     for (int d = 0; d < ntj + ntk - 1; d++) {
@@ -423,28 +527,7 @@ struct LuSlot {
     double mj[CART ? 1 : 6], mk[CART ? 1 : 6];
 };
 
-__device__ __forceinline__ bool lu_any_pending(const double* v, int n) {
-    bool p = false;
-#pragma unroll
-    for (int e = 0; e < 5; e++)
-        if (e < n) p = p || __double_as_longlong(v[e]) == kLuPending;
-    return p;
-}
 
-// re-read n values (stride nd) until none is the pending pattern; bounded
-__device__ __forceinline__ void lu_settle(double* v, int n, const double* base, long nd, int* err) {
-    int spins = 0;
-    while (lu_any_pending(v, n)) {
-        if (++spins > kLuSpinMax || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
-        }
-        __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-        for (int e = 0; e < 5; e++)
-            if (e < n) v[e] = lu_ldc(base + e * nd);
-    }
-}
 
 // Exchange layout: for the j boundary of tile row tj, value e of cell (oi, ok) (sweep offsets)
 // at xj[e * NXJ + (tj * nk + ok) * ni + oi]; for the k boundary, value e of (oi, oj) at
@@ -684,6 +767,19 @@ void sweep(const KZ& z, const LuArgs& a, hipStream_t st) {
 
 namespace {
 
+template <bool CART>
+void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
+    constexpr int CI = lu_ci<CART>();
+    int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    int nbi = (ni + CI - 1) / CI, nbj = (nj + 7) / 8, nbk = (nk + 7) / 8;
+    long nx = lu_flow_exchange_size(CART, ni, nj, nk);
+    unsigned grid = (unsigned)(a.flow_n < 2048 ? a.flow_n : 2048);
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
+    hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
+    hipLaunchKernelGGL((k_lu_flow<CART, false, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+}
+
 void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
@@ -755,8 +851,18 @@ void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const
     }
 }
 
+long lu_flow_exchange_size(bool cart, int ni, int nj, int nk) {
+    const int CI = cart ? lu_ci<true>() : lu_ci<false>();
+    long nb = (long)((ni + CI - 1) / CI) * ((nj + 7) / 8) * ((nk + 7) / 8);
+    return nb * 5 * (64 + 16 * CI);
+}
+
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
+    if (a.schedule == LU_DATAFLOW) {
+        if (z.typ == 2) flow<true>(z, a, st); else flow<false>(z, a, st);
+        return;
+    }
     if (a.schedule == LU_PIPELINE) {
         if (z.typ == 2) pipeline<true>(z, a, st); else pipeline<false>(z, a, st);
         return;
@@ -767,7 +873,7 @@ void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
 
 // error word of the pipelined sweeps (a bounded wait gave up); clears it
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st) {
-    if (a.schedule != LU_PIPELINE || !a.sync) return 0;
+    if ((a.schedule != LU_PIPELINE && a.schedule != LU_DATAFLOW) || !a.sync) return 0;
     int h = 0;
     if (hipMemcpyAsync(&h, a.sync + 1, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;

@@ -168,6 +168,10 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_OPT_LU_SCHEDULE 1
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
+/* FM_LU_DATAFLOW: one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
+   each starts as soon as its three neighbour blocks have handed over their face products
+   (bounded waits; a wait that gives up makes the next call report FM_E_HIP). */
+#define FM_LU_DATAFLOW 2
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
    FM_RES_KCOL (default) walks a column of 4 cells up k per thread, evaluating the k faces once. */

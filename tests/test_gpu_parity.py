@@ -100,13 +100,14 @@ def test_tgv_cart_modes_two_steps(ctxmod, orc, scheme, mode):
     assert_same(zo, zg, ghosts=True)
 
 
-@pytest.mark.parametrize("schedule", ["blocks", "pipeline"])
+@pytest.mark.parametrize("schedule", ["blocks", "pipeline", "dataflow"])
 @pytest.mark.parametrize("mesh", ["cart", "full", "wall"])
 @pytest.mark.parametrize("mode", ["bdf", "bdf_ss2"])
 def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
-    """Both LU-SGS schedules (one launch per block diagonal; persistent trailing tiles) on
-    zones of several tiles in j and k, in-place sub-iterations included."""
-    sch = ctxmod.LU_BLOCKS if schedule == "blocks" else ctxmod.LU_PIPELINE
+    """The LU-SGS schedules (one launch per block diagonal; persistent trailing tiles; blocks
+    handed out in diagonal order with a face exchange) on zones of several blocks in i, j and k,
+    in-place sub-iterations included."""
+    sch = {"blocks": ctxmod.LU_BLOCKS, "pipeline": ctxmod.LU_PIPELINE, "dataflow": ctxmod.LU_DATAFLOW}[schedule]
     nsub = 2 if mode == "bdf_ss2" else 1
     if mesh == "cart":
         mk = lambda: cases.tgv_zone((21, 19, 26), H.numerics("roe_kap", mode), perturb=0.02)
