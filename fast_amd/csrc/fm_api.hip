@@ -99,7 +99,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     HIPCHK(hipSetDevice(device));
     fm_ctx* c = new fm_ctx();
     c->device = device;
-    c->lu_schedule = LU_BLOCKS;
+    c->lu_schedule = LU_DATAFLOW;
     c->res_schedule = RES_KCOL;
     c->lu_batch = 1;
     c->lu_fp32 = 0;
@@ -943,7 +943,10 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
 static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx::GraphEntry* ge, bool capture) {
     const size_t nz = c->zones.size();
     // several zones: their LU-SGS sweeps share one launch per block diagonal
-    const bool batch = nz > 1 && c->lu_batch && c->lu_schedule == LU_BLOCKS;
+    // several zones: the batched blocks schedule (one launch per diagonal for all zones) also
+    // stands in for the dataflow schedule, whose per-zone launches and exchange fills cost more
+    // than they save on many small zones
+    const bool batch = nz > 1 && c->lu_batch && (c->lu_schedule == LU_BLOCKS || c->lu_schedule == LU_DATAFLOW);
     std::vector<int> lexit(nz);
     std::vector<double*> cl(nz);
     for (int nstep = n0; nstep <= n1; nstep++) {

@@ -49,7 +49,7 @@ struct LuArgs {
     int* sync;
     double* xch;                  // pipelined schedule: tile-boundary exchange buffer
     int schedule;                 // LU_BLOCKS, LU_PIPELINE or LU_DATAFLOW
-    int fp32;                     // blocks schedule: sweep arithmetic in FP32 (state, right-hand
+    int fp32;                     // blocks / dataflow: sweep arithmetic in FP32 (state, right-hand
                                   // side, results and Newton update stay FP64)
     // dataflow schedule: the blocks in diagonal order, and the products each block hands to
     // the blocks after it (face exchange, pre-filled with the pending pattern)

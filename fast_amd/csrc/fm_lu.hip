@@ -457,9 +457,9 @@ __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int 
 // dispatch order), stage their own fields at once, read the products of their halo cells from
 // the face exchange as the neighbour blocks hand them over, and hand their own face products on.
 // Launch diagonals are no longer barriers: a block starts as soon as its three neighbours are done.
-template <bool CART, bool LOWER, int CI>
+template <bool CART, bool LOWER, int CI, typename T = double>
 __global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nbj, int nbk) {
-    __shared__ LuLds<CART, CI> S;
+    __shared__ LuLds<CART, CI, T> S;
     __shared__ int tk;
     for (;;) {
         if (threadIdx.x == 0) tk = atomicAdd(a.sync, 1);
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nb
         __syncthreads();
         if (t >= a.flow_n) break;
         const LuWork w = a.flow_work[t];
-        lu_block<CART, LOWER, CI, double, true>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
+        lu_block<CART, LOWER, CI, T, true>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
         __syncthreads();
     }
 }
@@ -775,9 +775,11 @@ void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
     long nx = lu_flow_exchange_size(CART, ni, nj, nk);
     unsigned grid = (unsigned)(a.flow_n < 2048 ? a.flow_n : 2048);
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
-    hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, true, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    else hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
-    hipLaunchKernelGGL((k_lu_flow<CART, false, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, false, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    else hipLaunchKernelGGL((k_lu_flow<CART, false, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
 }
 
 void fill_exchange(const LuArgs& a, long n, hipStream_t st) {

@@ -11,7 +11,7 @@ KNOBS = (
     ("residual", _lib.OPT_RESIDUAL_SCHEDULE, _lib.PHASE_RESIDUAL,
      (("cells", _lib.RES_CELLS), ("faces", _lib.RES_FACES), ("kcol", _lib.RES_KCOL))),
     ("lu", _lib.OPT_LU_SCHEDULE, _lib.PHASE_LU,
-     (("blocks", _lib.LU_BLOCKS), ("pipeline", _lib.LU_PIPELINE))),
+     (("dataflow", _lib.LU_DATAFLOW), ("blocks", _lib.LU_BLOCKS), ("pipeline", _lib.LU_PIPELINE))),
 )
 
 

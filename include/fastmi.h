@@ -162,15 +162,16 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    of fm_residual_vjp, both passes and three directions).  No counterpart in the reference. */
 /* Context options.  FM_OPT_LU_SCHEDULE selects how the LU-SGS sweeps are scheduled on the
    GPU (both give results bit-identical to the reference's lexicographic sweeps):
-   FM_LU_BLOCKS (default) one launch per diagonal of 8x8x8 blocks;
+   FM_LU_BLOCKS one launch per diagonal of 8x8x8 blocks;
    FM_LU_PIPELINE persistent 8x8-line tiles that trail their neighbours.  No counterpart in
    the reference. */
 #define FM_OPT_LU_SCHEDULE 1
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
-/* FM_LU_DATAFLOW: one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
+/* FM_LU_DATAFLOW (default): one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
    each starts as soon as its three neighbour blocks have handed over their face products
-   (bounded waits; a wait that gives up makes the next call report FM_E_HIP). */
+   (bounded waits; a wait that gives up makes the next call report FM_E_HIP).  With several
+   zones and FM_OPT_ZONE_BATCH on, the sweeps run as the batched blocks schedule. */
 #define FM_LU_DATAFLOW 2
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
@@ -191,7 +192,7 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 /* FM_OPT_LU_PRECISION: FM_LU_FP64 (default) runs the LU-SGS sweeps in FP64, bit-identical to the
    reference; FM_LU_FP32 runs the sweep arithmetic (products, solve, LDS image) in FP32 while the
    state, the right-hand side (Newton residual), the increments written back and the Newton
-   update stay FP64 (mixed precision; blocks schedule only).  LU-SGS is the approximate factor
+   update stay FP64 (mixed precision; blocks and dataflow schedules).  LU-SGS is the approximate factor
    of a defect correction, so the converged Newton residual is still FP64; the increments
    differ from the FP64 sweeps at the FP32 rounding level. */
 #define FM_OPT_LU_PRECISION 5

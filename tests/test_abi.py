@@ -49,7 +49,8 @@ def test_autotune_picks_the_fastest_schedule():
                          (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_FACES): 1.5,
                          (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_KCOL): 1.7,
                          (_lib.OPT_LU_SCHEDULE, _lib.LU_BLOCKS): 3.0,
-                         (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0}
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_DATAFLOW): 3.5}
 
         def set_option(self, opt, v):
             self.opts[opt] = v
