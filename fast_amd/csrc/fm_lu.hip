@@ -6,7 +6,9 @@
 // same expressions in the same order; only the schedule differs:
 //
 //  * the LU box is cut into blocks of CI x 8 x 8 cells (i x j x k).  Blocks with the same
-//    bi + bj + bk are independent: one launch per block diagonal, in sweep order;
+//    bi + bj + bk are independent: either one launch per block diagonal, in sweep order, or (the
+//    dataflow schedule) one launch per sweep whose workgroups take the blocks in diagonal order
+//    and receive their halo products from the neighbour blocks through a face exchange;
 //  * a block is first staged into LDS with coalesced loads (the latency of global memory is
 //    paid once per block, not once per cell), together with the products of its halo cells,
 //    which earlier launches finished;
