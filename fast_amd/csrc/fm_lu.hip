@@ -32,6 +32,10 @@
 #define FM_LU_CI_FULL 8
 #endif
 template <bool CART> constexpr int lu_ci() { return CART ? FM_LU_CI_CART : FM_LU_CI_FULL; }
+// Workgroups of a dataflow sweep (each takes blocks until none is left).
+#ifndef FM_LU_FLOW_GRID
+#define FM_LU_FLOW_GRID 2048
+#endif
 #include <array>
 #include <vector>
 
@@ -775,7 +779,16 @@ void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int nbi = (ni + CI - 1) / CI, nbj = (nj + 7) / 8, nbk = (nk + 7) / 8;
     long nx = lu_flow_exchange_size(CART, ni, nj, nk);
-    unsigned grid = (unsigned)(a.flow_n < 2048 ? a.flow_n : 2048);
+    // as many workgroups as can be resident at once (measured: 512 on a Cartesian zone beat 1024
+    // and 2048, whose extra workgroups only queue behind the resident ones)
+    int cus = 0, dev = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    const void* kfn = a.fp32 ? (const void*)k_lu_flow<CART, true, CI, float> : (const void*)k_lu_flow<CART, true, CI>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, 256, 0) != hipSuccess || per < 1) per = 1;
+    long cap = (long)per * cus;
+    if (cap > FM_LU_FLOW_GRID) cap = FM_LU_FLOW_GRID;
+    unsigned grid = (unsigned)(a.flow_n < cap ? a.flow_n : cap);
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
     if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, true, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     else hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
