@@ -18,7 +18,7 @@ enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 
        ITYPCP = 29, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
        NEQ_VENT = 40,
        NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
-       EXPLOC = 54, NSSITER = 58, SLOPE = 64, PT_BC = 70, NB_RELAX = 71, IMPLICITSOLVER = 74,
+       EXPLOC = 54, NSSITER = 58, CACHEBLCKI = 59, CACHEBLCKJ = 60, CACHEBLCKK = 61, SLOPE = 64, PT_BC = 70, NB_RELAX = 71, IMPLICITSOLVER = 74,
        LU_MATCH = 75, SENSORTYPE = 85, BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2, BC_NBDATA = 8 };
 enum { R_DTC = 0, R_GAMMA = 1, R_CVINF = 2, R_ROINF = 3, R_PINF = 4, R_VINF = 5, R_TINF = 6,
        R_VXINF = 19, R_VYINF = 20, R_VZINF = 21,
@@ -188,6 +188,13 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     if (!((kf == 5 && sl >= 1 && sl <= 3) || ((kf == 1 || kf == 2) && sl == 2)))
         return fail(FM_E_UNSUPPORTED, "scheme %d / slope %d not implemented", kf, sl);
     if (p[NIJK + 3] != 2 || p[NIJK + 4] != 2) return fail(FM_E_UNSUPPORTED, "two ghost layers required");
+    // cache blocking (CACHEBLCKI/J/K): any j and k block sizes give the reference's per-cell sum
+    // order; one block in i (the reference default 2048 covers any i extent below 4096)
+    if (p[CACHEBLCKI] < 1 || p[CACHEBLCKJ] < 1 || p[CACHEBLCKK] < 1)
+        return fail(FM_E_ARG, "cache block sizes must be positive");
+    if (p[IJKV] / p[CACHEBLCKI] > 1)
+        return fail(FM_E_UNSUPPORTED, "more than one cache block in i (ni %d, CACHEBLCKI %d)", p[IJKV],
+                    p[CACHEBLCKI]);
     Zone* z = new Zone();
     memset(&z->k, 0, sizeof(KZ));
     z->p.assign(p, p + n_int);
@@ -205,6 +212,12 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     k.mi = p[NIJK_MTR]; k.mj = p[NIJK_MTR + 1]; k.mk = p[NIJK_MTR + 2];
     k.mf = p[NIJK_MTR + 3]; k.mfk = p[NIJK_MTR + 4]; k.ndm = p[NDIMDX_MTR];
     k.xi = p[NIJK_XYZ]; k.xj = p[NIJK_XYZ + 1]; k.xf = p[NIJK_XYZ + 3]; k.xfk = p[NIJK_XYZ + 4];
+    // cache blocks of the flux pass for one thread owning the zone (WORK_DISTRIBUTION_BEGIN.for:
+    // the CACHEBLCK* slots as they are; crsdm_scater.for:42-44: max(1, n / size) blocks per
+    // direction, the last one taking the remainder)
+    k.cbj = p[CACHEBLCKJ]; k.cbk = p[CACHEBLCKK];
+    k.nbj = k.nj / k.cbj > 1 ? k.nj / k.cbj : 1;
+    k.nbk = k.nk / k.cbk > 1 ? k.nk / k.cbk : 1;
     fill_kz_params(z);
     // boundary-condition table (FastC._BCcompact layout)
     int pt_bcs = p[PT_BC];

@@ -22,6 +22,10 @@ struct KZ {
     long ndm;              // NDIMDX_MTR
     int xi, xj, xf, xfk;   // vertex grid (NIJK_XYZ)
     int kflu, slope, iflow, itypcp, dtloc, rk, nssiter, sensortype;
+    // cache blocks of the reference's flux and gradient passes in j and k (CACHEBLCKJ/K and
+    // the block counts max(1, n / size) of crsdm_scater.for; one block in i): they fix the
+    // order in which a cell receives the fluxes of its max faces
+    int cbj, cbk, nbj, nbk;
     double dtc, gam, cv, roinf, pinf, vinf, tinf, prandt, xmul0, temp0, cs, cfl, psiroe, wig_amp;
     // fields
     const double* x; const double* y; const double* z;

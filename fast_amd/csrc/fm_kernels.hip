@@ -332,6 +332,14 @@ __device__ void wall_correction(const FluxConst& f, const KZ& z, const double* _
 // Residual: fused init_rhs + 6 face fluxes + balance + update, one thread per interior cell.
 // Accumulation order per cell as the reference sweep (template_FluxAndBalance.for:192-271):
 //   init, +Fk(k), +Fj(j), +Fi(i), -Fi(i+1), -Fj(j+1), -Fk(k+1).
+// The reference runs that sweep per cache block (navier_stokes_struct.for LOOP_CACHE: k blocks
+// outermost, then j; one block in i) and the face between two blocks belongs to the later
+// block, which also follows the Roe wall corrections of the earlier one (correct_flux runs per
+// block).  So for a cell on the last row of a j block that is not the last (jn) or on the last
+// plane of such a k block (kn), the max faces come in the order
+//   [-Fj(j+1) unless jn] [-Fk(k+1) unless kn] [wall corrections] [-Fj(j+1) if jn] [-Fk(k+1) if kn]
+// which is the single-block order when neither holds.
+__device__ __forceinline__ bool cb_next(int c, int size, int nb) { return c % size == 0 && c / size < nb; }
 // Update: core3as2.for (implicit, writes the Newton residual into drodm) or core3ark3.for
 // (explicit, keeps drodm and writes the new stage state).
 // FM_RES_WAVES: waves per SIMD the register allocation of k_residual aims at (2: no spills)
@@ -367,13 +375,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES
     face_flux<KFLU, SLOPE, LAMIN, CART, 0>(f, z, rop, i + 1, j, k, flu);
 #pragma unroll
     for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
-    face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j + 1, k, flu);
+    const bool jn = cb_next(j, z.cbj, z.nbj), kn = cb_next(k, z.cbk, z.nbk);
+    double fj1[5];
+    face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j + 1, k, fj1);
+    if (!jn) {
 #pragma unroll
-    for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj1[e];
+    }
     face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k + 1, flu);
+    if (!kn) {
 #pragma unroll
-    for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+    }
     if (KFLU == 5 && a.ncorr > 0) wall_correction<SLOPE, CART>(f, z, rop, a, i, j, k, acc);
+    if (jn) {
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj1[e];
+    }
+    if (kn) {
+#pragma unroll
+        for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+    }
 
     res_update(z, a, rop, l, acc);
 }
@@ -414,13 +436,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES
         face_flux<KFLU, SLOPE, LAMIN, CART, 0>(f, z, rop, i + 1, j, k, flu);
 #pragma unroll
         for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        const bool jn = cb_next(j, z.cbj, z.nbj), kn = cb_next(k, z.cbk, z.nbk);
         face_flux<KFLU, SLOPE, LAMIN, CART, 1>(f, z, rop, i, j + 1, k, flu);
+        if (!jn) {
 #pragma unroll
-        for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        }
         face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k + 1, fk);
+        if (!kn) {
 #pragma unroll
-        for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk[e];
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk[e];
+        }
         if (KFLU == 5 && a.ncorr > 0) wall_correction<SLOPE, CART>(f, z, rop, a, i, j, k, acc);
+        if (jn) {
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - flu[e];
+        }
+        if (kn) {
+#pragma unroll
+            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk[e];
+        }
         res_update(z, a, rop, l, acc);
     }
 }
@@ -483,10 +518,23 @@ __global__ void __launch_bounds__(320) k_residual_faces(KZ z, RhsArgs a) {
             for (int e = 0; e < 5; e++) acc[e] = acc[e] + fi[e];
 #pragma unroll
             for (int e = 0; e < 5; e++) acc[e] = acc[e] - fi1[e];
+            const bool jn = cb_next(j, z.cbj, z.nbj), kn = cb_next(k, z.cbk, z.nbk);
+            if (!jn) {
 #pragma unroll
-            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj[par][ty + 1][e][tx];
+                for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj[par][ty + 1][e][tx];
+            }
+            if (!kn) {
 #pragma unroll
-            for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk1[e];
+                for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk1[e];
+            }
+            if (jn) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) acc[e] = acc[e] - fj[par][ty + 1][e][tx];
+            }
+            if (kn) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) acc[e] = acc[e] - fk1[e];
+            }
             res_update(z, a, rop, l, acc);
 #pragma unroll
             for (int e = 0; e < 5; e++) fk[e] = fk1[e];
@@ -891,7 +939,8 @@ __global__ void k_norm_final(const double* __restrict__ part, int nblk, double* 
 // component in the reference loop order, divided by the volume (cp_gradu.for), then
 // 1/2 rho |u|^2 vol, 1/2 rho |curl u|^2 vol and vol summed (cptaylor.for; the sums are reduced
 // in another order than the reference loop).  On 3dfull zones the i-face above the last cell
-// carries the metric index of the face below it, as in cp_gradu_3dfull.for.
+// carries the metric index of the face below it, as in cp_gradu_3dfull.for, and the max faces
+// follow the cache-block order of the residual.
 __device__ __forceinline__ double post_face(const double* __restrict__ q, long l, long inc) {
     const double c1 = 7. / 12, c2 = 1. / 12;
     return (q[l] + q[l - inc]) * c1 - (q[l + inc] + q[l - 2 * inc]) * c2;
@@ -913,6 +962,9 @@ __global__ void k_post_partial(KZ z, const double* __restrict__ ro, double* __re
         long lt = cart ? 0 : MTR(z, i, j, k);
         double vo = cart ? z.vol[0] : z.vol[lt];
         double volinv = 1. / vo;
+        // post.for runs per cache block: the max face from the next j block comes after the
+        // one from the same block in k (as in the residual)
+        const bool jn = cb_next(j, z.cbj, z.nbj), kn = cb_next(k, z.cbk, z.nbk);
         double g[3][3];
 #pragma unroll
         for (int v = 0; v < 3; v++) {
@@ -933,8 +985,13 @@ __global__ void k_post_partial(KZ z, const double* __restrict__ ro, double* __re
                     a = a - flo[1] * tm[1][lt + x * ndm];
                     a = a - flo[0] * tm[0][lt + x * ndm];
                     a = a + fhi[0] * tm[0][lhi + x * ndm];
-                    a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
-                    a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                    if (jn && !kn) {
+                        a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                        a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
+                    } else {
+                        a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
+                        a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                    }
                 }
                 g[v][x] = a * volinv;
             }

@@ -49,11 +49,14 @@ BC_TAGS = {"BCExtrapolate": 0, "BCDegenerateLine": 0, "BCFarfield": 1, "BCInflow
 
 
 class Numerics(dict):
-    """User numerics with the reference defaults (FastC/PyTree.py:1117-1331)."""
+    """User numerics with the reference defaults (FastC/PyTree.py:1117-1331), cache blocking
+    included (2048 / 2 / 7, FastC/PyTree.py:1299-1301): the block sizes decide in which order a
+    cell receives the fluxes of its faces, and the GPU residual follows that order for any j and
+    k block sizes (one block in i)."""
     DEFAULTS = dict(temporal_scheme="implicit", ss_iteration=30, rk=3, modulo_verif=200,
                     scheme="ausmpred", slope="o3", psiroe=0.1, epsi_newton=0.1, cfl=1.0,
-                    time_step=1.0e-4, time_step_nature="global", cache_blocking_I=1000000,
-                    cache_blocking_J=1000000, cache_blocking_K=1000000, sensor_type=1,
+                    time_step=1.0e-4, time_step_nature="global", cache_blocking_I=2048,
+                    cache_blocking_J=2, cache_blocking_K=7, sensor_type=1,
                     ausm_wiggle=1.0, io_thread=0, nit_inflow=10, epsi_inflow=1.0e-5,
                     prandtltb=None, implicit_solver="lussor", nb_relax=1, lu_match=0)
 

@@ -18,7 +18,7 @@ enum { NIJK = 0, NIJK_MTR = 5, NIJK_XYZ = 10, IJKV = 20, ITYPZONE = 25, IFLOW = 
        SIZE_SSDOM = 30, KFLUDOM = 33, LALE = 34, NEQ = 36, NEQ_IJ = 37, NEQ_K = 38, NEQ_COE = 39,
        NEQ_VENT = 40,
        NDIMDX = 41, NDIMDX_XYZ = 42, NDIMDX_MTR = 43, DTLOC = 46, RK = 52, LEVEL = 53,
-       NSSITER = 58, SLOPE = 64, PT_BC = 70, LU_MATCH = 75, SENSORTYPE = 85,
+       NSSITER = 58, CACHEBLCKI = 59, CACHEBLCKJ = 60, CACHEBLCKK = 61, SLOPE = 64, PT_BC = 70, LU_MATCH = 75, SENSORTYPE = 85,
        BC_TYPE = 0, BC_IDIR = 1, BC_FEN = 2, BC_NBDATA = 8 };
 enum { DTC = 0, GAMMA = 1, CVINF = 2, ROINF = 3, PINF = 4, VINF = 5, TINF = 6,
        RONUTILDEINF = 8, PRANDT = 10, XMUL0 = 11, TEMP0 = 12, CS = 13, CFL = 15, VXINF = 19,
@@ -775,6 +775,7 @@ static void face_flux(const FC* f, long i, long j, long k, int d, double flu[5])
 }
 
 static int cl_window(const int* p, int idir, const int* fen, int ind_cl[6]);
+static int cl_window_sdm(const int* p, int idir, const int* fen, const int sdm[6], int ind_cl[6]);
 static void shape_tab(int neq_mtr, int idir, int* ic, int* jc, int* kc, double* cim, double* cjm,
                       double* ckm);
 
@@ -786,8 +787,11 @@ static void shape_tab(int neq_mtr, int idir, int* ic, int* jc, int* kc, double* 
       averaged state, no mass or energy flux) or bflwallslip.for (slip wall: convective fluxes
       with the normal velocity of the averaged state), assemble_drodm_corr.for.
    Both act on the interior cell next to the face; the window is indice_cl_sdm.for with
-   npass = 0 and its normal range set to the face index (correct_flux.for:116-133).       */
-static void correct_flux_roe(const int* p, const double* pr, const FC* f, double* drodm)
+   npass = 0, the intersection of the BC window with the cache block sdm (navier_stokes_struct.for
+   calls correct_flux once per cache block, after that block's fluxes), and its normal range
+   set to the face index (correct_flux.for:116-133).                                       */
+static void correct_flux_roe(const int* p, const double* pr, const FC* f, const int* sdm,
+                             double* drodm)
 {
     const Ix* Z = f->Z;
     long nd = Z->nd, ndm = Z->ndm;
@@ -803,7 +807,7 @@ static void correct_flux_roe(const int* p, const double* pr, const FC* f, double
         int slip = t == 3, wall = t == 4 || t == 6 || t == 7 || t == 12;
         if (!slip && !wall) continue;
         int cl[6];
-        if (cl_window(p, idir, p + pt + BC_FEN, cl)) continue;
+        if (cl_window_sdm(p, idir, p + pt + BC_FEN, sdm, cl)) continue;
         switch (idir) {
         case 1: cl[0] = 1; cl[1] = 1; break;
         case 2: cl[1] = cl[0]; break;
@@ -912,13 +916,12 @@ static int fc_init(FC* fp, const Ix* Z, const int* p, const double* pr, const do
     return 0;
 }
 
-static int fo_flux(const int* p, const double* pr, const int* ind, const double* rop,
-                   double* drodm, const double* wig, const double* ti, const double* tj,
-                   const double* tk, const double* vol, const double* xmut)
+/* The flux pass of one cache block ind (template_FluxAndBalance.for:192-271): the faces of the
+   block's cells, then the max faces that no later block computes (icorr / jcorr at the last block
+   in i / j, the k complement at the last block in k). */
+static void flux_block(const FC* f, const int* ind, int icorr, int jcorr, int kcorr, double* drodm)
 {
-    Ix zz = mkix(p); Ix* Z = &zz;
-    FC f;
-    if (fc_init(&f, Z, p, pr, rop, wig, ti, tj, tk, vol, xmut)) return -1;
+    const Ix* Z = f->Z;
     long nd = Z->nd;
     double flu[5];
 #define PLUS(L) for (int e = 0; e < 5; e++) { drodm[(L) + e * nd] = drodm[(L) + e * nd] + flu[e]; }
@@ -927,40 +930,79 @@ static int fo_flux(const int* p, const double* pr, const int* ind, const double*
         for (long j = ind[2]; j <= ind[3]; j++) {
             for (long i = ind[0]; i <= ind[1]; i++) {
                 long l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 2, flu); PLUS(l); MINUS(l - Z->inck);
+                face_flux(f, i, j, k, 2, flu); PLUS(l); MINUS(l - Z->inck);
             }
             for (long i = ind[0]; i <= ind[1]; i++) {
                 long l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 1, flu); PLUS(l); MINUS(l - Z->incj);
+                face_flux(f, i, j, k, 1, flu); PLUS(l); MINUS(l - Z->incj);
             }
             for (long i = ind[0]; i <= ind[1]; i++) {
                 long l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 0, flu); PLUS(l); MINUS(l - 1);
+                face_flux(f, i, j, k, 0, flu); PLUS(l); MINUS(l - 1);
             }
-            {   /* icorr (single cache block: always the last block) */
+            if (icorr) {
                 long i = ind[1] + 1, l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 0, flu); MINUS(l - 1);
+                face_flux(f, i, j, k, 0, flu); MINUS(l - 1);
             }
         }
-        {
+        if (jcorr) {
             long j = ind[3] + 1;
             for (long i = ind[0]; i <= ind[1]; i++) {
                 long l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 1, flu); MINUS(l - Z->incj);
+                face_flux(f, i, j, k, 1, flu); MINUS(l - Z->incj);
             }
         }
     }
-    {
+    if (kcorr) {
         long k = ind[5] + 1;
         for (long j = ind[2]; j <= ind[3]; j++)
             for (long i = ind[0]; i <= ind[1]; i++) {
                 long l = IDX(Z, i, j, k);
-                face_flux(&f, i, j, k, 2, flu); MINUS(l - Z->inck);
+                face_flux(f, i, j, k, 2, flu); MINUS(l - Z->inck);
             }
     }
 #undef PLUS
 #undef MINUS
-    if (f.kflu == 5) correct_flux_roe(p, pr, &f, drodm);
+}
+
+/* Number of cache blocks of a direction of n cells (crsdm_scater.for:42-44, one thread: the
+   target size is the CACHEBLCK* slot unchanged, WORK_DISTRIBUTION_BEGIN.for:18-28). */
+static int cache_nblk(int n, int size)
+{
+    int b = n / size;
+    return b < 1 ? 1 : b;
+}
+
+/* The RHS flux pass of navier_stokes_struct.for:157-498 for one thread owning the box ind: the
+   cache blocks (LOOP_CACHE_BEGIN.for: kcache, jcache, icache from the outermost) each get their
+   fluxes and then, for Roe, their wall corrections.  Block c of a direction spans
+   [lo + (c-1) size, lo + c size - 1], the last one up to the end of the box
+   (indice_boucle_ssdom.for:40-50).  With one block per direction the per-cell accumulation order
+   is +Fk(k) +Fj(j) +Fi(i) -Fi(i+1) -Fj(j+1) -Fk(k+1); a face between two blocks is computed by
+   the later block, so block edges change the order in which a cell receives its max faces. */
+static int fo_flux(const int* p, const double* pr, const int* ind, const double* rop,
+                   double* drodm, const double* wig, const double* ti, const double* tj,
+                   const double* tk, const double* vol, const double* xmut)
+{
+    Ix zz = mkix(p); Ix* Z = &zz;
+    FC f;
+    if (fc_init(&f, Z, p, pr, rop, wig, ti, tj, tk, vol, xmut)) return -1;
+    int size[3] = {p[CACHEBLCKI], p[CACHEBLCKJ], p[CACHEBLCKK]}, nb[3];
+    for (int d = 0; d < 3; d++) {
+        if (size[d] < 1) return -1;
+        nb[d] = cache_nblk(ind[2 * d + 1] - ind[2 * d] + 1, size[d]);
+    }
+    for (int kc = 1; kc <= nb[2]; kc++)
+        for (int jc = 1; jc <= nb[1]; jc++)
+            for (int ic = 1; ic <= nb[0]; ic++) {
+                int c[3] = {ic, jc, kc}, sdm[6];
+                for (int d = 0; d < 3; d++) {
+                    sdm[2 * d] = ind[2 * d] + (c[d] - 1) * size[d];
+                    sdm[2 * d + 1] = c[d] == nb[d] ? ind[2 * d + 1] : ind[2 * d] + c[d] * size[d] - 1;
+                }
+                flux_block(&f, sdm, ic == nb[0], jc == nb[1], kc == nb[2], drodm);
+                if (f.kflu == 5) correct_flux_roe(p, pr, &f, sdm, drodm);
+            }
     return 0;
 }
 
@@ -1677,8 +1719,15 @@ typedef struct { long lo[3], hi[3]; } Box;
 /* indice_cl_sdm.for for a thread box equal to the zone and npass = 0 */
 static int cl_window(const int* p, int idir, const int* fen, int ind_cl[6])
 {
-    int ific = p[NIJK + 3], kfic = p[NIJK + 4];
     int sdm[6] = {1, p[IJKV], 1, p[IJKV + 1], 1, p[IJKV + 2]};
+    return cl_window_sdm(p, idir, fen, sdm, ind_cl);
+}
+
+/* indice_cl_sdm.for with npass = 0: the BC window intersected with the box sdm, widened to
+   the ghost layers in the normal direction; 1 if they do not intersect */
+static int cl_window_sdm(const int* p, int idir, const int* fen, const int sdm[6], int ind_cl[6])
+{
+    int ific = p[NIJK + 3], kfic = p[NIJK + 4];
     if (!(sdm[0] <= fen[1] && sdm[1] >= fen[0] && sdm[2] <= fen[3] && sdm[3] >= fen[2] &&
           sdm[4] <= fen[5] && sdm[5] >= fen[4])) return 1;
     for (int a = 0; a < 3; a++) {
@@ -2141,7 +2190,9 @@ int fo_apply_bc(int* p, double* pr, int nstep, double* rop, double* ti, double* 
 /* ------------------------------------------------------------------------------------ */
 /* One sub-iteration (gsdr3.cpp:152-444; navier_stokes_struct.for:216-496; lhs.cpp)     */
 /* cpcfl3.for, no mesh motion (LALE = 0): local CFL number dt / min(viscous, convective time)
-   of every interior cell; cfl[0] = max, cfl[1] = min, cfl[2] += sum, in the loop order k, j, i.
+   of every interior cell; cfl[0] = max, cfl[1] = min, cfl[2] += sum.  cptst3.for:114-119 calls
+   it once per cache block (over ind_sdm), so the sum runs block by block (kcache, jcache,
+   icache from the outermost) and in the loop order k, j, i inside a block.
    3dcart uses the single face areas (ddi = 2|ti|), 3dfull the sum of the two face areas.     */
 static void fo_cfl(const int* p, const double* pr, const double* rop, const double* xmut,
                    const double* ti, const double* tj, const double* tk, const double* vol,
@@ -2159,9 +2210,19 @@ static void fo_cfl(const int* p, const double* pr, const double* rop, const doub
         double ddi = 2. * sqrt(ti[0] * ti[0]), ddj = 2. * sqrt(tj[0] * tj[0]), ddk = 2. * sqrt(tk[0] * tk[0]);
         surf_c = 1. / fmax2(fmax2(ddi, ddj), ddk);
     }
-    for (long k = 1; k <= p[IJKV + 2]; k++)
-        for (long j = 1; j <= p[IJKV + 1]; j++)
-            for (long i = 1; i <= p[IJKV]; i++) {
+    int size[3] = {p[CACHEBLCKI], p[CACHEBLCKJ], p[CACHEBLCKK]}, nb[3];
+    for (int d = 0; d < 3; d++) nb[d] = cache_nblk(p[IJKV + d], size[d]);
+    for (int kc = 1; kc <= nb[2]; kc++)
+    for (int jc = 1; jc <= nb[1]; jc++)
+    for (int ic = 1; ic <= nb[0]; ic++) {
+    int c[3] = {ic, jc, kc}, sdm[6];
+    for (int d = 0; d < 3; d++) {
+        sdm[2 * d] = 1 + (c[d] - 1) * size[d];
+        sdm[2 * d + 1] = c[d] == nb[d] ? p[IJKV + d] : c[d] * size[d];
+    }
+    for (long k = sdm[4]; k <= sdm[5]; k++)
+        for (long j = sdm[2]; j <= sdm[3]; j++)
+            for (long i = sdm[0]; i <= sdm[1]; i++) {
                 long l = IDX(Z, i, j, k);
                 double surf, vo;
                 if (cart) { surf = surf_c; vo = vol[0]; }
@@ -2194,6 +2255,7 @@ static void fo_cfl(const int* p, const double* pr, const double* rop, const doub
                 cfl[1] = fmin2(cfl[1], cfloc);
                 cfl[2] = cfl[2] + cfloc;
             }
+    }
 }
 
 /* Kinetic energy and enstrophy sums (POST/post.for -> cp_gradu_3d{cart,full}.for with the
@@ -2221,11 +2283,27 @@ void fo_enstrophy(const int* p, const double* pr, const double* rop, const doubl
     const double* tm[3] = {ti, tj, tk};
     long minc[3] = {Z->mi, Z->mj, Z->mk};
     double tke = 0., enst = 0., cpt = 0.;
-    for (long k = 1; k <= p[IJKV + 2]; k++)
-        for (long j = 1; j <= p[IJKV + 1]; j++)
-            for (long i = 1; i <= p[IJKV]; i++) {
+    /* post.for:70-95 runs cp_gradu and cptaylor once per cache block (kcache, jcache, icache
+       from the outermost; blocks as in fo_flux).  A face between two blocks belongs to the later
+       block: a cell receives first the max faces its own block computes (i, j, k), then those of
+       the next block in i, in j, in k (in the order the blocks run); the sums run block by
+       block. */
+    int size[3] = {p[CACHEBLCKI], p[CACHEBLCKJ], p[CACHEBLCKK]}, nb[3];
+    for (int d = 0; d < 3; d++) nb[d] = cache_nblk(p[IJKV + d], size[d]);
+    for (int kc = 1; kc <= nb[2]; kc++)
+    for (int jc = 1; jc <= nb[1]; jc++)
+    for (int ic = 1; ic <= nb[0]; ic++) {
+    int cb[3] = {ic, jc, kc}, sdm[6];
+    for (int d = 0; d < 3; d++) {
+        sdm[2 * d] = 1 + (cb[d] - 1) * size[d];
+        sdm[2 * d + 1] = cb[d] == nb[d] ? p[IJKV + d] : cb[d] * size[d];
+    }
+    for (long k = sdm[4]; k <= sdm[5]; k++)
+        for (long j = sdm[2]; j <= sdm[3]; j++)
+            for (long i = sdm[0]; i <= sdm[1]; i++) {
                 long l = IDX(Z, i, j, k);
                 long lt = cart ? 0 : MTR(Z, i, j, k);
+                int nx[3] = {i == sdm[1] && ic < nb[0], j == sdm[3] && jc < nb[1], k == sdm[5] && kc < nb[2]};
                 double g[3][3];      /* g[velocity][x,y,z] */
                 for (int v = 0; v < 3; v++) {
                     const double* q = rop + (1 + v) * nd;
@@ -2247,10 +2325,10 @@ void fo_enstrophy(const int* p, const double* pr, const double* rop, const doubl
                             /* cp_gradu_3dfull.for:129-135: the i-face above the last
                                cell is added with the metric index of the previous loop
                                (the face below that cell), as the reference does */
-                            long lhi = (i == p[IJKV]) ? lt : lt + minc[0];
-                            a = a + fhi[0] * tm[0][lhi + x * ndm];
-                            a = a + fhi[1] * tm[1][lt + minc[1] + x * ndm];
-                            a = a + fhi[2] * tm[2][lt + minc[2] + x * ndm];
+                            long lhi[3] = {(i == p[IJKV]) ? lt : lt + minc[0], lt + minc[1], lt + minc[2]};
+                            for (int pass = 0; pass < 2; pass++)
+                                for (int d = 0; d < 3; d++)
+                                    if (nx[d] == pass) a = a + fhi[d] * tm[d][lhi[d] + x * ndm];
                             g[v][x] = a;
                         }
                     }
@@ -2269,6 +2347,7 @@ void fo_enstrophy(const int* p, const double* pr, const double* rop, const doubl
                 double w1 = (g[2][1] - g[1][2]), w2 = (g[0][2] - g[2][0]), w3 = (g[1][0] - g[0][1]);
                 enst = enst + 0.5 * rho * (w1 * w1 + w2 * w2 + w3 * w3) * vo;
             }
+    }
     out[0] = tke; out[1] = enst; out[2] = cpt;
 }
 

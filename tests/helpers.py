@@ -35,6 +35,11 @@ EXTRAP_I = [("BCExtrapolate", "imin", None), ("BCExtrapolate", "imax", None),
             ("BCautoperiod", "kmin", None), ("BCautoperiod", "kmax", None)]
 
 
+# One cache block per zone: a box cut into zones equals the single zone bit for bit only when no
+# cache-block edge changes the order in which a cell receives its face fluxes.
+SINGLE_BLOCK = dict(cache_blocking_I=1000000, cache_blocking_J=1000000, cache_blocking_K=1000000)
+
+
 def numerics(scheme, mode, **kw):
     if mode == "explicit":
         return P.Numerics(scheme=scheme, temporal_scheme="explicit", time_step=0.01, **kw)

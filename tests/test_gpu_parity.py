@@ -388,3 +388,55 @@ def test_tgv256_one_iteration_matches_oracle(ctxmod, orc):
         return z
     zo, zg, _, _ = run_pair(ctxmod, orc, make, 1)
     assert_same(zo, zg)
+
+
+CB = {"default": (2048, 2, 7), "j3k2": (2048, 3, 2), "j5k1": (2048, 5, 1), "single": (1000000,) * 3}
+
+
+@pytest.mark.parametrize("cb", sorted(CB))
+@pytest.mark.parametrize("schedule", ["cells", "faces", "kcol"])
+@pytest.mark.parametrize("case", ["cart", "channel", "inviscid"])
+@pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
+def test_cache_blocking(ctxmod, orc, cb, schedule, case, scheme):
+    """The reference's flux pass runs per cache block, which fixes the order in which a cell
+    receives its max faces and its Roe wall corrections.  Every residual schedule follows it for
+    any j and k block sizes: two implicit time steps equal the oracle bit for bit, ghost cells
+    included."""
+    bi, bj, bk = CB[cb]
+    num = H.numerics(scheme, "bdf", cache_blocking_I=bi, cache_blocking_J=bj, cache_blocking_K=bk)
+    if case == "cart":
+        mk = lambda: cases.tgv_zone((19, 10, 16), num, perturb=0.02)
+    elif case == "channel":
+        mk = lambda: cases.channel_zone(18, 12, 15, num)
+    else:
+        mk = lambda: H.jittered_box((9, 10, 9), num, bcs=H.INVISCID_J, flow="euler")
+    zo, zg = mk(), mk()
+    orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
+    zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
+    ctx = ctxmod.Context(0)
+    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
+    zid = ctx.add_zone(zg)
+    ctx.apply_bc(zid, 0)
+    t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+    ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+    for step in range(2):
+        fi = 0 if step == 0 else 1
+        orc.substep(zo, 1, first_it=fi)
+        H.rotate(zo)
+        ctx.compute(step, 1, 1, nit_c=1, modulo_verif=0, first_it=fi)
+    ctx.sync()
+    ctx.pull(zid, zg)
+    ctx.close()
+    assert_same(zo, zg, fields=("ro", "ro_m1", "ro_p1"), ghosts=True)
+
+
+def test_cache_blocking_limits(ctxmod):
+    """More than one cache block in i is refused (FM_E_UNSUPPORTED), as is a block size below 1."""
+    z = cases.tgv_zone((20, 6, 5), H.numerics("ausmpred", "bdf", cache_blocking_I=8))
+    ctx = ctxmod.Context(0)
+    with pytest.raises(ctxmod.FastMIError, match="cache block in i"):
+        ctx.add_zone(z)
+    z = cases.tgv_zone((20, 6, 5), H.numerics("ausmpred", "bdf", cache_blocking_J=0))
+    with pytest.raises(ctxmod.FastMIError, match="positive"):
+        ctx.add_zone(z)
+    ctx.close()

@@ -71,7 +71,7 @@ def test_match_indices():
 
 @pytest.mark.parametrize("scheme", ["roe_min", "roe_kap", "ausmpred", "senseur"])
 def test_split_box_equals_single_zone(orc, scheme):
-    num = H.numerics(scheme, "explicit")
+    num = H.numerics(scheme, "explicit", **H.SINGLE_BLOCK)
     zones, pairs, W = cases.tiled_box(NX, NY, NZ, num, flow="euler")
     racs = connect.build(zones, pairs)
     init(orc, zones, racs)

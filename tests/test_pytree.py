@@ -160,7 +160,7 @@ def test_warmup_compute_tiled_tree_matches_single_zone():
     equal the single periodic zone stepped by the oracle, bit for bit."""
     import oracle
     orc = oracle.Oracle()
-    num = H.numerics("ausmpred", "explicit")
+    num = H.numerics("ausmpred", "explicit", **H.SINGLE_BLOCK)
     t, zones, pairs = tiled_tree(num)
     t, tc, metrics = FP.warmup(t, None)
     assert len(FP.HOOK["raccords"]) == len(pairs)
