@@ -45,6 +45,8 @@ SIGNATURES = {
     "fm_transfer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fm_wall_effort": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                       _dp, _dp, _dp]),
+    "fm_wall_effort_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              _ip, _dp, _dp, _dp]),
     "fm_dpj_dpw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
     "fm_flux_balance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
@@ -184,6 +186,19 @@ class Context:
         check(self.lib.fm_wall_effort(self.h, zone, level, idir, w.ctypes.data_as(_ip),
                                       p.ctypes.data_as(_dp), eff.ctypes.data_as(_dp),
                                       fl.ctypes.data_as(_dp) if fields else None))
+        return (eff, fl.reshape(21, nf)) if fields else eff
+
+    def wall_effort_noalloc(self, zone, idir, window, pos=(0., 0., 0.), level=0, fields=True):
+        """wall_effort through fm_wall_effort_noalloc (no device allocation; at most 65536
+        faces per window)."""
+        w = np.ascontiguousarray(window, dtype=np.int32)
+        nf = int((w[1] - w[0] + 1) * (w[3] - w[2] + 1) * (w[5] - w[4] + 1))
+        eff = np.zeros(12)
+        fl = np.zeros(21 * nf) if fields else None
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        check(self.lib.fm_wall_effort_noalloc(self.h, zone, level, idir, w.ctypes.data_as(_ip),
+                                              p.ctypes.data_as(_dp), eff.ctypes.data_as(_dp),
+                                              fl.ctypes.data_as(_dp) if fields else None))
         return (eff, fl.reshape(21, nf)) if fields else eff
 
     def dpj_dpw(self, zone, idir, window, cosa, sina, surfinv, dcd, dcl, level=0):

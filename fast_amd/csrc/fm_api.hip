@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdarg>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "fastmi.h"
@@ -94,6 +95,29 @@ struct fm_ctx {
 
 extern "C" const char* fm_last_error(void) { return g_err.c_str(); }
 
+// Device scratch of fm_wall_effort_noalloc: the per-face fields and contributions of one face
+// window (33 x faces doubles) and their 12 sums, in a file-scope device array so that the entry
+// point allocates nothing.  Each device holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for that device
+// (g_scratch_ptr), resolved once per device by fm_ctx_create.
+constexpr long kScratchFaces = 65536;
+constexpr long kScratchDoubles = 33 * kScratchFaces + 16;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+static int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(FM_E_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return FM_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(FM_E_HIP, "scratch address: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)ptr;
+    return FM_OK;
+}
+
 extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     if (!out) return fail(FM_E_ARG, "null output");
     HIPCHK(hipSetDevice(device));
@@ -103,6 +127,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     c->res_schedule = RES_KCOL;
     c->lu_batch = 1;
     c->lu_fp32 = 0;
+    if (int rc = resolve_scratch(device)) { delete c; return rc; }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
     HIPCHK(hipEventCreate(&c->e0));
@@ -703,6 +728,47 @@ extern "C" int fm_wall_effort(fm_ctx* c, int zone, int level, int idir, const in
     for (long t = 0; t < nf; t++)
         for (int q = 0; q < 12; q++) effort[q] = effort[q] + h[(size_t)(21 + q) * nf + t];
     if (fields) memcpy(fields, h.data(), sizeof(double) * 21 * nf);
+    return FM_OK;
+}
+
+// fm_wall_effort without any allocation: the kernels write into the device scratch array and a
+// one-thread kernel forms the 12 sums in the same face order as fm_wall_effort's host loop, so
+// both give the same bits.  Calls that overlap on one device share the array and race.
+extern "C" int fm_wall_effort_noalloc(fm_ctx* c, int zone, int level, int idir, const int* window,
+                                      const double* pos, double* effort, double* fields) {
+    Zone* z = get_zone(c, zone);
+    if (!z || level < 0 || level > 2 || idir < 1 || idir > 6 || !window || !effort)
+        return fail(FM_E_ARG, "fm_wall_effort_noalloc: bad arguments");
+    const KZ& k = z->k;
+    int d = (idir - 1) / 2;
+    int lo[3] = {1, 1, 1}, hi[3] = {k.ni, k.nj, k.nk};
+    hi[d] = hi[d] + 1;
+    for (int a = 0; a < 3; a++)
+        if (window[2 * a] < lo[a] || window[2 * a + 1] > hi[a] || window[2 * a] > window[2 * a + 1])
+            return fail(FM_E_ARG, "fm_wall_effort_noalloc: window outside the zone");
+    if (window[2 * d] != window[2 * d + 1])
+        return fail(FM_E_ARG, "fm_wall_effort_noalloc: window is not one face plane");
+    long nf = (long)(window[1] - window[0] + 1) * (window[3] - window[2] + 1) * (window[5] - window[4] + 1);
+    if (nf > kScratchFaces)
+        return fail(FM_E_ARG, "fm_wall_effort_noalloc: %ld faces, at most %ld", nf, kScratchFaces);
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+        return fail(FM_E_HIP, "fm_wall_effort_noalloc: no scratch on device %d", c->device);
+    HIPCHK(hipSetDevice(c->device));
+    double* buf = g_scratch_ptr[c->device];
+    double* sums = buf + 33 * nf;
+    EffArgs a;
+    a.idir = idir;
+    for (int q = 0; q < 6; q++) a.w[q] = window[q];
+    for (int q = 0; q < 3; q++) a.pos[q] = pos ? pos[q] : 0.;
+    a.rop = z->ro[level]; a.out = buf;
+    if (launch_effort(k, a, c->stream)) return fail(FM_E_UNSUPPORTED, "no effort kernel for this scheme");
+    launch_effort_sum(buf, nf, sums, c->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(effort, sums, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && fields)
+        e = hipMemcpyAsync(fields, buf, sizeof(double) * 21 * nf, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(FM_E_HIP, "fm_wall_effort_noalloc: %s", hipGetErrorString(e));
     return FM_OK;
 }
 

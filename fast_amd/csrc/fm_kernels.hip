@@ -1263,6 +1263,22 @@ static void effort_m(const KZ& z, const EffArgs& a, dim3 g, hipStream_t st) {
     else { if (z.typ == 2) effort_d<KFLU, SLOPE, false, true>(z, a, g, st); else effort_d<KFLU, SLOPE, false, false>(z, a, g, st); }
 }
 
+// The 12 sums of a window's effort contributions (rows 21..32 of out), face by face in the
+// window order, as fm_wall_effort adds them on the host: thread q adds row q sequentially, so
+// the order is fixed.
+__global__ void k_effort_sum(const double* __restrict__ out, long nf, double* __restrict__ sums) {
+    const int q = threadIdx.x;
+    if (q >= 12 || blockIdx.x != 0) return;
+    const double* __restrict__ row = out + (21 + q) * nf;
+    double s = 0.;
+    for (long t = 0; t < nf; t++) s = s + row[t];
+    sums[q] = s;
+}
+
+void launch_effort_sum(const double* out, long nf, double* sums, hipStream_t st) {
+    hipLaunchKernelGGL(k_effort_sum, dim3(1), dim3(64), 0, st, out, nf, sums);
+}
+
 int launch_effort(const KZ& z, const EffArgs& a, hipStream_t st) {
     long nf = (long)(a.w[1] - a.w[0] + 1) * (a.w[3] - a.w[2] + 1) * (a.w[5] - a.w[4] + 1);
     if (nf <= 0) return 0;

@@ -89,6 +89,8 @@ void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a,
 // sums; out = 33 x nfaces (fields 0-20, then the 12 contributions)
 struct EffArgs { int idir, w[6]; double pos[3]; const double* rop; double* out; };
 int launch_effort(const KZ& z, const EffArgs& a, hipStream_t st);
+// sums[0..11] = the 12 contributions of out (33 x nf) summed face by face
+void launch_effort_sum(const double* out, long nf, double* sums, hipStream_t st);
 // gradients of the pressure drag / lift coefficients at a slip-wall window (fm_adjoint.hip)
 struct DpjArgs { int idir, w[6]; double cosa, sina, surfinv; const double* rop; double* dcd; double* dcl; };
 int launch_dpj_dpw(const KZ& z, const DpjArgs& a, hipStream_t st);

@@ -91,6 +91,15 @@ int fm_apply_bc(fm_ctx* ctx, int zone, int level, int nstep);
 int fm_wall_effort(fm_ctx* ctx, int zone, int level, int idir, const int* window,
                    const double* pos, double* effort, double* fields);
 
+/* fm_wall_effort for callers that cannot allocate device memory (code under HIP graph capture,
+   serving or optimisation loops that evaluate the wall force every iteration, harnesses that
+   forbid hipMalloc): no counterpart in the reference.  Same arguments and the same results, bit
+   for bit; the per-face work goes through a device array of the library sized for at most
+   65536 faces per window (larger windows return FM_E_ARG), which every call on a device shares:
+   calls that overlap on one device race.  Synchronous. */
+int fm_wall_effort_noalloc(fm_ctx* ctx, int zone, int level, int idir, const int* window,
+                           const double* pos, double* effort, double* fields);
+
 /* Gradients of the pressure drag and lift coefficients of one slip-wall face window with
    respect to the conservative state (rho, rho u, rho v, rho w, rho E) of the wall cells
    (replaces fasts.compute_dpJ_dpW, ADJOINT/compute_dpJ_dpW.cpp:33-261 -> dpj_dpw_calc_meth.for
