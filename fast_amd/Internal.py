@@ -7,6 +7,31 @@ same names and behaviour.
 """
 import numpy as np
 
+# The solver API (fast_amd.PyTree) keeps the flow fields on the GPU between calls and refreshes
+# the tree's arrays when code outside the API next reads the tree through these accessors: it
+# installs _access_hook, and wraps its own tree walks in quiet() so that they do not trigger it.
+_access_hook = None
+_quiet_depth = 0
+
+
+def _touch():
+    if _access_hook is not None and _quiet_depth == 0:
+        _access_hook()
+
+
+class quiet:
+    """Context manager: accessor calls inside it do not trigger the access hook."""
+
+    def __enter__(self):
+        global _quiet_depth
+        _quiet_depth += 1
+        return self
+
+    def __exit__(self, *exc):
+        global _quiet_depth
+        _quiet_depth -= 1
+        return False
+
 
 def createNode(name, ntype, value=None, children=None, parent=None):
     if isinstance(value, (int, np.integer)):
@@ -38,6 +63,7 @@ def createUniqueChild(parent, name, ntype, value=None, children=None):
 
 
 def getValue(node):
+    _touch()
     v = node[1]
     if isinstance(v, np.ndarray) and v.dtype.kind in ("S", "c"):
         return v.tobytes().decode()
@@ -51,6 +77,7 @@ def setValue(node, value):
 
 
 def getNodeFromName1(node, name):
+    _touch()
     for c in node[2]:
         if c[0] == name:
             return c
@@ -58,6 +85,7 @@ def getNodeFromName1(node, name):
 
 
 def getNodeFromName2(node, name):
+    _touch()
     for c in node[2]:
         if c[0] == name:
             return c
@@ -68,20 +96,27 @@ def getNodeFromName2(node, name):
 
 
 def getNodeFromName(node, name):
+    _touch()
+    return _node_from_name(node, name)
+
+
+def _node_from_name(node, name):
     if node[0] == name:
         return node
     for c in node[2]:
-        r = getNodeFromName(c, name)
+        r = _node_from_name(c, name)
         if r is not None:
             return r
     return None
 
 
 def getNodesFromType1(node, ntype):
+    _touch()
     return [c for c in node[2] if c[3] == ntype]
 
 
 def getNodesFromType2(node, ntype):
+    _touch()
     out = []
     for c in node[2]:
         if c[3] == ntype:
@@ -93,13 +128,63 @@ def getNodesFromType2(node, ntype):
 
 
 def getNodesFromType(node, ntype):
+    _touch()
+    return _nodes_from_type(node, ntype)
+
+
+def _nodes_from_type(node, ntype):
     out = [node] if node[3] == ntype else []
     for c in node[2]:
-        out.extend(getNodesFromType(c, ntype))
+        out.extend(_nodes_from_type(c, ntype))
+    return out
+
+
+def getNodeFromType(node, ntype):
+    _touch()
+    r = _nodes_from_type(node, ntype)
+    return r[0] if r else None
+
+
+getNodeByName1 = getNodeFromName1
+
+
+def getNodeFromPath(t, path):
+    """The node at path "Base/Zone/..." below t (a leading "/" or the root's name is allowed)."""
+    _touch()
+    parts = [q for q in path.split("/") if q]
+    if parts and parts[0] == t[0]:
+        parts = parts[1:]
+    node = t
+    for q in parts:
+        node = next((c for c in node[2] if c[0] == q), None)
+        if node is None:
+            return None
+    return node
+
+
+def getZonePaths(t):
+    _touch()
+    return ["%s/%s" % (b[0], z[0]) for b in getNodesFromType1(t, "CGNSBase_t")
+            for z in getNodesFromType1(b, "Zone_t")]
+
+
+def getPathsFromType(t, ntype):
+    """Paths (from below the root) of the nodes of type ntype."""
+    _touch()
+    out = []
+
+    def walk(node, prefix):
+        for c in node[2]:
+            p = c[0] if not prefix else prefix + "/" + c[0]
+            if c[3] == ntype:
+                out.append(p)
+            walk(c, p)
+    walk(t, "")
     return out
 
 
 def getNodeFromType1(node, ntype):
+    _touch()
     for c in node[2]:
         if c[3] == ntype:
             return c
@@ -107,10 +192,12 @@ def getNodeFromType1(node, ntype):
 
 
 def getBases(t):
+    _touch()
     return getNodesFromType1(t, "CGNSBase_t")
 
 
 def getZones(t):
+    _touch()
     if t[3] == "Zone_t":
         return [t]
     if t[3] == "CGNSBase_t":

@@ -16,10 +16,19 @@ Drop-in for the FastS compute path of ``Fast/FastS/FastS/PyTree.py``:
 * numerics setters ``_setNum2Base`` / ``_setNum2Zones``  (FastC/PyTree.py:115-137)
 
 The tree holds the fields exactly as the reference's does (FlowSolution#Centers Density,
-VelocityX/Y/Z, Temperature and their _M1/_P1 time levels, ghost cells included); the device
-copy is authoritative during ``_compute`` and the tree arrays are refreshed in place after it.
-Like the reference's module-global ``FastC.HOOK``, one solver state lives per process.
+VelocityX/Y/Z, Temperature and their _M1/_P1 time levels, ghost cells included).  The state
+lives on the GPU between calls: ``_compute`` leaves the tree arrays as they were and marks them
+stale; the next time code outside this API reads the tree through ``fast_amd.Internal`` (or
+calls ``_syncTree``, ``saveTree``) the three levels are copied back into the tree arrays in
+place.  Once the tree has been handed out that way it is the source of truth again: the next
+``_compute`` (or BC fill, stress, enstrophy) first copies the tree's levels to the GPU, so edits
+made to the tree arrays take effect.  A loop of ``_compute`` calls that does not read the tree
+therefore moves no field over PCIe.  Arrays fetched before a ``_compute`` must be fetched again
+(or ``_syncTree`` called) to see the new state.  Like the reference's module-global
+``FastC.HOOK``, one solver state lives per process.
 """
+import functools
+import os
 import sys
 
 import numpy as np
@@ -30,6 +39,65 @@ from .zone import HostZone
 
 VARS = ["Density", "VelocityX", "VelocityY", "VelocityZ", "Temperature"]
 HOOK = {}
+
+
+def _api(f):
+    """Solver API functions walk the tree without triggering the tree refresh."""
+    @functools.wraps(f)
+    def wrapper(*args, **kw):
+        with Internal.quiet():
+            return f(*args, **kw)
+    return wrapper
+
+
+def _pull_levels():
+    """Copy the three time levels of every zone from the GPU into the tree arrays (in place)."""
+    from . import _lib
+    ctx = HOOK["ctx"]
+    with Internal.quiet():
+        for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+            for sfx, f, a in (("", _lib.FIELD_RO, hz.ro), ("_M1", _lib.FIELD_RO_M1, hz.ro_m1),
+                              ("_P1", _lib.FIELD_RO_P1, hz.ro_p1)):
+                ctx.get(zid, f, a)
+                _level_to_tree(z, hz, sfx, a)
+    HOOK["stale"] = False
+    HOOK["exposed"] = True
+
+
+def _on_tree_access():
+    if HOOK.get("stale"):
+        _pull_levels()
+
+
+def _device_current():
+    """Before GPU work: if the tree has been handed out since the last GPU update, its arrays
+    are the state (they may have been edited): copy them to the GPU."""
+    from . import _lib
+    if not HOOK.get("exposed"):
+        return
+    ctx = HOOK["ctx"]
+    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+        for sfx, f, a in (("", _lib.FIELD_RO, hz.ro), ("_M1", _lib.FIELD_RO_M1, hz.ro_m1),
+                          ("_P1", _lib.FIELD_RO_P1, hz.ro_p1)):
+            _tree_to_level(z, hz, sfx, a)
+            ctx.set(zid, f, a)
+    HOOK["exposed"] = False
+
+
+def _gpu_updated():
+    """After GPU work that changed the state: the tree arrays are stale until next read."""
+    HOOK["stale"] = True
+    HOOK["exposed"] = False
+    Internal._access_hook = _on_tree_access
+
+
+def _syncTree(t=None):
+    """Copy the current state of every zone (levels N, N-1, N+1) into the tree arrays now."""
+    if HOOK.get("stale"):
+        _pull_levels()
+
+
+syncTree = _syncTree
 _MODELS = {"NSLaminar": "lamin", "Euler": "euler"}
 _STATE_KEYS = ("gamma", "cv", "ro", "p", "u", "T", "mach", "reynolds", "prandtl", "mus", "ts",
                "cs", "vx", "vy", "vz")
@@ -43,11 +111,13 @@ def _set_define(node, num):
         Internal.createUniqueChild(d, k, "DataArray_t", v)
 
 
+@_api
 def _setNum2Base(t, numb):
     for b in Internal.getBases(t):
         _set_define(b, numb)
 
 
+@_api
 def _setNum2Zones(t, numz):
     for z in Internal.getZones(t):
         _set_define(z, numz)
@@ -231,6 +301,7 @@ def _level_to_tree(z, hz, suffix, a):
 
 
 # ---------------------------------------------------------------------------------------
+@_api
 def allocate_metric(t, verbose=0):
     """Classify every zone, build its descriptor vectors, register it on the GPU and return
     the list of metric arrays (ti|tj|tk|vol per zone)."""
@@ -257,6 +328,7 @@ def allocate_metric(t, verbose=0):
     return metrics
 
 
+@_api
 def _init_metric(t, metrics, hook=None):
     """Face vectors and volumes (computed on the GPU when the zone was registered)."""
     from . import _lib
@@ -265,6 +337,7 @@ def _init_metric(t, metrics, hook=None):
         ctx.get(zid, _lib.FIELD_METRIC, m[0])
 
 
+@_api
 def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list_graph=None,
            Padding=None, verbose=0):
     """Build the solver state: metrics, time levels _M1/_P1, ghost cells.  -> (t, tc, metrics)
@@ -324,15 +397,22 @@ def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list
                     Internal.createUniqueChild(fs, v, "DataArray_t", np.zeros(shape, order="F"))
     HOOK["first_it"] = first_it
     HOOK["nitrun_done"] = 0
+    # the tree now holds the state and is in the caller's hands
+    HOOK["stale"] = False
+    HOOK["exposed"] = True
+    Internal._access_hook = _on_tree_access
     return t, tc, metrics
 
 
+@_api
 def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, layer="c", NIT=1,
              ucData=None, vtune=None):
-    """NIT time steps (all sub-iterations) of every zone on the GPU; tree fields updated in place."""
+    """NIT time steps (all sub-iterations) of every zone on the GPU (computePT.cpp:44-721).  The
+    state stays on the GPU; the tree arrays are refreshed when the tree is next read."""
     import time
     from . import _lib
     ctx = HOOK["ctx"]
+    _device_current()
     base = Internal.getBases(t)[0]
     bdef = _get_define(base)
     modulo_verif = int(bdef.get("modulo_verif", 200))
@@ -340,20 +420,21 @@ def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, lay
     nssiter = int(hz0.param_int[P.NSSITER])
     t0 = time.time()
     ctx.compute(nitrun, 1, nssiter, nit_c=NIT, modulo_verif=modulo_verif, first_it=HOOK["first_it"])
-    ctx.sync()
+    if vtune is not None:
+        ctx.sync()
     t1 = time.time()
     HOOK["first_it"] = 1
-    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
-        for sfx, f, a in (("", _lib.FIELD_RO, hz.ro), ("_M1", _lib.FIELD_RO_M1, hz.ro_m1),
-                          ("_P1", _lib.FIELD_RO_P1, hz.ro_p1)):
-            ctx.get(zid, f, a)
-            _level_to_tree(z, hz, sfx, a)
-        # CFL max / min / mean of the last verification step (computePT.cpp:629-645)
-        try:
-            cfl = ctx.cfl(zid)
-        except _lib.FastMIError:
-            cfl = None
-        if cfl is not None:
+    HOOK["nitrun_done"] = nitrun + NIT
+    _gpu_updated()
+    # CFL max / min / mean of the last verification step (computePT.cpp:629-645), stored in
+    # .Solver#ownData/CFL_minmaxmoy when a verification step ran
+    verified = modulo_verif > 0 and any((nitrun * NIT + it) % modulo_verif == 0 for it in range(NIT))
+    if verified:
+        for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+            try:
+                cfl = ctx.cfl(zid)
+            except _lib.FastMIError:
+                continue
             o = Internal.createUniqueChild(z, ".Solver#ownData", "UserDefinedData_t")
             Internal.createUniqueChild(o, "CFL_minmaxmoy", "DataArray_t", cfl)
     if vtune is not None:
@@ -361,85 +442,381 @@ def _compute(t, metrics, nitrun, tc=None, graph=None, tc2=None, graph2=None, lay
     return None
 
 
+@_api
 def _fillGhostcells(zones, tc, metrics, timelevel_target, vars, nstep, omp_mode=0, hook1=None,
                     nitmax=1, rk=1, exploc=0, num_passage=1):
     """Transfers between zones then physical boundary conditions on the N level of every zone
     (Fast/FastS/FastS/PyTree.py:1162-1230 _fillGhostcells: setInterpTransfers then _applyBC)."""
-    from . import _lib
     ctx = HOOK["ctx"]
-    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
-        _tree_to_level(z, hz, "", hz.ro)
-        ctx.set(zid, _lib.FIELD_RO, hz.ro)
+    _device_current()
     ctx.transfer(0)
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
         ctx.apply_bc(zid, 0, nstep)
-        ctx.get(zid, _lib.FIELD_RO, hz.ro)
-        _level_to_tree(z, hz, "", hz.ro)
+    _gpu_updated()
 
 
+@_api
 def _applyBC(t, metrics, hook1=None, nstep=1, var="Density"):
-    from . import _lib
+    """Physical boundary conditions on the N level of every zone (BC/applyBC.cpp:33-255)."""
     ctx = HOOK["ctx"]
+    _device_current()
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
-        _tree_to_level(z, hz, "", hz.ro)
-        ctx.set(zid, _lib.FIELD_RO, hz.ro)
         ctx.apply_bc(zid, 0, nstep)
-        ctx.get(zid, _lib.FIELD_RO, hz.ro)
-        _level_to_tree(z, hz, "", hz.ro)
+    _gpu_updated()
 
 
+# ---------------------------------------------------------------------------------------
+# Convergence monitor (display_ss_iteration.cpp, dpssiter.for, conv2pytree.for)
+_FORMATS = {None: 1, "double": 0, "flush": -1, "ascii": -2, "store": 3}
+_CUT0X = 1.e-30
+
+
+def _log10c(v):
+    return np.log10(np.maximum(v, _CUT0X))
+
+
+def _fortran_e(v, w=15, d=8):
+    """Fortran Ew.d: 0.ddddddddE+xx, right-justified in w columns."""
+    if v == 0.0 or not np.isfinite(v):
+        m, e = (0.0, 0) if v == 0.0 else (v, 0)
+        body = ("%.*f" % (d, m)) if np.isfinite(v) else str(v)
+        return (body + "E+00").rjust(w)
+    e = int(np.floor(np.log10(abs(v)))) + 1
+    m = abs(v) / 10.0 ** e
+    if round(m, d) >= 1.0:
+        m /= 10.0
+        e += 1
+    body = "%s0.%0*dE%s%02d" % ("-" if v < 0 else "", d, int(round(m * 10 ** d)), "-" if e < 0 else "+", abs(e))
+    return body.rjust(w)
+
+
+def _dpssiter(name, nitrun, rdm, lft, iverb):
+    """dpssiter.for: print the log10 residuals of the recorded sub-iterations (first one always,
+    every one when verbose or when Linf grew) and the difference last - first; return the 4 neq
+    values conv2pytree records (log10 of the first L2 and Linf, log10 differences last - first)."""
+    neq = 5
+    nrec = rdm.shape[0]
+    l2, li = rdm[:, :neq], rdm[:, neq:]
+    iverbs = iverb
+    if np.any(li[nrec - 1] > li[0]):
+        iverbs = 2
+    if lft < 3:
+        it_reelle = list(range(1, nrec)) + [nrec]
+        fmt = (lambda v: "  %14.9f" % v) if lft == 0 else (lambda v: "  %7.2f" % v)
+        for i in range(nrec):
+            if i == 0 or iverbs > 1:
+                print(" Res_L2 (zone=%s, ssiter=%3d %3d)%s" % (name, it_reelle[i], i + 1, "".join(fmt(v) for v in _log10c(l2[i]))))
+                print(" Res_Loo(zone=%s, ssiter=%3d %3d)%s" % (name, it_reelle[i], i + 1, "".join(fmt(v) for v in _log10c(li[i]))))
+        d2 = _log10c(l2[nrec - 1]) - _log10c(l2[0])
+        di = _log10c(li[nrec - 1]) - _log10c(li[0])
+        print(" Res_L2 (zone=%s, Diff Ssi 1-%3d)%s" % (name, nrec, "".join(fmt(v) for v in d2)))
+        print(" Res_Loo(zone=%s, Diff Ssi 1-%3d)%s" % (name, nrec, "".join(fmt(v) for v in di)))
+        if d2.max() > 0:
+            print("Warning convergence L2  Newton: dom=%s Pas de temps=%6d divergence=%12.5f" % (name, nitrun, d2.max()))
+        if di.max() > 0:
+            print("Warning convergence Loo Newton: dom=%s Pas de temps=%6d divergence=%12.5f" % (name, nitrun, di.max()))
+    cvg = np.concatenate([_log10c(l2[0]), _log10c(li[0]), _log10c(l2[nrec - 1]) - _log10c(l2[0]),
+                          _log10c(li[nrec - 1]) - _log10c(li[0])])
+    return cvg
+
+
+def _history_nodes(h):
+    g = lambda n: Internal.getNodeFromName1(h, n)[1]
+    return g("IterationNumber"), g("RSD_L2"), g("RSD_oo"), g("RSD_L2_diff"), g("RSD_oo_diff")
+
+
+def _conv2pytree(z, cvg, nitrun, lft, nd):
+    """conv2pytree.for: append one record (iteration, log10 L2 / Linf of the first recorded
+    sub-iteration and their drop over the step) to the zone's ZoneConvergenceHistory, whose
+    value is the number of records; format "flush" appends the records to residuals.bin
+    (Fortran unformatted) and empties the history, "ascii" appends them to residuals.dat.
+    A full history records nothing more (the reference writes past the end of its arrays)."""
+    h = Internal.getNodeFromName1(z, "ZoneConvergenceHistory")
+    if h is None:
+        return
+    neq = 5
+    itnum, rl2, roo, rl2d, rood = _history_nodes(h)
+    nrec = itnum.size
+    last = int(np.asarray(h[1]).ravel()[0])
+    if lft < 0:
+        name40 = z[0][:40].ljust(40)
+        if lft == -1:
+            import struct
+            payload = name40.encode() + struct.pack("<ii", neq, last)
+            for i in range(last):
+                payload += struct.pack("<i", int(itnum[i]))
+                for j in range(neq):
+                    payload += struct.pack("<dd", rl2[i * neq + j], roo[i * neq + j])
+            with open("residuals.bin", "ab") as f:
+                f.write(struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload)))
+            last = 0
+        else:
+            with open("residuals.dat", "a") as f:
+                if nd == 0:
+                    f.write('VARIABLES="It" "Ro_L2" "RoU_L2" "RoV_L2" "RoW_L2" "RoE_L2" '
+                            '"Ro_oo" "RoU_oo" "RoV_oo" "RoW_oo" "RoE_oo"\n')
+                f.write('ZONE T="%s"\n' % z[0][:40])
+                for i in range(last):
+                    vals = []
+                    for j in range(neq):
+                        vals += [rl2[i * neq + j], roo[i * neq + j]]
+                    f.write("%7d%s\n" % (int(itnum[i]), "".join(" " + _fortran_e(v) for v in vals)))
+    if lft >= 0 and last < nrec:
+        itnum[last] = nitrun
+        sl = slice(last * neq, (last + 1) * neq)
+        rl2[sl] = cvg[0:neq]; roo[sl] = cvg[neq:2 * neq]
+        rl2d[sl] = cvg[2 * neq:3 * neq]; rood[sl] = cvg[3 * neq:4 * neq]
+        last += 1
+    h[1] = np.array([last], dtype=np.asarray(h[1]).dtype)
+
+
+@_api
 def display_temporal_criteria(t, metrics, nitrun, format=None, gmres=None, verbose="firstlast",
                               stopAtNan=True):
-    """Residual L2 / Linf per zone and equation of the last verified sub-iteration
-    (display_ss_iteration.cpp:112-145).  format "store" records them in the tree's
-    ZoneConvergenceHistory; returns {"L2": ..., "Loo": ...} arrays [zone, equation]."""
+    """Convergence monitor of the last verified step (PyTree.py:2039-2076 ->
+    display_ss_iteration.cpp:29-191): per zone, the L2 and Linf residuals of every recorded
+    sub-iteration (divided by DTC), printed as dpssiter.for does (format None: short, "double":
+    long), recorded in the zone's ZoneConvergenceHistory if it exists (conv2pytree.for;
+    "store" records without printing, "flush" / "ascii" also write the records to
+    residuals.bin / residuals.dat).  Returns None, or with ``gmres`` given
+    {"L2": sum of the L2 norms over zones, equations and sub-iterations, "Loo": their maximum
+    Linf}.  stopAtNan: exit(1) when a zone's density is not finite."""
+    from . import _lib
+    if format not in _FORMATS:
+        raise ValueError("format must be one of %s" % sorted(k for k in _FORMATS if k))
+    lft = _FORMATS[format]
+    iverb = 0 if verbose == "firstlast" else 2
     ctx = HOOK["ctx"]
-    l2, loo = [], []
-    for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
-        try:
-            r = ctx.residual_norms(zid)
-        except Exception:
-            r = np.full(10, np.nan)
-        l2.append(r[:5]); loo.append(r[5:])
-        if format == "store":
-            h = Internal.getNodeFromName1(z, "ZoneConvergenceHistory")
-            if h is not None:
-                it = Internal.getNodeFromName1(h, "IterationValues")[1]
-                rl2 = Internal.getNodeFromName1(h, "RSD_L2")[1]
-                rlo = Internal.getNodeFromName1(h, "RSD_oo")[1]
-                k = int(Internal.getNodeFromName1(h, "LastRecordedIteration")[1][0])
-                if k < it.size:
-                    it[k] = nitrun; rl2[k * 5:(k + 1) * 5] = r[:5]; rlo[k * 5:(k + 1) * 5] = r[5:]
-                    Internal.getNodeFromName1(h, "LastRecordedIteration")[1][0] = k + 1
-        elif format is not None or verbose != "none":
-            fmt = "%.16e" if format == "double" else "%.6e"
-            print("iteration %d zone %s: L2 %s Loo %s" % (nitrun, z[0], " ".join(fmt % v for v in r[:5]),
-                                                         " ".join(fmt % v for v in r[5:])))
+    res_l2, res_li = 0.0, -1.e15
+    for nd, ((z, hz), zid) in enumerate(zip(HOOK["zones"], HOOK["ids"])):
+        cvg = None
+        implicit = int(hz.param_int[P.ITYPCP]) <= 1
+        if lft >= 0 and implicit:
+            o = Internal.getNodeFromName1(z, ".Solver#ownData")
+            c = Internal.getNodeFromName1(o, "CFL_minmaxmoy") if o is not None else None
+            if c is not None and lft < 3:
+                cf = np.asarray(c[1]).ravel()
+                print("CFL_MOY_MAX_MIN(Zone= %32.32s, nitrun=%d) %f    %f    %f " % (z[0], nitrun, cf[2], cf[0], cf[1]))
+            try:
+                rdm = ctx.residual_history(zid)
+            except _lib.FastMIError as e:
+                if "no residual" not in str(e):
+                    raise
+                rdm = None
+            if rdm is not None and rdm.shape[0] > 0:
+                res_l2 += float(rdm[:, :5].sum())
+                res_li = max(res_li, float(rdm[:, 5:].max()))
+                cvg = _dpssiter(z[0], nitrun, rdm, lft, iverb)
+        if cvg is not None or lft < 0:
+            _conv2pytree(z, cvg, nitrun, lft, nd)
     if stopAtNan:
-        for z, hz in HOOK["zones"]:
-            if not np.isfinite(hz.ro).all():
+        for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
+            # the kinetic-energy sum is not finite when an interior density or velocity is not
+            if not np.isfinite(ctx.tke(zid, 0)[0]):
                 sys.exit(1)
-    out = {"L2": np.array(l2), "Loo": np.array(loo)}
-    return out if gmres is not None else out
+    if gmres is None:
+        return None
+    return {"L2": res_l2, "Loo": res_li}
 
 
 displayTemporalCriteria = display_temporal_criteria
+_displayTemporalCriteria = display_temporal_criteria
 
 
+@_api
 def _createConvergenceHistory(t, nrecs):
-    for z in Internal.getZones(t):
-        h = Internal.createUniqueChild(z, "ZoneConvergenceHistory", "ConvergenceHistory_t")
-        Internal.createUniqueChild(h, "LastRecordedIteration", "DataArray_t", np.zeros(1, np.int32))
-        Internal.createUniqueChild(h, "IterationValues", "DataArray_t", np.zeros(nrecs, np.int32))
-        Internal.createUniqueChild(h, "RSD_L2", "DataArray_t", np.zeros(5 * nrecs))
-        Internal.createUniqueChild(h, "RSD_oo", "DataArray_t", np.zeros(5 * nrecs))
+    """GlobalConvergenceHistory per base and ZoneConvergenceHistory per zone, each valued with
+    its record count (PyTree.py:2152-2190): IterationNumber (nrecs), RSD_L2, RSD_L2_diff,
+    RSD_oo, RSD_oo_diff (5 nrecs; -inf for the global Linf)."""
+    neqs = 5
+    for b in Internal.getBases(t):
+        g = Internal.createUniqueChild(b, "GlobalConvergenceHistory", "ConvergenceHistory_t", 0)
+        g[2] = []
+        Internal.createChild(g, "IterationNumber", "DataArray_t", np.zeros(nrecs, dtype=np.int32))
+        for v in ("RSD_L2", "RSD_L2_diff"):
+            Internal.createChild(g, v, "DataArray_t", np.zeros(nrecs * neqs))
+        for v in ("RSD_oo", "RSD_oo_diff"):
+            Internal.createChild(g, v, "DataArray_t", np.full(nrecs * neqs, -np.inf))
+        for z in Internal.getNodesFromType1(b, "Zone_t"):
+            h = Internal.createUniqueChild(z, "ZoneConvergenceHistory", "ConvergenceHistory_t", 0)
+            h[2] = []
+            Internal.createChild(h, "IterationNumber", "DataArray_t", np.zeros(nrecs, dtype=np.int32))
+            for v in ("RSD_L2", "RSD_L2_diff", "RSD_oo", "RSD_oo_diff"):
+                Internal.createChild(h, v, "DataArray_t", np.zeros(nrecs * neqs))
+    return None
 
 
+@_api
+def _calc_global_convergence(t, nrec=None):
+    """GlobalConvergenceHistory of every base from its zones (PyTree.py:3852-3910): iteration
+    numbers of the first zone, Linf the maximum over the zones, L2 the average weighted by each
+    zone's point count (the product of the zone's vertex counts, as the reference takes it);
+    all records, or record ``nrec`` only."""
+    neqs = 5
+    ib, ie = (None, None) if nrec is None else (neqs * int(nrec), neqs * (int(nrec) + 1))
+    for b in Internal.getBases(t):
+        zones = Internal.getNodesFromType1(b, "Zone_t")
+        g = Internal.getNodeFromName1(b, "GlobalConvergenceHistory")
+        z0 = Internal.getNodeFromName1(zones[0], "ZoneConvergenceHistory")
+        it0 = Internal.getNodeFromName1(z0, "IterationNumber")[1]
+        itb = Internal.getNodeFromName1(g, "IterationNumber")[1]
+        cnt = np.asarray(g[1]).ravel()
+        if nrec is None:
+            cnt[0] = itb.shape[0]
+            itb[:] = it0[:]
+        else:
+            cnt[0] += 1
+            itb[int(nrec)] = it0[int(nrec)]
+        g[1] = cnt.reshape(np.asarray(g[1]).shape)
+        ncb = 0
+        for z in zones:
+            h = Internal.getNodeFromName1(z, "ZoneConvergenceHistory")
+            ic = Internal.getNodeFromName1(z, "isCalc")
+            if ic is not None and Internal.getValue(ic) != 1:
+                continue
+            d = z[1]
+            ncz = int(d[0][0]) * int(d[1][0]) * int(d[2][0])
+            ncb += ncz
+            for v in ("RSD_oo", "RSD_oo_diff"):
+                gb = Internal.getNodeFromName1(g, v)[1]
+                gb[ib:ie] = np.maximum(gb[ib:ie], Internal.getNodeFromName1(h, v)[1][ib:ie])
+            for v in ("RSD_L2", "RSD_L2_diff"):
+                gb = Internal.getNodeFromName1(g, v)[1]
+                gb[ib:ie] += Internal.getNodeFromName1(h, v)[1][ib:ie] * ncz
+        for v in ("RSD_L2", "RSD_L2_diff"):
+            gb = Internal.getNodeFromName1(g, v)[1]
+            gb[ib:ie] /= float(ncb)
+    return None
+
+
+_calcGlobalConvergence = _calc_global_convergence
+
+
+def write_plt_format(t, path, fcvg, nd, convergence_name="ZoneConvergenceHistory"):
+    """One block of the tecplot ascii history (PyTree.py:2196-2240): the records of the
+    convergence node below ``path``."""
+    node = Internal.getNodeFromPath(t, path + "/" + convergence_name)
+    last = int(np.asarray(node[1]).ravel()[0])
+    it = Internal.getNodeFromName1(node, "IterationNumber")[1]
+    nrec = it.size
+    a = Internal.getNodeFromName1(node, "RSD_L2")[1]
+    neq = a.size // nrec
+    arrs = [np.reshape(Internal.getNodeFromName1(node, v)[1], (neq, nrec), order="F")
+            for v in ("RSD_L2", "RSD_oo", "RSD_L2_diff", "RSD_oo_diff")]
+    if nd == 0:
+        fcvg.write("VARIABLES = it RO_l2 ROU_l2 ROV_l2 ROW_l2 ROE_l2 ROoo ROUoo ROVoo ROWoo ROEoo "
+                   "RO_l2_diff ROU_l2_diff ROV_l2_diff ROW_l2_diff ROE_l2_diff ROoo_diff ROUoo_diff "
+                   "ROVoo_diff ROWoo_diff ROEoo_diff\n")
+    fcvg.write('ZONE T="bloc %s " I=%d F=POINT\n' % (path, last))
+    for l in range(last):
+        vals = "".join("{0:7f} ".format(arr[k, l]) for arr in arrs for k in range(neq))
+        fcvg.write("%s %s\n" % (1 + it[l], vals))
+    return 1
+
+
+@_api
+def _extractConvergenceHistory(t, fileout, perZones=True, perBases=True):
+    """The convergence histories of the tree in a tecplot ascii file (PyTree.py:2245-2270):
+    one block per zone, then one per base whose GlobalConvergenceHistory has records."""
+    nd = 0
+    with open(fileout, "w") as f:
+        if perZones:
+            for path in Internal.getZonePaths(t):
+                nd = write_plt_format(t, path, f, nd, "ZoneConvergenceHistory")
+        if perBases:
+            for path in Internal.getPathsFromType(t, "CGNSBase_t"):
+                b = Internal.getNodeFromPath(t, path)
+                g = Internal.getNodeFromName1(b, "GlobalConvergenceHistory")
+                if g is not None and g[2]:
+                    nd = write_plt_format(t, path, f, nd, "GlobalConvergenceHistory")
+    return None
+
+
+def extractConvergenceHistory(t, fileout, perZones=True, perBases=True):
+    return _extractConvergenceHistory(t, fileout, perZones=perZones, perBases=perBases)
+
+
+# ---------------------------------------------------------------------------------------
+# Restart (FastC.save / load, Fast/FastC/FastC/PyTree.py:3514-3800).  The reference writes CGNS
+# through Cassiopee (HDF5), which is not available: the same tree goes into a NumPy archive
+# (no pickled objects) under the given file name.
+@_api
+def saveTree(t, fileName="restart.cgns", split="single", compress=0):
+    """Save the tree for a restart: the current state is pulled from the GPU first; the _P1
+    level and .Solver#ownData are left out as in the reference (FastC/PyTree.py:3755-3800), so
+    a restart has N and N-1 and runs BDF2 from its first step."""
+    if HOOK.get("stale"):
+        _pull_levels()
+    nodes, arrays = [], {}
+
+    def walk(node, parent):
+        if node[0] == ".Solver#ownData" or node[0].endswith("_P1"):
+            return
+        idx = len(nodes)
+        v = node[1]
+        kind = "none"
+        if isinstance(v, np.ndarray):
+            kind = "array"
+            arrays["v%d" % idx] = np.asfortranarray(v) if v.ndim > 1 else v
+        elif v is not None:
+            kind = "array"
+            arrays["v%d" % idx] = np.asarray(v)
+        nodes.append((node[0], node[3], parent, kind, int(isinstance(v, np.ndarray) and v.ndim > 1 and v.flags.f_contiguous)))
+        for c in node[2]:
+            walk(c, idx)
+    walk(t, -1)
+    import json
+    arrays["__index__"] = np.frombuffer(json.dumps(nodes).encode(), dtype=np.uint8)
+    with open(fileName, "wb") as f:
+        np.savez(f, **arrays)
+
+
+def loadTree(fileName="t.cgns", split="single", graph=False, exploc=0):
+    """Read a tree written by saveTree."""
+    import json
+    d = np.load(fileName, allow_pickle=False)
+    nodes = json.loads(bytes(d["__index__"]).decode())
+    built = []
+    for idx, (name, ntype, parent, kind, fortran) in enumerate(nodes):
+        v = None
+        if kind == "array":
+            v = d["v%d" % idx]
+            if fortran:
+                v = np.asfortranarray(v)
+        node = [name, v, [], ntype]
+        if parent >= 0:
+            built[parent][2].append(node)
+        built.append(node)
+    t = built[0]
+    return (t, None) if graph else t
+
+
+def save(t, fileName="restart.cgns", tc=None, fileNameC="tc_restart.cgns", ts=None,
+         fileNameS="tstat_restart.cgns", split="single", compress=0):
+    """Save the trees (FastC.PyTree.save): t, and tc / ts when given."""
+    saveTree(t, fileName=fileName, split=split, compress=compress)
+    if tc is not None and not isinstance(tc, list):
+        saveTree(tc, fileName=fileNameC, split=split, compress=compress)
+    if ts is not None:
+        saveTree(ts, fileName=fileNameS, split=split, compress=compress)
+
+
+def load(fileName="t.cgns", fileNameC=None, fileNameS=None, split="single", exploc=0):
+    """(t, tc, ts, graph) as FastC.PyTree.load: t from fileName, tc / ts when their names are
+    given, no communication graph (one GPU)."""
+    t = loadTree(fileName, split=split)
+    tc = loadTree(fileNameC, split=split) if fileNameC is not None else None
+    ts = loadTree(fileNameS, split=split) if fileNameS is not None else None
+    return t, tc, ts, None
+
+
+@_api
 def _computeEnstrophy(t, metrics):
     """(enstrophy, tke) volume averages over the interior cells of all zones
     (computePT_enstrophy.cpp:206-227): sum 1/2 rho |curl u|^2 vol / sum vol and
     sum 1/2 rho |u|^2 vol / sum vol, from the N level."""
     ctx = HOOK["ctx"]
+    _device_current()
     k = e = v = 0.0
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):
         s = ctx.tke(zid, 0)
@@ -457,6 +834,7 @@ EFF_VARS = ["Density", "MomentumX", "MomentumY", "MomentumZ", "EnergyStagnationD
 EFF_LOOP, EFF_I0, EFF_NONZ, EFF_IDIR = 23, 29, 34, 35
 
 
+@_api
 def createStressNodes(t, BC=None, windows=None):
     """A tree of surface zones, one per selected wall window, to receive the efforts
     (PyTree.py:2277-2530).  BC: list of BC types (e.g. ['BCWall', 'BCWallViscous']); windows:
@@ -536,12 +914,14 @@ ADJ_VARS = {c: ["dp%sp_dp%s" % (c.upper(), v) for v in ("Density", "MomentumX", 
                                                          "EnergyStagDens")] for c in ("cd", "cl")}
 
 
+@_api
 def _compute_dpJ_dpW(t, teff, metrics, cosAoA, sinAoA, surfinv):
     """Gradients of the pressure drag and lift coefficients of the slip-wall windows of teff with
     respect to the conservative state, assigned into the fields dpCDp_dp* and dpCLp_dp* of the
     wall cells (compute_dpJ_dpW.cpp:33-261 via PyTree.py:3719-3735; the fields are created by
     warmup(..., Adjoint=True)).  AUSM+(P) o3 Euler zones."""
     ctx = HOOK["ctx"]
+    _device_current()
     for ze in Internal.getZones(teff):
         pe = np.asarray(Internal.getNodeFromName1(Internal.getNodeFromName1(ze, ".Solver#ownData"),
                                                   "Parameter_int")[1])
@@ -565,6 +945,7 @@ def _compute_dpJ_dpW(t, teff, metrics, cosAoA, sinAoA, surfinv):
     return None
 
 
+@_api
 def _computeStress(t, teff, metrics, xyz_ref=(0., 0., 0.)):
     """Efforts on the windows of teff from the N level of every zone, fields stored in teff
     (compute_effort.cpp -> bceffort.for).  Returns the 11 sums of the reference: forces x, y, z
@@ -572,6 +953,7 @@ def _computeStress(t, teff, metrics, xyz_ref=(0., 0., 0.)):
     scaled by 2 / (rho_inf V_inf^2)), the wetted area, and the mass, x-, y-, z-momentum flux
     sums."""
     ctx = HOOK["ctx"]
+    _device_current()
     effort = np.zeros(11)
     for z in Internal.getZones(teff):
         pe = np.asarray(Internal.getNodeFromName1(Internal.getNodeFromName1(z, ".Solver#ownData"),

@@ -41,6 +41,7 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_float)]),
     "fm_ctx_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "fm_cfl": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "fm_residual_history": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip]),
     "fm_transfers_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _lp, _lp, _lp]),
     "fm_transfer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fm_wall_effort": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
@@ -61,6 +62,7 @@ OPT_RESIDUAL_SCHEDULE = 2
 RES_CELLS, RES_FACES, RES_KCOL = 0, 1, 2
 OPT_ZONE_BATCH = 3
 OPT_GRAPH = 4
+OPT_LU_GIVE_UP = 6
 
 
 class FastMIError(RuntimeError):
@@ -246,6 +248,14 @@ class Context:
         out = np.zeros(10)
         check(self.lib.fm_residual_norms(self.h, zone, dptr(out)))
         return out
+
+    def residual_history(self, zone, nmax=64):
+        """(nrec, 10) array: L2 (5) and Linf (5) of every sub-iteration of the last verified
+        step, divided by DTC."""
+        out = np.zeros(10 * nmax)
+        n = ctypes.c_int()
+        check(self.lib.fm_residual_history(self.h, zone, out.ctypes.data_as(_dp), nmax, ctypes.byref(n)))
+        return out[:10 * min(n.value, nmax)].reshape(-1, 10)
 
     def cfl(self, zone):
         out = np.zeros(3)

@@ -58,6 +58,9 @@ struct Zone {
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
+    // residuals of the sub-iterations of the last verified step, one slot of 10 doubles (sum of
+    // squares, max |R| per equation) per sub-iteration; rec_lo..rec_hi are the recorded ones
+    double* norm_hist; int rec_lo, rec_hi;
     long n_metric;
 };
 
@@ -139,7 +142,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
 static void free_zone(Zone* z) {
     double* bufs[] = {z->ro[0], z->ro[1], z->ro[2], z->ro_scratch, z->k.xmut, z->k.drodm,
                       z->k.coe, z->k.wig, z->k.ti, (double*)z->k.x, (double*)z->k.y,
-                      (double*)z->k.z, z->norm_part, z->norm_out};
+                      (double*)z->k.z, z->norm_part, z->norm_out, z->norm_hist};
     for (double* b : bufs) if (b) (void)hipFree(b);
     for (BcRec& b : z->bcs) if (b.pext) (void)hipFree(b.pext);
     if (z->lu_dws) (void)hipFree(z->lu_dws);
@@ -227,7 +230,8 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
     z->lu_fx = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
-    z->norm_part = z->norm_out = nullptr; z->have_norms = 0; z->have_cfl = 0;
+    z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
+    z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
     k.ni = p[IJKV]; k.nj = p[IJKV + 1]; k.nk = p[IJKV + 2];
     k.nit = p[NIJK]; k.njt = p[NIJK + 1]; k.nkt = p[NIJK + 2];
@@ -327,7 +331,8 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     k.vol = mtr + ndm * (2 * p[NEQ_IJ] + p[NEQ_K]);
     k.x = xd; k.y = yd; k.z = zd;
     z->norm_blocks = 512;
-    if ((rc = dmalloc(&z->norm_part, 10 * z->norm_blocks)) || (rc = dmalloc(&z->norm_out, 16))) {
+    if ((rc = dmalloc(&z->norm_part, 10 * z->norm_blocks)) || (rc = dmalloc(&z->norm_out, 16)) ||
+        (rc = dmalloc(&z->norm_hist, 10 * (long)(k.nssiter > 0 ? k.nssiter : 1)))) {
         free_zone(z);
         return rc;
     }
@@ -390,6 +395,8 @@ extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* h
     return FM_OK;
 }
 
+static int check_lu(fm_ctx* c);
+
 extern "C" int fm_zone_get_field(fm_ctx* c, int zone, int field, double* host, long n) {
     Zone* z = get_zone(c, zone);
     if (!z || !host) return fail(FM_E_ARG, "fm_zone_get_field: bad arguments");
@@ -397,6 +404,8 @@ extern "C" int fm_zone_get_field(fm_ctx* c, int zone, int field, double* host, l
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
+    // a sweep whose bounded wait gave up leaves wrong values: report it before handing them out
+    if (int rc = check_lu(c)) return rc;
     HIPCHK(hipMemcpy(host, d, sizeof(double) * n, hipMemcpyDeviceToHost));
     return FM_OK;
 }
@@ -627,8 +636,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
         bczone(c, z, 1, k.drodm, shift_lu, &lcorner);
         if (lcorner == 0) coins(c, z, k.drodm, shift_lu);
         if (lssiter_verif == 1) {
-            launch_norms(k, k.drodm, z->norm_part, z->norm_blocks, z->norm_out, st);
-            z->have_norms = 1;
+            launch_norms(k, k.drodm, z->norm_part, z->norm_blocks, z->norm_hist + 10 * (nitcfg - 1), st);
         }
         if (stop == 2) return FM_OK;
         if (lexit_lu == 0) {
@@ -690,6 +698,12 @@ extern "C" int fm_zone_substep(fm_ctx* c, int zone, int nitcfg, int nitrun, int 
     HIPCHK(hipSetDevice(c->device));
     int rc = substep(c, z, nitcfg, first_it, lssiter_verif, lexit_lu, stop);
     if (rc) return rc;
+    if (lssiter_verif == 1 && z->k.itypcp <= 1 && stop != 1) {
+        // a later sub-iteration extends the record of the step, an earlier one starts a new one
+        if (z->have_norms && nitcfg == z->rec_hi + 1) z->rec_hi = nitcfg;
+        else { z->rec_lo = z->rec_hi = nitcfg; }
+        z->have_norms = 1;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return check_lu(c);
 }
@@ -1121,6 +1135,10 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
         }
         first_it = 1;
         for (Zone* z : c->zones) {
+            // a verified step records the residual of every sub-iteration it ran
+            if (verif && z->k.itypcp <= 1 && nstep_fin >= nstep_deb) {
+                z->rec_lo = nstep_deb; z->rec_hi = nstep_fin; z->have_norms = 1;
+            }
             double* m1 = z->ro[1]; z->ro[1] = z->ro[0]; z->ro[0] = z->ro[2]; z->ro[2] = m1;
         }
     }
@@ -1144,7 +1162,7 @@ extern "C" int fm_compute_timed(fm_ctx* c, int nitrun, int nstep_deb, int nstep_
     HIPCHK(hipEventRecord(c->e1, c->stream));
     HIPCHK(hipEventSynchronize(c->e1));
     if (ms) HIPCHK(hipEventElapsedTime(ms, c->e0, c->e1));
-    return FM_OK;
+    return check_lu(c);
 }
 
 extern "C" int fm_sync(fm_ctx* c) {
@@ -1172,6 +1190,8 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
     if (option == FM_OPT_LU_SCHEDULE) {
         if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE && value != FM_LU_DATAFLOW)
             return fail(FM_E_ARG, "LU schedule %d", value);
+        if (value == FM_LU_PIPELINE && c->lu_fp32)
+            return fail(FM_E_UNSUPPORTED, "the pipelined LU-SGS schedule has no FP32 sweeps");
         c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : value == FM_LU_DATAFLOW ? LU_DATAFLOW : LU_BLOCKS;
         c->graph_epoch++;
         return FM_OK;
@@ -1191,8 +1211,20 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
     }
     if (option == FM_OPT_LU_PRECISION) {
         if (value != FM_LU_FP64 && value != FM_LU_FP32) return fail(FM_E_ARG, "LU precision %d", value);
+        if (value == FM_LU_FP32 && c->lu_schedule == LU_PIPELINE)
+            return fail(FM_E_UNSUPPORTED, "the pipelined LU-SGS schedule has no FP32 sweeps");
         c->lu_fp32 = value == FM_LU_FP32;
         c->graph_epoch++;
+        return FM_OK;
+    }
+    if (option == FM_OPT_LU_GIVE_UP) {
+        // failure-handling test hook: set every zone's LU error word as a wait that gave up would
+        if (value != 1) return fail(FM_E_ARG, "LU give-up %d", value);
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int one = 1;
+        for (Zone* z : c->zones)
+            if (z->lu_sync) HIPCHK(hipMemcpy(z->lu_sync + 1, &one, sizeof(int), hipMemcpyHostToDevice));
         return FM_OK;
     }
     if (option == FM_OPT_GRAPH) {
@@ -1210,13 +1242,37 @@ extern "C" int fm_residual_norms(fm_ctx* c, int zone, double* out) {
     if (!z->have_norms) return fail(FM_E_ARG, "no residual computed yet");
     double h[10];
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(h, z->norm_out, sizeof h, hipMemcpyDeviceToHost));
+    if (int rc = check_lu(c)) return rc;
+    HIPCHK(hipMemcpy(h, z->norm_hist + 10 * (z->rec_hi - 1), sizeof h, hipMemcpyDeviceToHost));
     double ncell = (double)z->k.ni * z->k.nj * z->k.nk;
     double dtc = z->k.dtc;
     for (int e = 0; e < 5; e++) {
         out[e] = sqrt(h[e] / ncell) / dtc;
         out[5 + e] = h[5 + e] / dtc;
     }
+    return FM_OK;
+}
+
+// display_ss_iteration.cpp:96-137: L2 = sqrt(sum / cells) / DTC and Linf = max / DTC of every
+// sub-iteration the last verified step recorded, in sub-iteration order.
+extern "C" int fm_residual_history(fm_ctx* c, int zone, double* out, int nmax, int* nrec) {
+    Zone* z = get_zone(c, zone);
+    if (!z || !out || !nrec || nmax < 0) return fail(FM_E_ARG, "fm_residual_history: bad arguments");
+    if (!z->have_norms) return fail(FM_E_ARG, "no residual computed yet");
+    const int n = z->rec_hi - z->rec_lo + 1;
+    *nrec = n;
+    const int m = n < nmax ? n : nmax;
+    if (m == 0) return FM_OK;
+    std::vector<double> h((size_t)10 * m);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (int rc = check_lu(c)) return rc;
+    HIPCHK(hipMemcpy(h.data(), z->norm_hist + 10 * (z->rec_lo - 1), sizeof(double) * 10 * m, hipMemcpyDeviceToHost));
+    const double ncell = (double)z->k.ni * z->k.nj * z->k.nk, dtc = z->k.dtc;
+    for (int r = 0; r < m; r++)
+        for (int e = 0; e < 5; e++) {
+            out[10 * r + e] = sqrt(h[10 * r + e] / ncell) / dtc;
+            out[10 * r + 5 + e] = h[10 * r + 5 + e] / dtc;
+        }
     return FM_OK;
 }
 
@@ -1299,5 +1355,5 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, c->e0, c->e1));
     *ms = t / reps;
-    return FM_OK;
+    return check_lu(c);
 }

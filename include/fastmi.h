@@ -150,6 +150,12 @@ int fm_sync(fm_ctx* ctx);
 /* Residual of the last verified sub-iteration: out[0..4] L2, out[5..9] Linf per equation,
    already divided by DTC like display_ss_iteration.cpp:140-141. */
 int fm_residual_norms(fm_ctx* ctx, int zone, double* out);
+/* Residuals of every sub-iteration of the last verified step (the table display_ss_iteration
+   assembles, DISPLAY/display_ss_iteration.cpp:96-137, from the cprdu3s1.for block values): for
+   the r-th recorded sub-iteration, out[10 r + e] = L2 and out[10 r + 5 + e] = Linf of equation
+   e, divided by DTC.  *nrec receives the number of recorded sub-iterations; at most nmax are
+   written. */
+int fm_residual_history(fm_ctx* ctx, int zone, double* out, int nmax, int* nrec);
 /* CFL monitor of the last verification step (cpcfl3.for, computePT.cpp:629-645):
    out[0] max, out[1] min, out[2] mean of the local CFL number over the interior cells. */
 int fm_cfl(fm_ctx* ctx, int zone, double* out);
@@ -178,9 +184,12 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
 /* FM_LU_DATAFLOW (default): one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
-   each starts as soon as its three neighbour blocks have handed over their face products
-   (bounded waits; a wait that gives up makes the next call report FM_E_HIP).  With several
-   zones and FM_OPT_ZONE_BATCH on, the sweeps run as the batched blocks schedule. */
+   each starts as soon as its three neighbour blocks have handed over their face products.  The
+   waits are bounded: one that gives up sets an error word, the sweeps still run to their end
+   (with wrong values), and the next call that waits for the stream -- fm_sync, fm_zone_substep,
+   fm_zone_get_field, fm_residual_norms, fm_compute_timed, fm_time_phase -- returns FM_E_HIP and
+   clears the word (fm_compute itself is asynchronous and returns FM_OK).  With several zones
+   and FM_OPT_ZONE_BATCH on, the sweeps run as the batched blocks schedule. */
 #define FM_LU_DATAFLOW 2
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
@@ -207,6 +216,11 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_OPT_LU_PRECISION 5
 #define FM_LU_FP64 0
 #define FM_LU_FP32 1
+/* FM_OPT_LU_GIVE_UP, value 1: test hook for the failure path above; sets the error word of every
+   zone as a wait that gave up would (the next sweeps skip their waits and finish, the next
+   waiting call reports FM_E_HIP).  FP32 sweeps and FM_LU_PIPELINE exclude each other
+   (FM_E_UNSUPPORTED, in either order of setting). */
+#define FM_OPT_LU_GIVE_UP 6
 int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
 
 #define FM_PHASE_COE 0

@@ -440,3 +440,39 @@ def test_cache_blocking_limits(ctxmod):
     with pytest.raises(ctxmod.FastMIError, match="positive"):
         ctx.add_zone(z)
     ctx.close()
+
+
+def test_lu_wait_give_up_is_reported(ctxmod):
+    """The failure path of the bounded LU-SGS waits, forced from the host (FM_OPT_LU_GIVE_UP sets
+    the error word a wait that gives up would set): the dataflow sweeps still finish, the next
+    call that waits for the stream reports FM_E_HIP instead of handing out the state, the word is
+    cleared, and the following steps run normally."""
+    num = H.numerics("ausmpred", "bdf")
+    z = cases.tgv_zone((24, 20, 18), num, perturb=0.02)
+    ctx = ctxmod.Context(0)
+    ctx.set_option(ctxmod.OPT_LU_SCHEDULE, ctxmod.LU_DATAFLOW)
+    zid = ctx.add_zone(z)
+    ctx.apply_bc(zid, 0)
+    ctx.compute(0, 1, 1, nit_c=1, modulo_verif=0, first_it=0)
+    ctx.sync()
+    ctx.set_option(ctxmod.OPT_LU_GIVE_UP, 1)
+    ctx.compute(1, 1, 1, nit_c=2, modulo_verif=0, first_it=1)
+    buf = np.zeros_like(z.ro)
+    with pytest.raises(ctxmod.FastMIError, match="timed out"):
+        ctx.get(zid, ctxmod.FIELD_RO, buf)
+    ctx.get(zid, ctxmod.FIELD_RO, buf)          # reported once, then cleared
+    ctx.compute(3, 1, 1, nit_c=1, modulo_verif=0, first_it=1)
+    ctx.sync()
+    ctx.close()
+
+
+def test_fp32_and_pipeline_exclude_each_other(ctxmod):
+    ctx = ctxmod.Context(0)
+    ctx.set_option(ctxmod.OPT_LU_SCHEDULE, ctxmod.LU_PIPELINE)
+    with pytest.raises(ctxmod.FastMIError, match="FP32"):
+        ctx.set_option(ctxmod.OPT_LU_PRECISION, ctxmod.LU_FP32)
+    ctx.set_option(ctxmod.OPT_LU_SCHEDULE, ctxmod.LU_DATAFLOW)
+    ctx.set_option(ctxmod.OPT_LU_PRECISION, ctxmod.LU_FP32)
+    with pytest.raises(ctxmod.FastMIError, match="FP32"):
+        ctx.set_option(ctxmod.OPT_LU_SCHEDULE, ctxmod.LU_PIPELINE)
+    ctx.close()

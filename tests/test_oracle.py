@@ -17,7 +17,9 @@ from fast_amd.zone import HostZone
 from tests import helpers as H
 import oracle
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+# the case fixtures of gen_golden.py (gen_histories.py writes the *_history / *_ke ones)
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                if not os.path.basename(p).startswith(("channel257_history", "tgv64_rk3_ke")))
 
 
 @pytest.fixture(scope="module")
