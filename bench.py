@@ -34,7 +34,8 @@ KERNELS = {"coe": "k_coe", "bc": "k_bc_face",
            "residual": {"cells": "k_residual", "faces": "k_residual_faces", "kcol": "k_residual_kcol"},
            "lu": {"blocks": "k_lu_block (lower + upper sweeps, one launch per block diagonal)",
                   "pipeline": "k_lu_pipe (lower + upper sweeps)",
-                  "dataflow": "k_lu_flow (lower + upper sweeps, one launch each)"}}
+                  "dataflow": "k_lu_flow (lower + upper sweeps, one launch each)",
+                  "stream": "k_lu_stream (lower + upper sweeps, one launch each)"}}
 
 
 def parse():
@@ -48,7 +49,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal timing")
     ap.add_argument("--phase-reps", type=int, default=5)
-    ap.add_argument("--lu", choices=["pipeline", "blocks", "dataflow"], default="dataflow")
+    ap.add_argument("--lu", choices=["pipeline", "blocks", "dataflow", "stream"], default="dataflow")
     ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="kcol")
     ap.add_argument("--lu-precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp32: mixed precision, LU-SGS sweep arithmetic in FP32 (state, residual FP64)")
@@ -247,7 +248,7 @@ def main():
     z = make_zone(a.n, a.scheme)
     ctx = _lib.Context(0)
     ctx.set_option(_lib.OPT_LU_SCHEDULE, {"pipeline": _lib.LU_PIPELINE, "blocks": _lib.LU_BLOCKS,
-                                          "dataflow": _lib.LU_DATAFLOW}[a.lu])
+                                          "dataflow": _lib.LU_DATAFLOW, "stream": _lib.LU_STREAM}[a.lu])
     ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
                    {"cells": _lib.RES_CELLS, "faces": _lib.RES_FACES, "kcol": _lib.RES_KCOL}[a.residual])
     ctx.set_option(_lib.OPT_GRAPH, a.graph)

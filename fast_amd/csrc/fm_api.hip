@@ -55,6 +55,7 @@ struct Zone {
     double* lu_xch;          // tile-boundary exchange of the pipelined LU sweeps (on first use)
     double* adj_buf;         // face contributions of the transposed product (on first use)
     double* lu_fx;           // dataflow LU schedule: face exchange and block order (on first use)
+    double* lu_stx;          // streaming LU schedule: pencil exchange (on first use)
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
@@ -150,6 +151,7 @@ static void free_zone(Zone* z) {
     if (z->lu_xch) (void)hipFree(z->lu_xch);
     if (z->adj_buf) (void)hipFree(z->adj_buf);
     if (z->lu_fx) (void)hipFree(z->lu_fx);
+    if (z->lu_stx) (void)hipFree(z->lu_stx);
     if (z->flow_work) (void)hipFree(z->flow_work);
     delete z;
 }
@@ -229,7 +231,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
-    z->lu_fx = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
+    z->lu_fx = nullptr; z->lu_stx = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
     z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
@@ -499,8 +501,31 @@ static int ensure_exchange(fm_ctx* c, Zone* z) {
 
 // Dataflow LU schedule: the face exchange (sized for the largest LU box, interior plus a wall
 // layer per side) and the block order of the LU box b (rebuilt when the box changes).
+// Streaming LU schedule of a Cartesian zone: the pencil exchange, sized for the largest LU box and
+// filled with the pending pattern once (the sweeps leave it that way).
+static bool uses_stream(const fm_ctx* c, const Zone* z) {
+    return c->lu_schedule == LU_STREAM && z->k.typ == 2 && !c->lu_fp32;
+}
+
+static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
+    la.stx = nullptr;
+    if (!uses_stream(c, z)) return FM_OK;
+    if (!z->lu_stx) {
+        const KZ& k = z->k;
+        long n = lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMalloc(&z->lu_stx, sizeof(double) * n));
+        fill_stream_exchange(z->lu_stx, n, c->stream);
+        HIPCHK(hipGetLastError());
+    }
+    la.stx = z->lu_stx;
+    return FM_OK;
+}
+
 static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
-    if (c->lu_schedule != LU_DATAFLOW) return FM_OK;
+    // the dataflow schedule, and what the streaming schedule falls back to (curvilinear zones,
+    // FP32 sweeps)
+    if (c->lu_schedule != LU_DATAFLOW && !(c->lu_schedule == LU_STREAM && !uses_stream(c, z))) return FM_OK;
     const KZ& k = z->k;
     if (!z->lu_fx) {
         long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
@@ -651,6 +676,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             {
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
                 if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
+                if (int rc = ensure_stream(c, z, la)) return rc;
             }
             la.rop = ro_ssiter;
             if (ro_ssiter == ro_p1) {
@@ -684,7 +710,15 @@ static int check_lu(fm_ctx* c) {
         la.schedule = LU_PIPELINE; la.sync = z->lu_sync;
         int f = lu_pipeline_failed(la, c->stream);
         if (f < 0) return fail(FM_E_HIP, "reading the LU error word failed");
-        if (f > 0) return fail(FM_E_HIP, "LU-SGS sweep timed out waiting for a neighbour tile or block");
+        if (f > 0) {
+            // a sweep that gave up may leave handed-over values in the streaming exchange
+            if (z->lu_stx) {
+                const KZ& k = z->k;
+                fill_stream_exchange(z->lu_stx, lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2), c->stream);
+                (void)hipStreamSynchronize(c->stream);
+            }
+            return fail(FM_E_HIP, "LU-SGS sweep timed out waiting for a neighbour tile or block");
+        }
     }
     return FM_OK;
 }
@@ -1039,7 +1073,8 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
     // several zones: the batched blocks schedule (one launch per diagonal for all zones) also
     // stands in for the dataflow schedule, whose per-zone launches and exchange fills cost more
     // than they save on many small zones
-    const bool batch = nz > 1 && c->lu_batch && (c->lu_schedule == LU_BLOCKS || c->lu_schedule == LU_DATAFLOW);
+    const bool batch = nz > 1 && c->lu_batch &&
+                       (c->lu_schedule == LU_BLOCKS || c->lu_schedule == LU_DATAFLOW || c->lu_schedule == LU_STREAM);
     std::vector<int> lexit(nz);
     std::vector<double*> cl(nz);
     for (int nstep = n0; nstep <= n1; nstep++) {
@@ -1188,11 +1223,12 @@ extern "C" int fm_cfl(fm_ctx* c, int zone, double* out) {
 extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
     if (!c) return fail(FM_E_ARG, "null context");
     if (option == FM_OPT_LU_SCHEDULE) {
-        if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE && value != FM_LU_DATAFLOW)
+        if (value != FM_LU_BLOCKS && value != FM_LU_PIPELINE && value != FM_LU_DATAFLOW && value != FM_LU_STREAM)
             return fail(FM_E_ARG, "LU schedule %d", value);
         if (value == FM_LU_PIPELINE && c->lu_fp32)
             return fail(FM_E_UNSUPPORTED, "the pipelined LU-SGS schedule has no FP32 sweeps");
-        c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : value == FM_LU_DATAFLOW ? LU_DATAFLOW : LU_BLOCKS;
+        c->lu_schedule = value == FM_LU_PIPELINE ? LU_PIPELINE : value == FM_LU_DATAFLOW ? LU_DATAFLOW
+                       : value == FM_LU_STREAM ? LU_STREAM : LU_BLOCKS;
         c->graph_epoch++;
         return FM_OK;
     }
@@ -1321,6 +1357,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
+        if (int rc = ensure_stream(c, z, la)) return rc;
     }
     Box b = {1 - k.nf, k.ni + k.nf, 1 - k.nf, k.nj + k.nf, 1 - k.nfk, k.nk + k.nfk};
     if (phase == FM_PHASE_VJP && (k.nf < 2 || k.nfk < 2))

@@ -56,8 +56,14 @@ struct LuArgs {
     const LuWork* flow_work;
     int flow_n;
     double* fx;
+    // streaming schedule: the pencil-to-pencil exchange (pending pattern when idle)
+    double* stx;
 };
-enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2 };
+enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
+// doubles of the streaming exchange of an LU box of ni x nj x nk cells, and its fill with the
+// pending pattern (once when allocated, and after a wait gave up)
+long lu_stream_exchange_size(int ni, int nj, int nk);
+void fill_stream_exchange(double* x, long n, hipStream_t st);
 // doubles of the dataflow face exchange of an LU box of ni x nj x nk cells
 long lu_flow_exchange_size(bool cart, int ni, int nj, int nk);
 

@@ -732,6 +732,304 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Streaming schedule (LU_STREAM, Cartesian zones).  One wave per pencil of 8 x 8 (j, k) lines,
+// sweeping the whole i extent of the LU box as a skewed wavefront: lane (jj, kk) handles cell
+// ii = s - jj - kk of its line at step s, so the cell it needs from the line below in j (k) was
+// finished one step earlier by lane L - 1 (L - 8).
+//
+//  * The wave solves its 64 cells (the reference's neighbour terms and division) and forms the
+//    three products of each for its consumers in i, j and k: the i product stays in the lane's
+//    registers, the j product moves to lane L + 1 by a DPP row shift, the k product to lane L + 8
+//    by ds_bpermute.  No barrier: the wave runs the pencil alone, and four pencils share a CU.
+//  * Inputs reach LDS by direct loads (global_load_lds_dwordx4): every lane fetches the cells of
+//    its own line two at a time (16 contiguous bytes per array), into a ring of two pairs; a pair
+//    is refilled with the cells four steps ahead as soon as its two steps have read it.
+//  * Pencils hand their j-face (k-face) products to the next pencil in j (k) through a global
+//    exchange, one value per (pencil, lane, variable, cell), with agent-scope stores; the
+//    consumer lanes load them two steps ahead, re-read values that are still the pending pattern
+//    (bounded waits, error word) and store the pending pattern back once they have read a value,
+//    so the exchange is ready for the next sweep without a fill.
+//  * Pencils are taken from a counter in diagonal order: a pencil only waits on pencils handed
+//    out before it, so there is no deadlock whatever the dispatch order.
+// Every cell runs the expressions of k_lu_block in the same order, so the results are the
+// reference's bit for bit.
+//
+// Requirements, checked by the host: a Cartesian zone, one ghost cell at least beyond the LU box
+// at both ends in i (a pair may straddle the box end by one cell).
+
+// lane-owned ring of two pairs: pair m holds, for lane L, the two cells of its line that it
+// handles at steps 2m and 2m + 1 of a window of four steps, in memory order; arrays 0-4 are the
+// right-hand side (dW* in the upper sweep), 5-9 the state, 10-14 coe
+struct LuStPair { double v[15][64][2]; };
+
+// steps of slack a consumer lane builds up once it had to wait for the previous pencil
+constexpr int kLuStSlack = 6;
+
+// exchange slot of value e of cell ii of lane l8 on the j (face 0) or k (face 1) side of tile t
+__device__ __forceinline__ long lu_st_slot(int face, int t, int l8, int e, int ii, int ntile, int ni) {
+    return (long)face * ntile * 40 * ni + ((long)(t * 8 + l8) * 5 + e) * ni + ii;
+}
+
+long lu_stream_exchange_size(int ni, int nj, int nk) {
+    long nt = (long)((nj + 7) / 8) * ((nk + 7) / 8);
+    return 2 * nt * 40 * (long)ni;
+}
+
+// the value of lane L - 1 (row shift by one inside rows of 16 lanes: lanes 0 and 8 of a row
+// keep their own value, they take the j neighbour from the exchange)
+__device__ __forceinline__ double lu_from_jm1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x111, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x111, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// the value of lane L - 8 (lanes 0..7 read their own value; they take the k neighbour from the
+// exchange)
+__device__ __forceinline__ double lu_from_km1(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int src = (lane >= 8 ? lane - 8 : lane) << 2;
+    const int lo = __builtin_amdgcn_ds_bpermute(src, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <bool LOWER>
+__global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int ntk) {
+    __shared__ LuStPair P0, P1;
+    const int L = threadIdx.x, jj = L & 7, kk = L >> 3;
+    const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    const int ntile = ntj * ntk;
+    const long nd = z.nd;
+    const LUC<> c = lu_const<double>(z);
+    const double signe = LOWER ? 0.5 : -0.5;
+    const double* __restrict__ rop = a.rop;
+    const double* __restrict__ coe = z.coe;
+    double* __restrict__ dw = a.dw;
+    double* __restrict__ X = a.stx;
+    int* err = a.sync + 1;
+    const bool kalias = !LOWER && a.rop_k != a.rop;
+    const bool newton = !LOWER && a.newton;
+    const double tci = z.ti[0], tcj = z.tj[0], tck = z.tk[0];
+    const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
+    const double pending = __longlong_as_double(kLuPending);
+    for (;;) {
+        int tk0 = 0;
+        if (L == 0) tk0 = atomicAdd(a.sync, 1);
+        const int t = __shfl(tk0, 0, 64);
+        if (t >= ntile) break;
+        int jt, kt;
+        lu_tile_of(t, ntj, ntk, jt, kt);
+        const int tileid = jt + ntj * kt;
+        const int oj = jt * 8 + jj, ok = kt * 8 + kk;
+        const bool lane_ok = oj < nj && ok < nk;
+        const int j = LOWER ? a.j0 + oj : a.j1 - oj, k = LOWER ? a.k0 + ok : a.k1 - ok;
+        // memory index of the lane's cell ii: lb + ii (lower sweep) or lb - ii (upper sweep)
+        const long lb = lane_ok ? IDX(z, LOWER ? a.i0 : a.i1, j, k) : 0;
+        const int nsteps = ni + 14;
+        const int nwin = (nsteps + 3) / 4;
+        // the products of this lane's line feed the next pencil in j (k)
+        const bool put_j = jj == 7 && oj + 1 < nj && lane_ok;
+        const bool put_k = kk == 7 && ok + 1 < nk && lane_ok;
+        // the lanes of the first line in j (k) take the j (k) products of the previous pencil
+        const bool get_j = jj == 0 && jt > 0 && lane_ok;
+        const bool get_k = kk == 0 && kt > 0 && lane_ok;
+        double* hjp = X + lu_st_slot(0, get_j ? (jt - 1) + ntj * kt : 0, kk, 0, 0, ntile, ni);
+        double* hkp = X + lu_st_slot(1, get_k ? jt + ntj * (kt - 1) : 0, jj, 0, 0, ntile, ni);
+
+        // pair m of window w: cells c0 = 4 w + 2 m - jj - kk and c0 + 1 of the lane's line; all
+        // lanes load (lanes without a cell there read the start of the arrays, never used)
+        auto dma = [&](int w, int m) {
+            const int c0 = 4 * w + 2 * m - jj - kk;
+            const bool any = lane_ok && c0 + 1 >= 0 && c0 < ni;
+            const long lm = any ? (LOWER ? lb + c0 : lb - (c0 + 1)) : 0;
+            LuStPair& Pm = m ? P1 : P0;
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+                __builtin_amdgcn_global_load_lds(dw + lm + e * nd, (lds_void*)&Pm.v[e][0][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(rop + lm + e * nd, (lds_void*)&Pm.v[5 + e][0][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(coe + lm + e * nd, (lds_void*)&Pm.v[10 + e][0][0], 16, 0, 0);
+            }
+        };
+        // halo products of step s, two steps ahead (buffer s & 1), loaded by the lanes that take
+        // them (the exchange is read by those lanes only: no other lane touches it)
+        double hj[2][5], hk[2][5];
+        auto hload = [&](int s, int bsel) {
+            const int ii = s - jj - kk;
+            const bool v = ii >= 0 && ii < ni;
+            if (get_j && v) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) hj[bsel][e] = lu_ldc(hjp + e * ni + ii);
+            }
+            if (get_k && v) {
+#pragma unroll
+                for (int e = 0; e < 5; e++) hk[bsel][e] = lu_ldc(hkp + e * ni + ii);
+            }
+        };
+        // the products of the previous step: i (own next cell), and the j and k products of the
+        // lanes below, already moved
+        double pi[5], bj[5], bk[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) { pi[e] = 0.; bj[e] = 0.; bk[e] = 0.; }
+
+        dma(0, 0);
+        dma(0, 1);
+        hload(0, 0);
+        hload(1, 1);
+        for (int win = 0; win < nwin; win++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int s = win * 4 + q;
+                const int ii = s - jj - kk;
+                const bool act = s < nsteps && lane_ok && ii >= 0 && ii < ni;
+                const int hb = q & 1;
+                // the inputs of the step (lanes without a cell read stale values, never used)
+                const LuStPair& Pm = q < 2 ? P0 : P1;
+                const int sl = LOWER ? (q & 1) : 1 - (q & 1);
+                double x[5], wv[5], cf[5];
+#pragma unroll
+                for (int e = 0; e < 5; e++) {
+                    x[e] = Pm.v[e][L][sl];
+                    wv[e] = Pm.v[5 + e][L][sl];
+                    cf[e] = Pm.v[10 + e][L][sl];
+                }
+                // the sweep's divisor, sign products and scaled right-hand side
+                const double xal = cf[0] * signe, diag = 1. / cf[4];
+                const double ci = cf[1] * signe, cj = cf[2] * signe, ck = cf[3] * signe;
+                if (!LOWER) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) x[e] = x[e] * cf[4];
+                }
+                // halo products of the previous pencils: while any lane still holds the pending
+                // pattern, the wave re-reads its values (bounded)
+                const bool uj = get_j && act, uk = get_k && act;
+                {
+                    auto pend = [&]() {
+                        bool p = false;
+#pragma unroll
+                        for (int e = 0; e < 5; e++)
+                            p = p | (uj & (__double_as_longlong(hj[hb][e]) == kLuPending)) |
+                                (uk & (__double_as_longlong(hk[hb][e]) == kLuPending));
+                        return p;
+                    };
+                    if (__builtin_amdgcn_ballot_w64(pend())) {
+                        int spins = 0;
+                        // the previous pencil is about to pass this step: wait until it has also
+                        // handed over the cell kLuStSlack steps ahead (it writes its cells in
+                        // order), so that the next steps find their values in place instead of
+                        // waiting one round trip each
+                        {
+                            const int iia = min(ii + kLuStSlack, ni - 1);
+                            const double* pa = uj ? hjp + 4 * ni + iia : hkp + 4 * ni + iia;
+                            const bool wa = uj || uk;
+                            double va = wa ? lu_ldc(pa) : 0.;
+                            while (__builtin_amdgcn_ballot_w64(wa && __double_as_longlong(va) == kLuPending)) {
+                                if (++spins > kLuSpinMax ||
+                                    ((spins & 63) == 1 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                                    break;
+                                __builtin_amdgcn_s_sleep(2);
+                                if (wa) va = lu_ldc(pa);
+                            }
+                            hload(s + 1, hb ^ 1);
+                        }
+                        do {
+                            if (++spins > kLuSpinMax ||
+                                ((spins & 63) == 1 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                                if (L == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                            hload(s, hb);
+                            // every access of the wave done: the steps after this rare path keep
+                            // exact memory counters
+                            __builtin_amdgcn_s_waitcnt(0);
+                        } while (__builtin_amdgcn_ballot_w64(pend()));
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 5; e++) {
+                    bj[e] = jj == 0 ? hj[hb][e] : bj[e];
+                    bk[e] = kk == 0 ? hk[hb][e] : bk[e];
+                }
+                const bool hi = ii > 0, hjn = oj > 0, hkn = ok > 0;
+                // neighbour terms in the reference order: lower J, K, I; upper K, J, I
+#pragma unroll
+                for (int qq = 0; qq < 3; qq++) {
+                    const int D = LOWER ? (qq == 0 ? 1 : qq == 1 ? 2 : 0) : (qq == 0 ? 2 : qq == 1 ? 1 : 0);
+                    const bool has = D == 0 ? hi : D == 1 ? hjn : hkn;
+#pragma unroll
+                    for (int e = 0; e < 5; e++) {
+                        const double bb = D == 0 ? pi[e] : D == 1 ? bj[e] : bk[e];
+                        const double xn = x[e] + bb * xal;
+                        x[e] = has ? xn : x[e];
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 5; e++) x[e] = x[e] * diag;
+                const double uu = wv[1], vv = wv[2], ww = wv[3], tt = wv[4];
+                // the i and j products (the j product moves to lane L + 1 by DPP)
+                double pk[5], pj[5];
+                lu_prod<true, 0, double>(c, uu, vv, ww, tt, ci, tci, 0., 0., x, pi);
+                lu_prod<true, 1, double>(c, uu, vv, ww, tt, cj, tcj, 0., 0., x, pj);
+#pragma unroll
+                for (int e = 0; e < 5; e++) bj[e] = lu_from_jm1(pj[e]);
+                double o[5] = {0., 0., 0., 0., 0.};
+                if (newton) {
+                    // mjr_newton.for: the Newton-updated state; in place (kalias) the k products
+                    // see it
+                    const double r0 = wv[0];
+                    const double r1 = r0 + x[0];
+                    const double r_1 = 1. / r1;
+                    const double u1 = (r0 * uu + x[1]) * r_1, v1 = (r0 * vv + x[2]) * r_1, w1 = (r0 * ww + x[3]) * r_1;
+                    const double roe_old = r0 * (z.cv * tt + 0.5 * (uu * uu + vv * vv + ww * ww));
+                    const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
+                    o[0] = r1; o[1] = u1; o[2] = v1; o[3] = w1; o[4] = t1;
+                }
+                // the k product (moves to lane L + 8 by ds_bpermute)
+                if (newton && kalias)
+                    lu_prod<true, 2, double>(c, o[1], o[2], o[3], o[4], ck, tck, 0., 0., x, pk);
+                else
+                    lu_prod<true, 2, double>(c, uu, vv, ww, tt, ck, tck, 0., 0., x, pk);
+#pragma unroll
+                for (int e = 0; e < 5; e++) bk[e] = lu_from_km1(pk[e], L);
+                if (act) {
+                    const long l = LOWER ? lb + ii : lb - ii;
+#pragma unroll
+                    for (int e = 0; e < 5; e++) dw[l + e * nd] = x[e];
+                    if (newton) {
+                        double* __restrict__ O = a.out;
+#pragma unroll
+                        for (int e = 0; e < 5; e++) O[l + e * nd] = o[e];
+                    }
+                }
+                // the face products for the next pencils, and the consumed halo slots back to
+                // pending for the next sweep
+                if (put_j && act) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(0, tileid, kk, e, ii, ntile, ni), pj[e]);
+                }
+                if (put_k && act) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(1, tileid, jj, e, ii, ntile, ni), pk[e]);
+                }
+                if (uj) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(hjp + e * ni + ii, pending);
+                }
+                if (uk) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(hkp + e * ni + ii, pending);
+                }
+                hload(s + 2, hb);
+                // both cells of the pair read: refill it with the cells four steps ahead
+                if ((q & 1) && win + 1 < nwin) dma(win + 1, q >> 1);
+            }
+        }
+    }
+}
+
 // fill n doubles with a 64-bit pattern and zero the tile counter
 __global__ void k_fill_lin(double* __restrict__ p, long n, long long bits, int* sync) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -801,6 +1099,25 @@ void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
 
+// Streaming sweeps of a Cartesian zone: the tile counter reset by a one-thread fill, then one
+// launch per sweep with as many workgroups as can be resident at once (at most one per pencil).
+void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
+    int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    (void)ni;
+    int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    int cus = 0, dev = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_lu_stream<true>, 64, 0) != hipSuccess || per < 1)
+        per = 1;
+    long cap = (long)per * cus;
+    unsigned grid = (unsigned)((long)ntj * ntk < cap ? (long)ntj * ntk : cap);
+    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
+    hipLaunchKernelGGL((k_lu_stream<true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
+    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
+    hipLaunchKernelGGL((k_lu_stream<false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
+}
+
 template <bool CART>
 void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
@@ -814,6 +1131,11 @@ void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k_lu_pipe<CART, false, CART ? 4 : 2>), dim3(ntj * ntk), dim3(64), 0, st, z, a, a.dws, a.dw, ntj, ntk);
 }
 }  // namespace
+
+void fill_stream_exchange(double* x, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, kLuPending, (int*)nullptr);
+}
+
 
 // Block lists of a set of zones of one mesh kind for the diagonal-batched sweeps: for every zone
 // (LU box of boxes[n]) its CI x 8 x 8 blocks in sweep coordinates, grouped by diagonal;
@@ -876,7 +1198,13 @@ long lu_flow_exchange_size(bool cart, int ni, int nj, int nk) {
 
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
-    if (a.schedule == LU_DATAFLOW) {
+    if (a.schedule == LU_STREAM && z.typ == 2 && !a.fp32) {
+        // a pair of cells may straddle each end of the box in i by one cell
+        if (a.i0 - 1 >= 1 - z.nf && a.i1 + 1 <= z.ni + z.nf) stream(z, a, st);
+        else { sweep<true, true>(z, a, st); sweep<true, false>(z, a, st); }
+        return;
+    }
+    if (a.schedule == LU_DATAFLOW || a.schedule == LU_STREAM) {
         if (z.typ == 2) flow<true>(z, a, st); else flow<false>(z, a, st);
         return;
     }
@@ -890,7 +1218,7 @@ void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
 
 // error word of the pipelined sweeps (a bounded wait gave up); clears it
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st) {
-    if ((a.schedule != LU_PIPELINE && a.schedule != LU_DATAFLOW) || !a.sync) return 0;
+    if ((a.schedule != LU_PIPELINE && a.schedule != LU_DATAFLOW && a.schedule != LU_STREAM) || !a.sync) return 0;
     int h = 0;
     if (hipMemcpyAsync(&h, a.sync + 1, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;

@@ -191,6 +191,14 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    clears the word (fm_compute itself is asynchronous and returns FM_OK).  With several zones
    and FM_OPT_ZONE_BATCH on, the sweeps run as the batched blocks schedule. */
 #define FM_LU_DATAFLOW 2
+/* FM_LU_STREAM: Cartesian zones; one wave per pencil of 8 x 8 (j, k) lines sweeps the whole i
+   extent as a skewed wavefront (neighbour products move between lanes by DPP and ds_bpermute,
+   inputs reach LDS by direct loads) and hands its face products to the next pencils through a
+   global exchange (bounded waits, reported as above).  Fastest on LU boxes of a few pencils
+   (about twice the dataflow schedule on one pencil); slower than the dataflow schedule on large
+   cross-sections, where its line-by-line memory accesses thrash the caches.  Curvilinear zones
+   and FP32 sweeps run the dataflow schedule. */
+#define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
    FM_RES_KCOL (default) walks a column of 4 cells up k per thread, evaluating the k faces once. */

@@ -100,17 +100,24 @@ def test_tgv_cart_modes_two_steps(ctxmod, orc, scheme, mode):
     assert_same(zo, zg, ghosts=True)
 
 
-@pytest.mark.parametrize("schedule", ["blocks", "pipeline", "dataflow"])
-@pytest.mark.parametrize("mesh", ["cart", "full", "wall"])
+@pytest.mark.parametrize("schedule", ["blocks", "pipeline", "dataflow", "stream"])
+@pytest.mark.parametrize("mesh", ["cart", "full", "wall", "cartwall", "cartbig"])
 @pytest.mark.parametrize("mode", ["bdf", "bdf_ss2"])
 def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
     """The LU-SGS schedules (one launch per block diagonal; persistent trailing tiles; blocks
     handed out in diagonal order with a face exchange) on zones of several blocks in i, j and k,
     in-place sub-iterations included."""
-    sch = {"blocks": ctxmod.LU_BLOCKS, "pipeline": ctxmod.LU_PIPELINE, "dataflow": ctxmod.LU_DATAFLOW}[schedule]
+    sch = {"blocks": ctxmod.LU_BLOCKS, "pipeline": ctxmod.LU_PIPELINE, "dataflow": ctxmod.LU_DATAFLOW,
+           "stream": ctxmod.LU_STREAM}[schedule]
     nsub = 2 if mode == "bdf_ss2" else 1
     if mesh == "cart":
         mk = lambda: cases.tgv_zone((21, 19, 26), H.numerics("roe_kap", mode), perturb=0.02)
+    elif mesh == "cartwall":
+        # Cartesian with walls: the LU box extends one ghost layer at the walls
+        mk = lambda: cases.tgv_zone((19, 23, 17), H.numerics("ausmpred", mode), perturb=0.02, bcs=H.WALL_J)
+    elif mesh == "cartbig":
+        # many pencils, ragged in every direction
+        mk = lambda: cases.tgv_zone((45, 37, 29), H.numerics("ausmpred", mode), perturb=0.02)
     elif mesh == "full":
         mk = lambda: H.jittered_box((11, 18, 17), H.numerics("ausmpred", mode))
     else:
