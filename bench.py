@@ -5,8 +5,9 @@ coefficients, fused inviscid (AUSM+ o3) + viscous face fluxes with the BDF resid
 lower and upper sweeps with the Newton update, and the ghost-cell fill (periodic faces and
 corners).  Residual-only verification sub-iterations are not part of a step.
 
-Prints one JSON line.  Timing: W untimed steps, then exactly K steps between two device
-synchronisations (wall clock), cross-checked with HIP events on the solver stream.
+Prints one JSON line.  Timing: W untimed steps, then R repeats of exactly K steps, each between
+two device synchronisations (wall clock) and cross-checked with HIP events on the solver stream;
+the line reports the median repeat (value, ms_per_step) and the spread of the R repeats.
 """
 import argparse
 import json
@@ -45,7 +46,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--scheme", default="ausmpred")
+    ap.add_argument("--repeats", type=int, default=5, help="timed repeats of K steps (median reported)")
     ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="processes of the parallel CPU leg (0: the host's cores, at most 16)")
+    ap.add_argument("--no-fasts-loop", action="store_true",
+                    help="skip the FastS-style PyTree._compute loop timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal timing")
     ap.add_argument("--phase-reps", type=int, default=5)
@@ -102,32 +108,97 @@ def traffic_from_profiles(phase):
         return None
 
 
-def cpu_baseline(n, scheme, iters):
-    """The reference FastS Fortran kernels (compiled from source into oracle/_ref) on the host,
-    one thread, on the same workload; falls back to the oracle's C restatement if the
-    reference library is not present."""
+def _reference_backend():
     import oracle
-    from fast_amd import params as P
-    kind = "reference"
     if oracle.Reference.available():
-        be = oracle.Reference()
-    else:
-        be = oracle.Oracle()
-        kind = "port"
-    z = make_zone(n, scheme)
+        return oracle.Reference(), "reference"
+    return oracle.Oracle(), "port"
+
+
+def _cpu_run(dims, scheme, iters, barrier=None):
+    """Seconds for `iters` implicit iterations of the TGV case of size dims on one host thread
+    (and the wall-clock start and end of the timed part)."""
+    from fast_amd import cases, params as P
+    be, kind = _reference_backend()
+    num = P.Numerics(scheme=scheme, slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="global", time_step=1.0)
+    z = cases.tgv_zone(dims, num)
+    z.param_real[P.DTC] = cases.tgv_dt(z, cfl=2.0)
     be.init_metric(z)
     be.apply_bc(z, z.ro)
     z.ro_m1[:] = z.ro
     z.ro_p1[:] = z.ro
+    if barrier is not None:
+        barrier.wait()
+    w0 = time.time()
     t0 = time.perf_counter()
     for s in range(iters):
         be.substep(z, 1, first_it=0 if s == 0 else 1)
         m1 = z.ro_m1.copy(); z.ro_m1[:] = z.ro; z.ro[:] = z.ro_p1; z.ro_p1[:] = m1
-    dt = time.perf_counter() - t0
-    ncell = n ** 3
-    return {"value": ncell * iters / dt / 1e6, "unit": "Mcell-iter/s", "cores": 1, "kind": kind,
-            "sample": "%d implicit iterations of the same TGV %d^3 %s case, single thread, %s"
-                      % (iters, n, scheme, platform.processor() or platform.machine())}
+    return time.perf_counter() - t0, kind, w0, time.time()
+
+
+_BARRIER = None
+
+
+def _cpu_init(barrier):
+    global _BARRIER
+    _BARRIER = barrier
+
+
+def _cpu_slab(args):
+    dims, scheme, iters = args
+    return _cpu_run(dims, scheme, iters, _BARRIER)
+
+
+def _host_description():
+    """Core count and model of the host the baseline ran on."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "processor": platform.processor() or platform.machine()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    return info
+
+
+def cpu_baseline(n, scheme, iters, procs):
+    """The reference FastS Fortran kernels (compiled from source into oracle/_ref) on the host
+    (the oracle's C restatement if the reference library is not present), on the same workload:
+    one thread on the whole 256^3 zone, and `procs` processes at once, each one thread on a slab
+    of n / procs planes in k.  The reference parallelises one zone over OpenMP threads that each
+    run the kernels on their own sub-box (block-Jacobi LU-SGS across the sub-boxes); independent
+    slabs run the same kernels with no synchronisation between them, so the parallel leg bounds
+    the OpenMP reference's throughput from above."""
+    import multiprocessing as mp
+    host = _host_description()
+    dt, kind, _, _ = _cpu_run(n, scheme, iters)
+    out = {"value": round(n ** 3 * iters / dt / 1e6, 3), "unit": "Mcell-iter/s", "cores": 1, "kind": kind,
+           "sample": "%d implicit iterations of the same TGV %d^3 %s case, one thread" % (iters, n, scheme),
+           "host": host}
+    if procs <= 0:
+        procs = min(16, host["affinity"])
+    if procs > 1:
+        nk = max(4, n // procs)
+        mpc = mp.get_context("spawn")
+        barrier = mpc.Barrier(procs)
+        with mpc.Pool(procs, initializer=_cpu_init, initargs=(barrier,)) as pool:
+            res = pool.map(_cpu_slab, [((n, n, nk), scheme, iters)] * procs, chunksize=1)
+        # all slabs start together (barrier); the leg lasts until the last one ends
+        wall = max(r[3] for r in res) - min(r[2] for r in res)
+        cells = n * n * nk * procs
+        out["parallel"] = {
+            "value": round(cells * iters / wall / 1e6, 3), "unit": "Mcell-iter/s", "cores": procs,
+            "kind": res[0][1],
+            "sample": "%d processes x 1 thread, each %d implicit iterations on a %dx%dx%d slab of the "
+                      "TGV case (independent slabs: block-Jacobi LU-SGS across them, no "
+                      "synchronisation; an upper bound of the OpenMP reference)" % (procs, iters, n, n, nk),
+            "slowest_process_s": round(max(r[0] for r in res), 3)}
+    return out
 
 
 # algorithmic bytes per interior cell for the other workloads (3dfull adds the 80 B of face
@@ -156,6 +227,42 @@ def adjoint_ratio(n, scheme, reps):
             "residual_ms": round(res, 4), "vjp_ms": round(vjp, 4), "ratio": round(vjp / res, 3),
             "vjp_gbs_algorithmic": round(480.0 * n ** 3 / (vjp * 1e-3) / 1e9, 1),
             "vjp_traffic": traffic_from_profiles("adjoint/vjp")}
+
+
+def fasts_loop(a, resident_ms):
+    """The same workload driven the way a FastS script drives it: a tree, warmup, then
+    `for it: _compute(t, metrics, it)` (Fast/FastS/FastS/PyTree.py _compute), one time step per
+    call, the state kept on the GPU between calls (no verification steps, as in the resident
+    loop).  Returns its ms per step and the ratio to the resident fm_compute loop."""
+    from fast_amd import Internal, PyTree as FP, params as P
+    z = make_zone(a.n, a.scheme)
+    num = P.Numerics(scheme=a.scheme, slope="o3", temporal_scheme="implicit", ss_iteration=1,
+                     time_step_nature="global", time_step=float(z.param_real[P.DTC]), modulo_verif=0)
+    t = Internal.newCGNSTree()
+    b = Internal.newCGNSBase("Base", parent=t)
+    FP.newZoneFromHost(z, b)
+    FP._setNum2Zones(t, dict(num))
+    FP._setNum2Base(t, {"modulo_verif": 0})
+    FP._addState(t, z.state, model="NSLaminar")
+    del z
+    t, tc, metrics = FP.warmup(t, None)
+    ctx = FP.HOOK["ctx"]
+    for it in range(a.warmup):
+        FP._compute(t, metrics, it)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for it in range(a.warmup, a.warmup + a.steps):
+        FP._compute(t, metrics, it)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) / a.steps * 1e3
+    dens = Internal.getNodeFromName(t, "Density")[1]   # reading the tree pulls the state once
+    ok = bool(np.isfinite(dens).all())
+    ctx.close()
+    FP.HOOK.clear()
+    return {"ms_per_step": round(ms, 4), "resident_ms_per_step": round(resident_ms, 4),
+            "ratio": round(ms / resident_ms, 4), "state_finite": ok,
+            "loop": "for it: PyTree._compute(t, metrics, it), %d steps after %d warm-up calls"
+                    % (a.steps, a.warmup)}
 
 
 def zones64_case():
@@ -244,6 +351,9 @@ def main():
     if a.workload != "tgv256":
         run_other(a)
         return
+    # the host baseline first, before this process touches the GPU (its parallel leg starts
+    # worker processes)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.n, a.scheme, a.cpu_iters, a.cpu_procs)
     from fast_amd import _lib
     z = make_zone(a.n, a.scheme)
     ctx = _lib.Context(0)
@@ -269,11 +379,17 @@ def main():
     # warm-up (first step BDF1, then BDF2)
     ctx.compute(0, 1, 1, nit_c=a.warmup, modulo_verif=0, first_it=0)
     ctx.sync()
-    t0 = time.perf_counter()
-    ms_ev = ctx.compute_timed(a.warmup, 1, 1, nit_c=a.steps, modulo_verif=0, first_it=1)
-    ctx.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    walls, evs = [], []
+    it = a.warmup
+    for r in range(max(1, a.repeats)):
+        t0 = time.perf_counter()
+        evs.append(ctx.compute_timed(it, 1, 1, nit_c=a.steps, modulo_verif=0, first_it=1))
+        ctx.sync()
+        walls.append(time.perf_counter() - t0)
+        it += a.steps
+    order = np.argsort(walls)
+    med = int(order[len(walls) // 2])
+    wall, ms_ev = walls[med], evs[med]
     value = ncell * a.steps / wall / 1e6
     # health check of the state after the timed steps
     tke = ctx.tke(zid, 0)
@@ -286,6 +402,7 @@ def main():
         phases[name] = ctx.time_phase(zid, ph, reps=a.phase_reps)
     ctx.close()
     step_ms = wall / a.steps * 1e3
+    rep_ms = [w / a.steps * 1e3 for w in walls]
     dom = max(("residual", "lu"), key=lambda k: phases[k])
     dom_ms = phases[dom]
     achieved = BYTES_PER_CELL[dom] * ncell / (dom_ms * 1e-3) / 1e9
@@ -312,11 +429,18 @@ def main():
            "iteration_hbm_frac": round(it_gbs / HBM_PEAK_GBS, 4),
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "event_ms_per_step": round(ms_ev / a.steps, 3),
+           "protocol": {"warmup_steps": a.warmup, "repeats": len(walls), "steps_per_repeat": a.steps,
+                        "ms_per_step_repeats": [round(v, 4) for v in rep_ms],
+                        "ms_per_step_median": round(float(np.median(rep_ms)), 4),
+                        "ms_per_step_stdev": round(float(np.std(rep_ms, ddof=1)) if len(rep_ms) > 1 else 0.0, 4),
+                        "event_ms_per_step_repeats": [round(v / a.steps, 4) for v in evs]},
            "roofline": roofline}
+    if not a.no_fasts_loop:
+        out["fasts_loop"] = fasts_loop(a, step_ms)
     if not a.no_adjoint and a.scheme == "ausmpred":
         out["adjoint"] = adjoint_ratio(a.n, a.scheme, a.phase_reps)
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.n, a.scheme, a.cpu_iters)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 
