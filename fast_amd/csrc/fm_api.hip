@@ -515,7 +515,7 @@ static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
         long n = lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMalloc(&z->lu_stx, sizeof(double) * n));
-        fill_stream_exchange(z->lu_stx, n, c->stream);
+        fill_pending(z->lu_stx, n, c->stream);
         HIPCHK(hipGetLastError());
     }
     la.stx = z->lu_stx;
@@ -531,6 +531,9 @@ static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
         long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMalloc(&z->lu_fx, sizeof(double) * n));
+        // pending once: the sweeps give back every slot they read
+        fill_pending(z->lu_fx, n, c->stream);
+        HIPCHK(hipGetLastError());
     }
     const Box& f = z->flow_box;
     if (!z->flow_work || f.i0 != b.i0 || f.i1 != b.i1 || f.j0 != b.j0 || f.j1 != b.j1 || f.k0 != b.k0 || f.k1 != b.k1) {
@@ -546,6 +549,11 @@ static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
         z->flow_box = b;
     }
     la.fx = z->lu_fx; la.flow_work = z->flow_work; la.flow_n = z->flow_n;
+    // a box a few blocks wide in j and k is latency-bound: its blocks start before their halo is
+    // complete and take the products step by step; wider boxes keep every resident workgroup busy
+    // with blocks whose halo is complete (measured on 256 x {8..256}^2 boxes)
+    const long nbjk = (long)((b.j1 - b.j0 + 1 + 7) / 8) * ((b.k1 - b.k0 + 1 + 7) / 8);
+    la.flow_step = nbjk < 256;
     return FM_OK;
 }
 
@@ -672,7 +680,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
             if (int rc = ensure_exchange(c, z)) return rc;
             la.xch = z->lu_xch;
-            la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0;
+            la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
             {
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
                 if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
@@ -711,12 +719,13 @@ static int check_lu(fm_ctx* c) {
         int f = lu_pipeline_failed(la, c->stream);
         if (f < 0) return fail(FM_E_HIP, "reading the LU error word failed");
         if (f > 0) {
-            // a sweep that gave up may leave handed-over values in the streaming exchange
-            if (z->lu_stx) {
-                const KZ& k = z->k;
-                fill_stream_exchange(z->lu_stx, lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2), c->stream);
-                (void)hipStreamSynchronize(c->stream);
-            }
+            // a sweep that gave up may leave handed-over values in the exchanges
+            const KZ& k = z->k;
+            if (z->lu_stx)
+                fill_pending(z->lu_stx, lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2), c->stream);
+            if (z->lu_fx)
+                fill_pending(z->lu_fx, lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2), c->stream);
+            (void)hipStreamSynchronize(c->stream);
             return fail(FM_E_HIP, "LU-SGS sweep timed out waiting for a neighbour tile or block");
         }
     }
@@ -1353,7 +1362,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
-    la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0;
+    la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;

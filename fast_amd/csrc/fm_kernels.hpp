@@ -56,14 +56,16 @@ struct LuArgs {
     const LuWork* flow_work;
     int flow_n;
     double* fx;
+    int flow_step;                // dataflow: blocks take their halo products step by step (FP64)
     // streaming schedule: the pencil-to-pencil exchange (pending pattern when idle)
     double* stx;
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
-// doubles of the streaming exchange of an LU box of ni x nj x nk cells, and its fill with the
-// pending pattern (once when allocated, and after a wait gave up)
+// doubles of the streaming exchange of an LU box of ni x nj x nk cells
 long lu_stream_exchange_size(int ni, int nj, int nk);
-void fill_stream_exchange(double* x, long n, hipStream_t st);
+// fill n doubles of an exchange with the pending pattern (once when allocated, and after a wait
+// gave up: the sweeps give back every slot they read)
+void fill_pending(double* x, long n, hipStream_t st);
 // doubles of the dataflow face exchange of an LU box of ni x nj x nk cells
 long lu_flow_exchange_size(bool cart, int ni, int nj, int nk);
 

@@ -269,7 +269,12 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         }
     }
     // 1b. products of the halo cells (finished by earlier launches; dataflow: handed over by the
-    //     neighbour blocks through the face exchange, read until they are no longer pending)
+    //     neighbour blocks through the face exchange).  With a.flow_step (FP64 sweeps) the block
+    //     fetches what is there and the solver lane waits for a value still pending at the step
+    //     that uses it, so a block starts while the blocks before it are still running; otherwise
+    //     the block waits here for its whole halo.  Every slot read is stored back as pending (by
+    //     the solver lane at its step, or here), so the exchange is ready for the next sweep
+    //     without a fill.
     for (int e = tid; e < 64 + 16 * CI; e += 256) {
         T b[5];
         if (FLOW) {
@@ -290,7 +295,11 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             }
 #pragma unroll
             for (int v2 = 0; v2 < 5; v2++) v[v2] = lu_ldc(a.fx + base + (long)v2 * LuFace<CI>::N);
-            lu_settle(v, 5, a.fx + base, LuFace<CI>::N, a.sync + 1);
+            if (!F64 || !a.flow_step) {
+                lu_settle(v, 5, a.fx + base, LuFace<CI>::N, a.sync + 1);
+#pragma unroll
+                for (int v2 = 0; v2 < 5; v2++) lu_stc(a.fx + base + (long)v2 * LuFace<CI>::N, __longlong_as_double(kLuPending));
+            }
             if (e < 64) {
 #pragma unroll
                 for (int v2 = 0; v2 < 5; v2++) S.hI[v2][e] = (T)v[v2];
@@ -396,6 +405,26 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
                 } else {
 #pragma unroll
                     for (int e = 0; e < 5; e++) b[e] = kk == 0 ? S.hK[e][ii][jj] : S.px[2][e][L - 8];
+                }
+                if (FLOW && F64 && a.flow_step && (D == 0 ? ii == 0 : D == 1 ? jj == 0 : kk == 0)) {
+                    // a product of the previous block in D: wait for it if it was still pending
+                    // when the block was staged, then give the slot back
+                    const long nb = D == 0 ? bi - 1 + (long)nbi * (bj + (long)nbj * bk)
+                                  : D == 1 ? bi + (long)nbi * (bj - 1 + (long)nbj * bk)
+                                           : bi + (long)nbi * (bj + (long)nbj * (bk - 1));
+                    const long base = LuFace<CI>::at(nb, 0, D, D == 0 ? L : D == 1 ? ii * 8 + kk : ii * 8 + jj);
+                    double v[5];
+#pragma unroll
+                    for (int e = 0; e < 5; e++) v[e] = (double)b[e];
+                    if (lu_any_pending(v, 5)) {
+#pragma unroll
+                        for (int e = 0; e < 5; e++) v[e] = lu_ldc(a.fx + base + (long)e * LuFace<CI>::N);
+                        lu_settle(v, 5, a.fx + base, LuFace<CI>::N, a.sync + 1);
+#pragma unroll
+                        for (int e = 0; e < 5; e++) b[e] = (T)v[e];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 5; e++) lu_stc(a.fx + base + (long)e * LuFace<CI>::N, __longlong_as_double(kLuPending));
                 }
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
@@ -1087,10 +1116,13 @@ void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
     long cap = (long)per * cus;
     if (cap > FM_LU_FLOW_GRID) cap = FM_LU_FLOW_GRID;
     unsigned grid = (unsigned)(a.flow_n < cap ? a.flow_n : cap);
-    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
+    // the exchange is pending before a sweep (filled when allocated, and every slot a sweep writes
+    // is read and given back by its consumer): only the block counter is reset
+    (void)nx;
+    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.fx, 0L, kLuPending, a.sync);
     if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, true, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     else hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
-    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.fx, nx, kLuPending, a.sync);
+    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.fx, 0L, kLuPending, a.sync);
     if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, false, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     else hipLaunchKernelGGL((k_lu_flow<CART, false, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
 }
@@ -1132,7 +1164,7 @@ void pipeline(const KZ& z, const LuArgs& a, hipStream_t st) {
 }
 }  // namespace
 
-void fill_stream_exchange(double* x, long n, hipStream_t st) {
+void fill_pending(double* x, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, kLuPending, (int*)nullptr);
 }
 
