@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 #   coefficients (read W 40, write mu 8 + coe 40), counted separately     =  88
 BYTES_PER_CELL = {"coe": 88.0, "residual": 120.0, "lu": 240.0, "bc": 0.0, "iteration": 400.0}
 KERNELS = {"coe": "k_coe", "bc": "k_bc_face",
-           "residual": {"cells": "k_residual", "faces": "k_residual_faces", "kcol": "k_residual_kcol"},
+           "residual": {"cells": "k_residual", "faces": "k_residual_faces", "kcol": "k_residual_kcol",
+                        "auto": "k_residual (six faces per cell, the default on Cartesian zones)"},
            "lu": {"blocks": "k_lu_block (lower + upper sweeps, one launch per block diagonal)",
                   "pipeline": "k_lu_pipe (lower + upper sweeps)",
                   "dataflow": "k_lu_flow (lower + upper sweeps, one launch each)",
@@ -56,7 +57,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal timing")
     ap.add_argument("--phase-reps", type=int, default=5)
     ap.add_argument("--lu", choices=["pipeline", "blocks", "dataflow", "stream"], default="dataflow")
-    ap.add_argument("--residual", choices=["cells", "faces", "kcol"], default="kcol")
+    ap.add_argument("--residual", choices=["auto", "cells", "faces", "kcol"], default="auto")
     ap.add_argument("--lu-precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp32: mixed precision, LU-SGS sweep arithmetic in FP32 (state, residual FP64)")
     ap.add_argument("--workload", choices=["tgv256", "channel", "zones64"], default="tgv256",
@@ -360,7 +361,8 @@ def main():
     ctx.set_option(_lib.OPT_LU_SCHEDULE, {"pipeline": _lib.LU_PIPELINE, "blocks": _lib.LU_BLOCKS,
                                           "dataflow": _lib.LU_DATAFLOW, "stream": _lib.LU_STREAM}[a.lu])
     ctx.set_option(_lib.OPT_RESIDUAL_SCHEDULE,
-                   {"cells": _lib.RES_CELLS, "faces": _lib.RES_FACES, "kcol": _lib.RES_KCOL}[a.residual])
+                   {"cells": _lib.RES_CELLS, "faces": _lib.RES_FACES, "kcol": _lib.RES_KCOL,
+                    "auto": _lib.RES_AUTO}[a.residual])
     ctx.set_option(_lib.OPT_GRAPH, a.graph)
     ctx.set_option(_lib.OPT_LU_PRECISION, _lib.LU_FP32 if a.lu_precision == "fp32" else _lib.LU_FP64)
     zid = ctx.add_zone(z)
@@ -409,7 +411,8 @@ def main():
     kname = KERNELS[dom] if isinstance(KERNELS[dom], str) else KERNELS[dom][a.lu if dom == "lu" else a.residual]
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(dom + "/" + (a.lu if dom == "lu" else a.residual)),
+                "traffic": traffic_from_profiles(dom + "/" + (a.lu if dom == "lu" else
+                                                              "cells" if a.residual == "auto" else a.residual)),
                 "bytes_per_cell": BYTES_PER_CELL[dom],
                 "launch_ms": round(dom_ms, 4),
                 "phase_gbs": {k: round(BYTES_PER_CELL[k] * ncell / (v * 1e-3) / 1e9, 1)

@@ -55,11 +55,12 @@ SIGNATURES = {
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC, PHASE_VJP = range(5)
 OPT_LU_PRECISION = 5
+OPT_RES_KCOL_N, OPT_LU_FLOW_GRID = 7, 8
 LU_FP64, LU_FP32 = 0, 1
 OPT_LU_SCHEDULE = 1
 LU_BLOCKS, LU_PIPELINE, LU_DATAFLOW, LU_STREAM = 0, 1, 2, 3
 OPT_RESIDUAL_SCHEDULE = 2
-RES_CELLS, RES_FACES, RES_KCOL = 0, 1, 2
+RES_CELLS, RES_FACES, RES_KCOL, RES_AUTO = 0, 1, 2, 3
 OPT_ZONE_BATCH = 3
 OPT_GRAPH = 4
 OPT_LU_GIVE_UP = 6

@@ -68,7 +68,7 @@ struct Zone {
 struct fm_ctx {
     int device;
     int lu_schedule;         // LU_PIPELINE or LU_BLOCKS
-    int res_schedule;        // RES_CELLS, RES_FACES or RES_KCOL
+    int res_schedule;        // RES_CELLS, RES_FACES, RES_KCOL or RES_AUTO (per zone)
     hipStream_t stream;
     std::vector<Zone*> zones;
     hipEvent_t e0, e1;
@@ -83,6 +83,8 @@ struct fm_ctx {
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
     int lu_batch;            // batch the LU of several zones (default on)
     int lu_fp32;             // mixed precision: LU-SGS sweep arithmetic in FP32 (default off)
+    int res_kcol_n;          // tile shapes: k-column length of RES_KCOL (0: built-in)
+    int lu_flow_grid;        //   workgroups of a dataflow sweep (0: all that are resident)
     // time steps captured as HIP graphs (FM_OPT_GRAPH): one per distinct configuration
     struct GraphEntry {
         std::vector<const void*> ptrs;  // every zone's three level pointers
@@ -128,9 +130,11 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     fm_ctx* c = new fm_ctx();
     c->device = device;
     c->lu_schedule = LU_DATAFLOW;
-    c->res_schedule = RES_KCOL;
+    c->res_schedule = RES_AUTO;
     c->lu_batch = 1;
     c->lu_fp32 = 0;
+    c->res_kcol_n = 0;
+    c->lu_flow_grid = 0;
     if (int rc = resolve_scratch(device)) { delete c; return rc; }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
@@ -501,6 +505,14 @@ static int ensure_exchange(fm_ctx* c, Zone* z) {
 
 // Dataflow LU schedule: the face exchange (sized for the largest LU box, interior plus a wall
 // layer per side) and the block order of the LU box b (rebuilt when the box changes).
+// The residual schedule of a zone: the context's, or with RES_AUTO six faces per cell on
+// Cartesian zones and columns up k on curvilinear ones (the faster of the two on each mesh kind,
+// measured on TGV 256^3 and the 257 x 129 x 33 channel)
+static int res_schedule_of(const fm_ctx* c, const Zone* z) {
+    if (c->res_schedule != RES_AUTO) return c->res_schedule;
+    return z->k.typ == 2 ? RES_CELLS : RES_KCOL;
+}
+
 // Streaming LU schedule of a Cartesian zone: the pencil exchange, sized for the largest LU box and
 // filled with the pending pattern once (the sweeps leave it that way).
 static bool uses_stream(const fm_ctx* c, const Zone* z) {
@@ -643,7 +655,8 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
     if (k.kflu == 2 && nitcfg == 1) launch_wiggle(k, ro_ssiter, st);
     RhsArgs a;
     memset(&a, 0, sizeof a);
-    a.schedule = c->res_schedule;
+    a.schedule = res_schedule_of(c, z);
+    a.kcol_n = c->res_kcol_n;
     if (k.kflu == 5) {
         int rc = wall_corrections(z, a);
         if (rc) return rc;
@@ -678,6 +691,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             la.k0 = shift_lu[4]; la.k1 = shift_lu[5];
             la.dw = k.drodm; la.newton = 1;
             la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
+    la.flow_grid = c->lu_flow_grid;
             if (int rc = ensure_exchange(c, z)) return rc;
             la.xch = z->lu_xch;
             la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
@@ -894,7 +908,8 @@ extern "C" int fm_flux_balance(fm_ctx* c, int zone, int level, double* drodm) {
     launch_coe(k, rop, 0, 0, b, k.iflow == 2, st);
     if (k.kflu == 2) launch_wiggle(k, rop, st);
     RhsArgs a;
-    if (int rc = balance_args(z, rop, a, c->res_schedule)) return rc;
+    if (int rc = balance_args(z, rop, a, res_schedule_of(c, z))) return rc;
+    a.kcol_n = c->res_kcol_n;
     HIPCHK(hipMemsetAsync(k.drodm, 0, sizeof(double) * 5 * (size_t)k.nd, st));
     if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
     HIPCHK(hipGetLastError());
@@ -1242,9 +1257,10 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
         return FM_OK;
     }
     if (option == FM_OPT_RESIDUAL_SCHEDULE) {
-        if (value != FM_RES_CELLS && value != FM_RES_FACES && value != FM_RES_KCOL)
+        if (value != FM_RES_CELLS && value != FM_RES_FACES && value != FM_RES_KCOL && value != FM_RES_AUTO)
             return fail(FM_E_ARG, "residual schedule %d", value);
-        c->res_schedule = value == FM_RES_FACES ? RES_FACES : value == FM_RES_KCOL ? RES_KCOL : RES_CELLS;
+        c->res_schedule = value == FM_RES_FACES ? RES_FACES : value == FM_RES_KCOL ? RES_KCOL
+                        : value == FM_RES_AUTO ? RES_AUTO : RES_CELLS;
         c->graph_epoch++;
         return FM_OK;
     }
@@ -1270,6 +1286,18 @@ extern "C" int fm_ctx_set_option(fm_ctx* c, int option, int value) {
         int one = 1;
         for (Zone* z : c->zones)
             if (z->lu_sync) HIPCHK(hipMemcpy(z->lu_sync + 1, &one, sizeof(int), hipMemcpyHostToDevice));
+        return FM_OK;
+    }
+    if (option == FM_OPT_RES_KCOL_N) {
+        if (value < 0 || value > 64) return fail(FM_E_ARG, "k-column length %d (0..64)", value);
+        c->res_kcol_n = value;
+        c->graph_epoch++;
+        return FM_OK;
+    }
+    if (option == FM_OPT_LU_FLOW_GRID) {
+        if (value < 0) return fail(FM_E_ARG, "dataflow grid %d", value);
+        c->lu_flow_grid = value;
+        c->graph_epoch++;
         return FM_OK;
     }
     if (option == FM_OPT_GRAPH) {
@@ -1348,7 +1376,8 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     double* ro = z->ro[0];
     RhsArgs a;
     memset(&a, 0, sizeof a);
-    a.schedule = c->res_schedule;
+    a.schedule = res_schedule_of(c, z);
+    a.kcol_n = c->res_kcol_n;
     if (k.kflu == 5) {
         int rc = wall_corrections(z, a);
         if (rc) return rc;
@@ -1360,6 +1389,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.i0 = 1; la.i1 = k.ni; la.j0 = 1; la.j1 = k.nj; la.k0 = 1; la.k1 = k.nk;
     la.dw = k.drodm; la.rop = ro; la.rop_k = ro; la.out = z->ro_scratch; la.newton = 1;
     la.dws = z->lu_dws; la.sync = z->lu_sync; la.schedule = c->lu_schedule; la.fp32 = c->lu_fp32;
+    la.flow_grid = c->lu_flow_grid;
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
     la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;

@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES
     res_update(z, a, rop, l, acc);
 }
 
-// Residual, a column of RES_KCOL_N cells up k per thread: the k face between two cells of the
+// Residual, a column of a.kcol_n cells up k per thread: the k face between two cells of the
 // column is evaluated once (as the top face of the lower cell and the bottom face of the upper
 // one), and the planes the k stencils share stay in the cache of the block.  Per cell the sum
 // order is the reference's, so the result is the one of k_residual.
@@ -408,14 +408,15 @@ template <int KFLU, int SLOPE, bool LAMIN, bool CART>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FM_RES_WAVES))) k_residual_kcol(KZ z, RhsArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    int k0 = blockIdx.z * RES_KCOL_N + 1;
+    const int kn_col = a.kcol_n > 0 ? a.kcol_n : RES_KCOL_N;
+    int k0 = blockIdx.z * kn_col + 1;
     if (i > z.ni || j > z.nj || k0 > z.nk) return;
     const double* __restrict__ rop = a.rop;
     long nd = z.nd;
     FluxConst f = flux_const(z);
     double fk[5], flu[5], acc[5];
     face_flux<KFLU, SLOPE, LAMIN, CART, 2>(f, z, rop, i, j, k0, fk);
-    const int k1 = k0 + RES_KCOL_N - 1 < z.nk ? k0 + RES_KCOL_N - 1 : z.nk;
+    const int k1 = k0 + kn_col - 1 < z.nk ? k0 + kn_col - 1 : z.nk;
     for (int k = k0; k <= k1; k++) {
         long l = IDX(z, i, j, k);
         if (a.zero_init) {
@@ -1047,7 +1048,8 @@ static void res_lc(const KZ& z, const RhsArgs& a, bool lamin, dim3 g, hipStream_
 
 template <int KFLU, int SLOPE>
 static void res_kcol(const KZ& z, const RhsArgs& a, bool lamin, hipStream_t st) {
-    dim3 g((z.ni + 63) / 64, (z.nj + 3) / 4, (z.nk + RES_KCOL_N - 1) / RES_KCOL_N);
+    const int n = a.kcol_n > 0 ? a.kcol_n : RES_KCOL_N;
+    dim3 g((z.ni + 63) / 64, (z.nj + 3) / 4, (z.nk + n - 1) / n);
     if (lamin) {
         if (z.typ == 2) hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, true, true>), g, blk2, 0, st, z, a);
         else hipLaunchKernelGGL((k_residual_kcol<KFLU, SLOPE, true, false>), g, blk2, 0, st, z, a);

@@ -24,12 +24,13 @@ struct RhsArgs {
     int implicit, nitcfg;
     double c1, c2, c3;      // BDF coefficients
     int schedule;           // RES_CELLS (six faces per cell), RES_FACES (each face once) or
-                            // RES_KCOL (a thread walks RES_KCOL_N cells up k, k faces once)
+                            // RES_KCOL (a thread walks kcol_n cells up k, k faces once)
+    int kcol_n;             // RES_KCOL column length (0: RES_KCOL_N)
     int ncorr;              // Roe: wall faces whose Euler flux is replaced, in BC order
     WallCorr corr[kMaxWallCorr];
     double ck_vent, c_ale;  // shape_tab_mtr.for mesh-motion factors of the wall flux
 };
-enum { RES_CELLS = 0, RES_FACES = 1, RES_KCOL = 2 };
+enum { RES_CELLS = 0, RES_FACES = 1, RES_KCOL = 2, RES_AUTO = 3 };
 #ifndef FM_RES_KCOL_N
 #define FM_RES_KCOL_N 4
 #endif
@@ -57,6 +58,7 @@ struct LuArgs {
     int flow_n;
     double* fx;
     int flow_step;                // dataflow: blocks take their halo products step by step (FP64)
+    int flow_grid;                // dataflow: workgroups of a sweep (0: as many as are resident)
     // streaming schedule: the pencil-to-pencil exchange (pending pattern when idle)
     double* stx;
 };

@@ -1115,6 +1115,7 @@ void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, 256, 0) != hipSuccess || per < 1) per = 1;
     long cap = (long)per * cus;
     if (cap > FM_LU_FLOW_GRID) cap = FM_LU_FLOW_GRID;
+    if (a.flow_grid > 0 && a.flow_grid < cap) cap = a.flow_grid;
     unsigned grid = (unsigned)(a.flow_n < cap ? a.flow_n : cap);
     // the exchange is pending before a sweep (filled when allocated, and every slot a sweep writes
     // is read and given back by its consumer): only the block counter is reset

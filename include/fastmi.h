@@ -201,9 +201,12 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
-   FM_RES_KCOL (default) walks a column of 4 cells up k per thread, evaluating the k faces once. */
+   FM_RES_KCOL walks a column of 4 cells up k per thread, evaluating the k faces once;
+   FM_RES_AUTO (default) takes FM_RES_CELLS on Cartesian zones and FM_RES_KCOL on curvilinear
+   ones.  Same results bit for bit. */
 #define FM_OPT_RESIDUAL_SCHEDULE 2
 #define FM_RES_CELLS 0
+#define FM_RES_AUTO 3
 #define FM_RES_FACES 1
 #define FM_RES_KCOL 2
 /* FM_OPT_ZONE_BATCH: 1 (default) runs the LU-SGS sweeps of all zones together, one launch per
@@ -229,6 +232,11 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    waiting call reports FM_E_HIP).  FP32 sweeps and FM_LU_PIPELINE exclude each other
    (FM_E_UNSUPPORTED, in either order of setting). */
 #define FM_OPT_LU_GIVE_UP 6
+/* Tile shapes (same results bit for bit, only the speed changes; fast_amd.tune picks them):
+   FM_OPT_RES_KCOL_N: cells per thread column of FM_RES_KCOL (1..64, 0 = built-in 4);
+   FM_OPT_LU_FLOW_GRID: workgroups of a dataflow sweep (0 = as many as can be resident). */
+#define FM_OPT_RES_KCOL_N 7
+#define FM_OPT_LU_FLOW_GRID 8
 int fm_ctx_set_option(fm_ctx* ctx, int option, int value);
 
 #define FM_PHASE_COE 0

@@ -50,7 +50,8 @@ def test_autotune_picks_the_fastest_schedule():
                          (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_KCOL): 1.7,
                          (_lib.OPT_LU_SCHEDULE, _lib.LU_BLOCKS): 3.0,
                          (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0,
-                         (_lib.OPT_LU_SCHEDULE, _lib.LU_DATAFLOW): 3.5}
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_DATAFLOW): 3.5,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_STREAM): 3.2}
 
         def set_option(self, opt, v):
             self.opts[opt] = v
@@ -64,3 +65,38 @@ def test_autotune_picks_the_fastest_schedule():
     assert out["residual"][0] == "faces" and out["lu"][0] == "blocks"
     assert c.opts[_lib.OPT_RESIDUAL_SCHEDULE] == _lib.RES_FACES
     assert c.opts[_lib.OPT_LU_SCHEDULE] == _lib.LU_BLOCKS
+
+
+def test_autotune_shapes_the_schedule_it_kept():
+    """Tile shapes are tuned only for the schedule chosen: the k-column length when the column
+    residual wins, the dataflow grid when the dataflow LU-SGS wins."""
+    from fast_amd import tune, _lib
+
+    class FakeCtx:
+        def __init__(self):
+            self.opts = {_lib.OPT_RES_KCOL_N: 0, _lib.OPT_LU_FLOW_GRID: 0}
+            self.cost = {(_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_CELLS): 2.0,
+                         (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_FACES): 2.5,
+                         (_lib.OPT_RESIDUAL_SCHEDULE, _lib.RES_KCOL): 1.7,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_BLOCKS): 3.0,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_PIPELINE): 4.0,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_DATAFLOW): 2.5,
+                         (_lib.OPT_LU_SCHEDULE, _lib.LU_STREAM): 3.2}
+            self.calls = []
+
+        def set_option(self, opt, v):
+            self.opts[opt] = v
+
+        def time_phase(self, zone, phase, reps=1):
+            self.calls.append(phase)
+            if phase == _lib.PHASE_RESIDUAL:
+                base = self.cost[(_lib.OPT_RESIDUAL_SCHEDULE, self.opts[_lib.OPT_RESIDUAL_SCHEDULE])]
+                return base - {0: 0., 2: 0.1, 4: 0., 8: 0.2}[self.opts[_lib.OPT_RES_KCOL_N]]
+            base = self.cost[(_lib.OPT_LU_SCHEDULE, self.opts[_lib.OPT_LU_SCHEDULE])]
+            return base - {0: 0., 256: 0.3, 384: 0.1}[self.opts[_lib.OPT_LU_FLOW_GRID]]
+
+    c = FakeCtx()
+    out = tune.autotune(c, 0)
+    assert out["residual"][0] == "kcol" and out["kcol_n"][0] == "8"
+    assert out["lu"][0] == "dataflow" and out["flow_grid"][0] == "256"
+    assert c.opts[_lib.OPT_RES_KCOL_N] == 8 and c.opts[_lib.OPT_LU_FLOW_GRID] == 256

@@ -25,7 +25,8 @@ def orc():
     return oracle.Oracle()
 
 
-def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True, schedule=None):
+def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True, schedule=None,
+             options=()):
     zo = make()
     zg = make()
     orc.init_metric(zo)
@@ -34,6 +35,8 @@ def run_pair(ctxmod, orc, make, nsub, nsteps=1, first_it=0, stop=0, verif=True, 
     ctx = ctxmod.Context(0)
     if schedule is not None:
         ctx.set_option(ctxmod.OPT_LU_SCHEDULE, schedule)
+    for o, v in options:
+        ctx.set_option(o, v)
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     # same N-1 / N+1 initialisation as the oracle side
@@ -100,7 +103,7 @@ def test_tgv_cart_modes_two_steps(ctxmod, orc, scheme, mode):
     assert_same(zo, zg, ghosts=True)
 
 
-@pytest.mark.parametrize("schedule", ["blocks", "pipeline", "dataflow", "stream"])
+@pytest.mark.parametrize("schedule", ["blocks", "pipeline", "dataflow", "dataflow_g7", "stream"])
 @pytest.mark.parametrize("mesh", ["cart", "full", "wall", "cartwall", "cartbig"])
 @pytest.mark.parametrize("mode", ["bdf", "bdf_ss2"])
 def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
@@ -108,7 +111,9 @@ def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
     handed out in diagonal order with a face exchange) on zones of several blocks in i, j and k,
     in-place sub-iterations included."""
     sch = {"blocks": ctxmod.LU_BLOCKS, "pipeline": ctxmod.LU_PIPELINE, "dataflow": ctxmod.LU_DATAFLOW,
-           "stream": ctxmod.LU_STREAM}[schedule]
+           "dataflow_g7": ctxmod.LU_DATAFLOW, "stream": ctxmod.LU_STREAM}[schedule]
+    # dataflow_g7: the dataflow sweeps on 7 workgroups (FM_OPT_LU_FLOW_GRID)
+    opts = ((ctxmod.OPT_LU_FLOW_GRID, 7),) if schedule == "dataflow_g7" else ()
     nsub = 2 if mode == "bdf_ss2" else 1
     if mesh == "cart":
         mk = lambda: cases.tgv_zone((21, 19, 26), H.numerics("roe_kap", mode), perturb=0.02)
@@ -122,7 +127,7 @@ def test_lu_schedules(ctxmod, orc, schedule, mesh, mode):
         mk = lambda: H.jittered_box((11, 18, 17), H.numerics("ausmpred", mode))
     else:
         mk = lambda: H.jittered_box((10, 17, 20), H.numerics("senseur", mode), bcs=H.WALL_J)
-    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2, schedule=sch)
+    zo, zg, _, _ = run_pair(ctxmod, orc, mk, nsub, nsteps=2, schedule=sch, options=opts)
     assert_same(zo, zg, ghosts=True)
 
 
@@ -197,7 +202,7 @@ def test_roe_wall_correction_other_schedules(ctxmod, orc, schedule):
     zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
     orc.substep(zo, 1, first_it=0, stop=1)
     ctx = ctxmod.Context(0)
-    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
+    set_res(ctxmod, ctx, schedule)
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
@@ -280,16 +285,22 @@ def test_cfl_monitor(ctxmod, orc, mesh, flow):
     np.testing.assert_allclose(got[2], cfl[2] / n, rtol=1e-12)
 
 
-RES = {"cells": 0, "faces": 1, "kcol": 2}
+# schedule -> (FM_OPT_RESIDUAL_SCHEDULE, FM_OPT_RES_KCOL_N)
+RES = {"cells": (0, 0), "faces": (1, 0), "kcol": (2, 0), "kcol3": (2, 3), "auto": (3, 0)}
 
 
-@pytest.mark.parametrize("schedule", ["cells", "faces", "kcol"])
+def set_res(ctxmod, ctx, schedule):
+    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule][0])
+    ctx.set_option(ctxmod.OPT_RES_KCOL_N, RES[schedule][1])
+
+
+@pytest.mark.parametrize("schedule", ["cells", "faces", "kcol", "kcol3", "auto"])
 @pytest.mark.parametrize("mesh", ["cart", "full"])
 @pytest.mark.parametrize("scheme", ["roe_min", "ausmpred", "senseur"])
 def test_residual_schedules(ctxmod, orc, schedule, mesh, scheme):
     """Face balance with six faces per cell, with every face once (63-cell i-chunks, 4-row
-    j-chunks, 32-plane k-chunks) and with 4-cell k columns: the sizes straddle every chunk
-    boundary (35 and 34 planes are not multiples of 4 or 32)."""
+    j-chunks, 32-plane k-chunks), with 4-cell and 3-cell k columns and with the per-mesh default:
+    the sizes straddle every chunk boundary (35 and 34 planes are not multiples of 3, 4 or 32)."""
     num = H.numerics(scheme, "bdf")
     if mesh == "cart":
         mk = lambda: cases.tgv_zone((70, 9, 35), num, perturb=0.02)
@@ -300,7 +311,7 @@ def test_residual_schedules(ctxmod, orc, schedule, mesh, scheme):
     zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
     orc.substep(zo, 1, first_it=0, stop=1)
     ctx = ctxmod.Context(0)
-    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
+    set_res(ctxmod, ctx, schedule)
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
@@ -421,7 +432,7 @@ def test_cache_blocking(ctxmod, orc, cb, schedule, case, scheme):
     orc.init_metric(zo); orc.apply_bc(zo, zo.ro)
     zo.ro_m1[:] = zo.ro; zo.ro_p1[:] = zo.ro
     ctx = ctxmod.Context(0)
-    ctx.set_option(ctxmod.OPT_RESIDUAL_SCHEDULE, RES[schedule])
+    set_res(ctxmod, ctx, schedule)
     zid = ctx.add_zone(zg)
     ctx.apply_bc(zid, 0)
     t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
