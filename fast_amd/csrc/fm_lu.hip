@@ -54,67 +54,101 @@ __device__ __forceinline__ LUC<T> lu_const(const KZ& z) {
     return c;
 }
 
-// b = B_D(u, v, w, t; cs, face normal) d   (invlu_l.for / invlu_u.for neighbour term)
+// The matrix B_D(u, v, w, t; cs, face normal) of a neighbour term (invlu_l.for / invlu_u.for),
+// row by row: m[r][c] multiplies d[c] in row r.  Only the entries the Cartesian rows use are set
+// there; the expressions are the reference's (lu_i/j/k_3dcart.for, lu_*_3dfull.for).
+template <typename T = double>
+struct LuB { T m[5][5]; };
+
 template <bool CART, int D, typename T = double>
-__device__ __forceinline__ void lu_prod(const LUC<T>& c, T u, T v, T w, T t, T cs, T tcx, T tcy, T tcz,
-                                        const T d[5], T b[5]) {
+__device__ __forceinline__ LuB<T> lu_coef(const LUC<T>& c, T u, T v, T w, T t, T cs, T tcx, T tcy, T tcz) {
+    LuB<T> B;
     T q2 = (T).5 * (u * u + v * v + w * w);
     T h = c.cp * t + q2;
     T ph2 = c.gamm1 * q2;
     T gam1 = c.gam1, gamm1 = c.gamm1, gam2 = c.gam2;
-    T d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3], d5 = d[4];
     if (CART) {
         T tc = tcx;
         if (D == 0) {
             T qn = tc * u;
-            T b21 = (tc * ph2 - u * qn), b22 = (qn - tc * gam2 * u) + cs, b23 = (-tc * gamm1 * v);
-            T b24 = (-tc * gamm1 * w), b25 = tc * gamm1;
-            T b51 = qn * (ph2 - h), b52 = (tc * h - gamm1 * u * qn), b53 = (-gamm1 * v * qn);
-            T b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-            b[0] = cs * d1 + tc * d2;
-            b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
-            b[2] = (-v * qn) * d1 + (tc * v) * d2 + ((qn) + cs) * d3;
-            b[3] = (-w * qn) * d1 + (tc * w) * d2 + ((qn) + cs) * d4;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+            B.m[0][0] = cs; B.m[0][1] = tc;
+            B.m[1][0] = (tc * ph2 - u * qn); B.m[1][1] = (qn - tc * gam2 * u) + cs; B.m[1][2] = (-tc * gamm1 * v);
+            B.m[1][3] = (-tc * gamm1 * w); B.m[1][4] = tc * gamm1;
+            B.m[2][0] = (-v * qn); B.m[2][1] = (tc * v); B.m[2][2] = ((qn) + cs);
+            B.m[3][0] = (-w * qn); B.m[3][1] = (tc * w); B.m[3][3] = ((qn) + cs);
+            B.m[4][0] = qn * (ph2 - h); B.m[4][1] = (tc * h - gamm1 * u * qn); B.m[4][2] = (-gamm1 * v * qn);
+            B.m[4][3] = (-gamm1 * w * qn); B.m[4][4] = (gam1 * qn) + cs;
         } else if (D == 1) {
             T qn = tc * v;
-            T b31 = (tc * ph2 - v * qn), b32 = (-tc * gamm1 * u), b33 = (qn - tc * gam2 * v) + cs;
-            T b34 = (-tc * gamm1 * w), b35 = tc * gamm1;
-            T b51 = qn * (ph2 - h), b52 = (-gamm1 * u * qn), b53 = (tc * h - gamm1 * v * qn);
-            T b54 = (-gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-            b[0] = cs * d1 + tc * d3;
-            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d3;
-            b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
-            b[3] = (-w * qn) * d1 + (tc * w) * d3 + ((qn) + cs) * d4;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+            B.m[0][0] = cs; B.m[0][2] = tc;
+            B.m[1][0] = (-u * qn); B.m[1][1] = (qn + cs); B.m[1][2] = (tc * u);
+            B.m[2][0] = (tc * ph2 - v * qn); B.m[2][1] = (-tc * gamm1 * u); B.m[2][2] = (qn - tc * gam2 * v) + cs;
+            B.m[2][3] = (-tc * gamm1 * w); B.m[2][4] = tc * gamm1;
+            B.m[3][0] = (-w * qn); B.m[3][2] = (tc * w); B.m[3][3] = ((qn) + cs);
+            B.m[4][0] = qn * (ph2 - h); B.m[4][1] = (-gamm1 * u * qn); B.m[4][2] = (tc * h - gamm1 * v * qn);
+            B.m[4][3] = (-gamm1 * w * qn); B.m[4][4] = (gam1 * qn) + cs;
         } else {
             T qn = tc * w;
-            T b41 = tc * ph2 - w * qn, b42 = -tc * gamm1 * u, b43 = -tc * gamm1 * v;
-            T b44 = (qn - tc * gam2 * w) + cs, b45 = tc * gamm1;
-            T b51 = qn * (ph2 - h), b52 = -gamm1 * u * qn, b53 = -gamm1 * v * qn;
-            T b54 = tc * h - gamm1 * w * qn, b55 = gam1 * qn + cs;
-            b[0] = cs * d1 + tc * d4;
-            b[1] = (-u * qn) * d1 + (qn + cs) * d2 + (tc * u) * d4;
-            b[2] = (-v * qn) * d1 + (qn + cs) * d3 + (tc * v) * d4;
-            b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
-            b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+            B.m[0][0] = cs; B.m[0][3] = tc;
+            B.m[1][0] = (-u * qn); B.m[1][1] = (qn + cs); B.m[1][3] = (tc * u);
+            B.m[2][0] = (-v * qn); B.m[2][2] = (qn + cs); B.m[2][3] = (tc * v);
+            B.m[3][0] = tc * ph2 - w * qn; B.m[3][1] = -tc * gamm1 * u; B.m[3][2] = -tc * gamm1 * v;
+            B.m[3][3] = (qn - tc * gam2 * w) + cs; B.m[3][4] = tc * gamm1;
+            B.m[4][0] = qn * (ph2 - h); B.m[4][1] = -gamm1 * u * qn; B.m[4][2] = -gamm1 * v * qn;
+            B.m[4][3] = tc * h - gamm1 * w * qn; B.m[4][4] = gam1 * qn + cs;
+        }
+        return B;
+    }
+    T qn = tcx * u + tcy * v + tcz * w;
+    B.m[0][0] = cs; B.m[0][1] = tcx; B.m[0][2] = tcy; B.m[0][3] = tcz;
+    B.m[1][0] = (tcx * ph2 - u * qn); B.m[1][1] = (qn - tcx * gam2 * u) + cs;
+    B.m[1][2] = (tcy * u - tcx * gamm1 * v); B.m[1][3] = (tcz * u - tcx * gamm1 * w); B.m[1][4] = tcx * gamm1;
+    B.m[2][0] = (tcy * ph2 - v * qn); B.m[2][1] = (tcx * v - tcy * gamm1 * u);
+    B.m[2][2] = (qn - tcy * gam2 * v) + cs; B.m[2][3] = (tcz * v - tcy * gamm1 * w); B.m[2][4] = tcy * gamm1;
+    B.m[3][0] = (tcz * ph2 - w * qn); B.m[3][1] = (tcx * w - tcz * gamm1 * u);
+    B.m[3][2] = (tcy * w - tcz * gamm1 * v); B.m[3][3] = (qn - tcz * gam2 * w) + cs; B.m[3][4] = tcz * gamm1;
+    B.m[4][0] = qn * (ph2 - h); B.m[4][1] = (tcx * h - gamm1 * u * qn); B.m[4][2] = (tcy * h - gamm1 * v * qn);
+    B.m[4][3] = (tcz * h - gamm1 * w * qn); B.m[4][4] = (gam1 * qn) + cs;
+    return B;
+}
+
+// b = B d, each row summed left to right over the entries the row has (the reference's order)
+template <bool CART, int D, typename T = double>
+__device__ __forceinline__ void lu_apply(const LuB<T>& B, const T d[5], T b[5]) {
+    const T d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3], d5 = d[4];
+    if (CART) {
+        if (D == 0) {
+            b[0] = B.m[0][0] * d1 + B.m[0][1] * d2;
+            b[1] = B.m[1][0] * d1 + B.m[1][1] * d2 + B.m[1][2] * d3 + B.m[1][3] * d4 + B.m[1][4] * d5;
+            b[2] = B.m[2][0] * d1 + B.m[2][1] * d2 + B.m[2][2] * d3;
+            b[3] = B.m[3][0] * d1 + B.m[3][1] * d2 + B.m[3][3] * d4;
+            b[4] = B.m[4][0] * d1 + B.m[4][1] * d2 + B.m[4][2] * d3 + B.m[4][3] * d4 + B.m[4][4] * d5;
+        } else if (D == 1) {
+            b[0] = B.m[0][0] * d1 + B.m[0][2] * d3;
+            b[1] = B.m[1][0] * d1 + B.m[1][1] * d2 + B.m[1][2] * d3;
+            b[2] = B.m[2][0] * d1 + B.m[2][1] * d2 + B.m[2][2] * d3 + B.m[2][3] * d4 + B.m[2][4] * d5;
+            b[3] = B.m[3][0] * d1 + B.m[3][2] * d3 + B.m[3][3] * d4;
+            b[4] = B.m[4][0] * d1 + B.m[4][1] * d2 + B.m[4][2] * d3 + B.m[4][3] * d4 + B.m[4][4] * d5;
+        } else {
+            b[0] = B.m[0][0] * d1 + B.m[0][3] * d4;
+            b[1] = B.m[1][0] * d1 + B.m[1][1] * d2 + B.m[1][3] * d4;
+            b[2] = B.m[2][0] * d1 + B.m[2][2] * d3 + B.m[2][3] * d4;
+            b[3] = B.m[3][0] * d1 + B.m[3][1] * d2 + B.m[3][2] * d3 + B.m[3][3] * d4 + B.m[3][4] * d5;
+            b[4] = B.m[4][0] * d1 + B.m[4][1] * d2 + B.m[4][2] * d3 + B.m[4][3] * d4 + B.m[4][4] * d5;
         }
         return;
     }
-    T qn = tcx * u + tcy * v + tcz * w;
-    T b21 = (tcx * ph2 - u * qn), b22 = (qn - tcx * gam2 * u) + cs;
-    T b23 = (tcy * u - tcx * gamm1 * v), b24 = (tcz * u - tcx * gamm1 * w), b25 = tcx * gamm1;
-    T b31 = (tcy * ph2 - v * qn), b32 = (tcx * v - tcy * gamm1 * u);
-    T b33 = (qn - tcy * gam2 * v) + cs, b34 = (tcz * v - tcy * gamm1 * w), b35 = tcy * gamm1;
-    T b41 = (tcz * ph2 - w * qn), b42 = (tcx * w - tcz * gamm1 * u);
-    T b43 = (tcy * w - tcz * gamm1 * v), b44 = (qn - tcz * gam2 * w) + cs, b45 = tcz * gamm1;
-    T b51 = qn * (ph2 - h), b52 = (tcx * h - gamm1 * u * qn), b53 = (tcy * h - gamm1 * v * qn);
-    T b54 = (tcz * h - gamm1 * w * qn), b55 = (gam1 * qn) + cs;
-    b[0] = cs * d1 + tcx * d2 + tcy * d3 + tcz * d4;
-    b[1] = b21 * d1 + b22 * d2 + b23 * d3 + b24 * d4 + b25 * d5;
-    b[2] = b31 * d1 + b32 * d2 + b33 * d3 + b34 * d4 + b35 * d5;
-    b[3] = b41 * d1 + b42 * d2 + b43 * d3 + b44 * d4 + b45 * d5;
-    b[4] = b51 * d1 + b52 * d2 + b53 * d3 + b54 * d4 + b55 * d5;
+    b[0] = B.m[0][0] * d1 + B.m[0][1] * d2 + B.m[0][2] * d3 + B.m[0][3] * d4;
+#pragma unroll
+    for (int r = 1; r < 5; r++)
+        b[r] = B.m[r][0] * d1 + B.m[r][1] * d2 + B.m[r][2] * d3 + B.m[r][3] * d4 + B.m[r][4] * d5;
+}
+
+// b = B_D(u, v, w, t; cs, face normal) d   (invlu_l.for / invlu_u.for neighbour term)
+template <bool CART, int D, typename T = double>
+__device__ __forceinline__ void lu_prod(const LUC<T>& c, T u, T v, T w, T t, T cs, T tcx, T tcy, T tcz,
+                                        const T d[5], T b[5]) {
+    lu_apply<CART, D, T>(lu_coef<CART, D, T>(c, u, v, w, t, cs, tcx, tcy, tcz), d, b);
 }
 
 // Face normal of direction D used for the product of the cell at metric index lt: the mean of
@@ -366,12 +400,33 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         O[l] = r1; O[l + nd] = u1; O[l + 2 * nd] = v1; O[l + 3 * nd] = w1; O[l + 4 * nd] = t1;
         if (kalias) { S.wk[0][L] = (T)u1; S.wk[1][L] = (T)v1; S.wk[2][L] = (T)w1; S.wk[3][L] = (T)t1; }
     };
+    // the product a wave forms: B_D of the cell it handles, from the staged state and coefficients
+    auto coef = [&](int ii, T uu, T vv, T ww, T tt) {
+        T tx, ty, tz;
+        if (wv == 1) {
+            if (CART) { tx = tci; ty = tz = (T)0.; }
+            else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
+            return lu_coef<CART, 0, T>(c, uu, vv, ww, tt, S.st[6][ii][L], tx, ty, tz);
+        } else if (wv == 2) {
+            if (CART) { tx = tcj; ty = tz = (T)0.; }
+            else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
+            return lu_coef<CART, 1, T>(c, uu, vv, ww, tt, S.st[7][ii][L], tx, ty, tz);
+        }
+        if (CART) { tx = tck; ty = tz = (T)0.; }
+        else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
+        return lu_coef<CART, 2, T>(c, uu, vv, ww, tt, S.st[8][ii][L], tx, ty, tz);
+    };
     for (int s = 0; s < nsteps; s++) {
         const int ii = s - jj - kk;
         const bool act = lane_ok && ii >= 0 && ii < ci;
         T x[5];
         double xd[5], r0 = 0., u0 = 0., v0 = 0., w0 = 0., t0 = 0.;
         long l = 0;
+        // the product waves form their B matrix while wave 0 solves (it does not depend on the
+        // solution; with in-place sub-iterations the k products wait for the updated state)
+        LuB<T> cb;
+        if (wv > 0 && act && !(wv == 3 && kalias))
+            cb = coef(ii, S.st[1][ii][L], S.st[2][ii][L], S.st[3][ii][L], S.st[4][ii][L]);
         if (wv == 0 && act) {
             const int oi = oi0 + ii;
             l = IDX(z, I(oi), j, k);
@@ -443,22 +498,10 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             T b[5];
 #pragma unroll
             for (int e = 0; e < 5; e++) x[e] = S.xs[e][L];
-            T uu = S.st[1][ii][L], vv = S.st[2][ii][L], ww = S.st[3][ii][L], tt = S.st[4][ii][L];
-            T tx, ty, tz;
-            if (wv == 1) {
-                if (CART) { tx = tci; ty = tz = (T)0.; }
-                else { tx = S.st[15][ii][L]; ty = S.st[16][ii][L]; tz = S.st[17][ii][L]; }
-                lu_prod<CART, 0, T>(c, uu, vv, ww, tt, S.st[6][ii][L], tx, ty, tz, x, b);
-            } else if (wv == 2) {
-                if (CART) { tx = tcj; ty = tz = (T)0.; }
-                else { tx = S.st[18][ii][L]; ty = S.st[19][ii][L]; tz = S.st[20][ii][L]; }
-                lu_prod<CART, 1, T>(c, uu, vv, ww, tt, S.st[7][ii][L], tx, ty, tz, x, b);
-            } else {
-                if (kalias) { uu = S.wk[0][L]; vv = S.wk[1][L]; ww = S.wk[2][L]; tt = S.wk[3][L]; }
-                if (CART) { tx = tck; ty = tz = (T)0.; }
-                else { tx = S.st[21][ii][L]; ty = S.st[22][ii][L]; tz = S.st[23][ii][L]; }
-                lu_prod<CART, 2, T>(c, uu, vv, ww, tt, S.st[8][ii][L], tx, ty, tz, x, b);
-            }
+            if (wv == 3 && kalias) cb = coef(ii, S.wk[0][L], S.wk[1][L], S.wk[2][L], S.wk[3][L]);
+            if (wv == 1) lu_apply<CART, 0, T>(cb, x, b);
+            else if (wv == 2) lu_apply<CART, 1, T>(cb, x, b);
+            else lu_apply<CART, 2, T>(cb, x, b);
 #pragma unroll
             for (int e = 0; e < 5; e++) S.px[wv - 1][e][L] = b[e];
             if (FLOW) {
