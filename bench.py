@@ -66,7 +66,7 @@ def parse():
                          "(4x4x4 tiling, sizes from default_rng(1) in [16, 96]) joined by transfers")
     ap.add_argument("--autotune", action="store_true",
                     help="time the interchangeable schedules first and keep the fastest (fast_amd.tune)")
-    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--graph", type=int, default=1, choices=[0, 1],
                     help="1 replays captured HIP graphs of the time step (FM_OPT_GRAPH)")
     ap.add_argument("--zone-batch", type=int, default=1, choices=[0, 1],
                     help="zones64: 1 batches the LU-SGS of all zones per block diagonal, 0 loops")

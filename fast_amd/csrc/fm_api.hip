@@ -81,6 +81,7 @@ struct fm_ctx {
     std::vector<LuList> lu_lists;
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
+    int* flow_ctr;           // block counter of the batched dataflow sweeps
     int lu_batch;            // batch the LU of several zones (default on)
     int lu_fp32;             // mixed precision: LU-SGS sweep arithmetic in FP32 (default off)
     int res_kcol_n;          // tile shapes: k-column length of RES_KCOL (0: built-in)
@@ -133,6 +134,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     c->res_schedule = RES_AUTO;
     c->lu_batch = 1;
     c->lu_fp32 = 0;
+    c->graph_on = 1;
     c->res_kcol_n = 0;
     c->lu_flow_grid = 0;
     if (int rc = resolve_scratch(device)) { delete c; return rc; }
@@ -173,6 +175,7 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
         delete g;
     }
     if (c->lu_tab) (void)hipFree(c->lu_tab);
+    if (c->flow_ctr) (void)hipFree(c->flow_ctr);
     for (int q = 0; q < 2; q++) {
         if (c->lu_host[q]) (void)hipHostFree(c->lu_host[q]);
         if (c->lu_ev[q]) (void)hipEventDestroy(c->lu_ev[q]);
@@ -534,19 +537,25 @@ static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
     return FM_OK;
 }
 
+// The dataflow face exchange of a zone, sized for the largest LU box (interior plus a wall layer
+// per side) and filled with the pending pattern once: the sweeps give back every slot they read.
+static int ensure_flow_exchange(fm_ctx* c, Zone* z) {
+    if (z->lu_fx) return FM_OK;
+    const KZ& k = z->k;
+    long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMalloc(&z->lu_fx, sizeof(double) * n));
+    fill_pending(z->lu_fx, n, c->stream);
+    HIPCHK(hipGetLastError());
+    return FM_OK;
+}
+
 static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
     // the dataflow schedule, and what the streaming schedule falls back to (curvilinear zones,
     // FP32 sweeps)
     if (c->lu_schedule != LU_DATAFLOW && !(c->lu_schedule == LU_STREAM && !uses_stream(c, z))) return FM_OK;
     const KZ& k = z->k;
-    if (!z->lu_fx) {
-        long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMalloc(&z->lu_fx, sizeof(double) * n));
-        // pending once: the sweeps give back every slot they read
-        fill_pending(z->lu_fx, n, c->stream);
-        HIPCHK(hipGetLastError());
-    }
+    if (int rc = ensure_flow_exchange(c, z)) return rc;
     const Box& f = z->flow_box;
     if (!z->flow_work || f.i0 != b.i0 || f.i1 != b.i1 || f.j0 != b.j0 || f.j1 != b.j1 || f.k0 != b.k0 || f.k1 != b.k1) {
         std::vector<LuWork> work;
@@ -1008,12 +1017,26 @@ extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
 static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector<LuArgs>& las,
                       fm_ctx::GraphEntry* ge = nullptr, int nstep = 0, bool capture = false) {
     hipStream_t st = c->stream;
+    // FP64 dataflow: one launch per sweep over the blocks of all zones, each zone with its own face
+    // exchange; otherwise (blocks schedule, FP32 sweeps) one launch per block diagonal
+    const bool flow = (c->lu_schedule == LU_DATAFLOW || c->lu_schedule == LU_STREAM) && !c->lu_fp32;
+    if (flow && !c->flow_ctr) {
+        if (capture) return fail(FM_E_HIP, "graph capture: dataflow counter missing");
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMalloc(&c->flow_ctr, 4 * sizeof(int)));
+        HIPCHK(hipMemset(c->flow_ctr, 0, 4 * sizeof(int)));
+    }
     for (int cart = 0; cart < 2; cart++) {
         std::vector<Box> boxes;
         std::vector<LuZone> tab;
         for (size_t n = 0; n < zs.size(); n++) {
             if ((zs[n]->k.typ == 2) != (cart == 1)) continue;
-            const LuArgs& a = las[n];
+            LuArgs a = las[n];
+            if (flow) {
+                if (!zs[n]->lu_fx && capture) return fail(FM_E_HIP, "graph capture: face exchange missing");
+                if (int rc = ensure_flow_exchange(c, zs[n])) return rc;
+                a.fx = zs[n]->lu_fx; a.flow_step = 0; a.schedule = LU_DATAFLOW;
+            }
             boxes.push_back(Box{a.i0, a.i1, a.j0, a.j1, a.k0, a.k1});
             tab.push_back(LuZone{zs[n]->k, a});
         }
@@ -1050,8 +1073,12 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
                 ge->tabs.push_back({key, d});
             }
             if (!capture) HIPCHK(hipMemcpy(d, tab.data(), tab.size() * sizeof(LuZone), hipMemcpyHostToDevice));
-            launch_lu_list(cart == 1, true, c->lu_fp32, d, L->work, L->offs, st);
-            launch_lu_list(cart == 1, false, c->lu_fp32, d, L->work, L->offs, st);
+            if (flow) {
+                launch_lu_flow_list(cart == 1, d, L->work, L->offs.back(), c->flow_ctr, st);
+            } else {
+                launch_lu_list(cart == 1, true, c->lu_fp32, d, L->work, L->offs, st);
+                launch_lu_list(cart == 1, false, c->lu_fp32, d, L->work, L->offs, st);
+            }
             HIPCHK(hipGetLastError());
             continue;
         }
@@ -1072,8 +1099,12 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         memcpy(c->lu_host[q], tab.data(), tab.size() * sizeof(LuZone));
         HIPCHK(hipMemcpyAsync(c->lu_tab, c->lu_host[q], tab.size() * sizeof(LuZone), hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(c->lu_ev[q], st));
-        launch_lu_list(cart == 1, true, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
-        launch_lu_list(cart == 1, false, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
+        if (flow) {
+            launch_lu_flow_list(cart == 1, c->lu_tab, L->work, L->offs.back(), c->flow_ctr, st);
+        } else {
+            launch_lu_list(cart == 1, true, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
+            launch_lu_list(cart == 1, false, c->lu_fp32, c->lu_tab, L->work, L->offs, st);
+        }
         HIPCHK(hipGetLastError());
     }
     for (size_t n = 0; n < zs.size(); n++) {
@@ -1155,7 +1186,7 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
     for (int it = 0; it < nit_c; it++) {
         int nitrun_loc = nitrun * nit_c + it;
         int verif = modulo_verif > 0 && (nitrun_loc % modulo_verif == 0);
-        if (!c->graph_on || c->lu_schedule != LU_BLOCKS) {
+        if (!c->graph_on) {
             int rc = time_step(c, first_it, verif, nstep_deb, nstep_fin, nullptr, false);
             if (rc) return rc;
         } else {

@@ -121,6 +121,10 @@ struct LuWork { int zone, bi, bj, bk; };
 void lu_work_list(bool cart, const std::vector<Box>& boxes, std::vector<LuWork>& work, std::vector<int>& offs);
 void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const LuWork* work,
                     const std::vector<int>& offs, hipStream_t st);
+// dataflow sweeps (lower then upper, FP64) of several zones of one mesh kind in one launch each:
+// the work list in diagonal order, every zone's LuArgs with its face exchange (fx) and error word
+void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,
+                         hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);
 // doubles of the tile-boundary exchange buffer the pipelined sweeps of an LU box need
 long lu_exchange_size(int ni, int nj, int nk);

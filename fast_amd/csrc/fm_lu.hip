@@ -551,6 +551,30 @@ __global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nb
     }
 }
 
+// Dataflow sweeps of several zones in one launch: the blocks of every zone's LU box in one list
+// ordered by block diagonal, handed out from one counter.  A block only waits on blocks of its own
+// zone on earlier diagonals, which were handed out before it; each zone has its own face exchange
+// and error word.
+template <bool CART, bool LOWER, int CI>
+__global__ __launch_bounds__(256) void k_lu_flow_list(const LuZone* __restrict__ zones,
+                                                      const LuWork* __restrict__ work, int nwork, int* counter) {
+    __shared__ LuLds<CART, CI, double> S;
+    __shared__ int tk;
+    for (;;) {
+        if (threadIdx.x == 0) tk = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = tk;
+        __syncthreads();
+        if (t >= nwork) break;
+        const LuWork w = work[t];
+        const KZ z = zones[w.zone].z;
+        const LuArgs a = zones[w.zone].a;
+        const int nbi = (a.i1 - a.i0 + CI) / CI, nbj = (a.j1 - a.j0 + 8) / 8, nbk = (a.k1 - a.k0 + 8) / 8;
+        lu_block<CART, LOWER, CI, double, true>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
+        __syncthreads();
+    }
+}
+
 // Several zones in one launch per block diagonal: block b of the launch takes its zone and its
 // block coordinates from the work list (zones are independent: each keeps its own diagonal
 // order, diagonal d of every zone goes into launch d).
@@ -1262,6 +1286,29 @@ void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const
         } else {
             if (lower) hipLaunchKernelGGL((k_lu_block_list<false, true, CF>), dim3(n), dim3(256), 0, st, zones, w);
             else hipLaunchKernelGGL((k_lu_block_list<false, false, CF>), dim3(n), dim3(256), 0, st, zones, w);
+        }
+    }
+}
+
+void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,
+                         hipStream_t st) {
+    if (nwork <= 0) return;
+    constexpr int CC = lu_ci<true>(), CF = lu_ci<false>();
+    int cus = 0, dev = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    const void* kfn = cart ? (const void*)k_lu_flow_list<true, true, CC> : (const void*)k_lu_flow_list<false, true, CF>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, 256, 0) != hipSuccess || per < 1) per = 1;
+    long cap = (long)per * cus;
+    const unsigned grid = (unsigned)(nwork < cap ? nwork : cap);
+    for (int lower = 1; lower >= 0; lower--) {
+        hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, (double*)nullptr, 0L, kLuPending, counter);
+        if (cart) {
+            if (lower) hipLaunchKernelGGL((k_lu_flow_list<true, true, CC>), dim3(grid), dim3(256), 0, st, zones, work, nwork, counter);
+            else hipLaunchKernelGGL((k_lu_flow_list<true, false, CC>), dim3(grid), dim3(256), 0, st, zones, work, nwork, counter);
+        } else {
+            if (lower) hipLaunchKernelGGL((k_lu_flow_list<false, true, CF>), dim3(grid), dim3(256), 0, st, zones, work, nwork, counter);
+            else hipLaunchKernelGGL((k_lu_flow_list<false, false, CF>), dim3(grid), dim3(256), 0, st, zones, work, nwork, counter);
         }
     }
 }

@@ -213,10 +213,10 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    block diagonal over a work list of (zone, block) pairs (the blocks schedule); 0 sweeps the
    zones one after the other.  Same results. */
 #define FM_OPT_ZONE_BATCH 3
-/* FM_OPT_GRAPH: 1 captures each distinct time-step configuration (level pointers, first step,
-   verification, sub-iteration range) into a HIP graph the second time it runs and replays it
-   afterwards (blocks LU schedule); 0 (default) launches every kernel.  Same results.  Changing
-   options, param_real, zones or transfers drops the captured graphs. */
+/* FM_OPT_GRAPH: 1 (default) captures each distinct time-step configuration (level pointers, first
+   step, verification, sub-iteration range) into a HIP graph the second time it runs and replays it
+   afterwards; 0 launches every kernel.  Same results.  Changing options, param_real, zones or
+   transfers drops the captured graphs. */
 #define FM_OPT_GRAPH 4
 /* FM_OPT_LU_PRECISION: FM_LU_FP64 (default) runs the LU-SGS sweeps in FP64, bit-identical to the
    reference; FM_LU_FP32 runs the sweep arithmetic (products, solve, LDS image) in FP32 while the
