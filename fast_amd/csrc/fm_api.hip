@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <cstdarg>
 #include <mutex>
 #include <string>
@@ -82,6 +83,12 @@ struct fm_ctx {
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
     int* flow_ctr;           // block counter of the batched dataflow sweeps
+    // several zones: each zone's right-hand side and boundary conditions run on a stream of its
+    // own (forked from and joined back to `stream`); `cur` is the stream the zone work goes to
+    hipStream_t cur;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
+    hipEvent_t fork_ev;
     int lu_batch;            // batch the LU of several zones (default on)
     int lu_fp32;             // mixed precision: LU-SGS sweep arithmetic in FP32 (default off)
     int res_kcol_n;          // tile shapes: k-column length of RES_KCOL (0: built-in)
@@ -135,11 +142,14 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     c->lu_batch = 1;
     c->lu_fp32 = 0;
     c->graph_on = 1;
+    c->cur = nullptr;
+    c->fork_ev = nullptr;
     c->res_kcol_n = 0;
     c->lu_flow_grid = 0;
     if (int rc = resolve_scratch(device)) { delete c; return rc; }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(FM_E_HIP, "stream: %s", hipGetErrorString(e)); }
+    c->cur = c->stream;
     HIPCHK(hipEventCreate(&c->e0));
     HIPCHK(hipEventCreate(&c->e1));
     *out = c;
@@ -176,6 +186,9 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
     }
     if (c->lu_tab) (void)hipFree(c->lu_tab);
     if (c->flow_ctr) (void)hipFree(c->flow_ctr);
+    for (hipStream_t s2 : c->side) (void)hipStreamDestroy(s2);
+    for (hipEvent_t e2 : c->side_ev) (void)hipEventDestroy(e2);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     for (int q = 0; q < 2; q++) {
         if (c->lu_host[q]) (void)hipHostFree(c->lu_host[q]);
         if (c->lu_ev[q]) (void)hipEventDestroy(c->lu_ev[q]);
@@ -467,7 +480,7 @@ static void bczone(fm_ctx* c, Zone* z, int lrhs, double* rop, int shift_lu[6], i
             for (int l = 0; l < 6; l++) fen[l] = ind_dm[l];
             if (idir % 2 == 0) { fen[idir] = 1; fen[idir + 1] = 1; } else fen[idir - 1] = fen[idir];
             BcArgs a; a.idir = idir + 1; a.kind = 0; a.lrhs = 1;
-            if (cl_window(z, idir + 1, fen, a.w)) launch_bc_face(k, rop, a, c->stream);
+            if (cl_window(z, idir + 1, fen, a.w)) launch_bc_face(k, rop, a, c->cur);
         }
     }
     for (const BcRec& b : z->bcs) {
@@ -483,7 +496,7 @@ static void bczone(fm_ctx* c, Zone* z, int lrhs, double* rop, int shift_lu[6], i
         for (int q = 0; q < 5; q++) a.state[q] = b.state[q];
         a.pext = b.pext;
         for (int q = 0; q < 3; q++) a.inc_bc[q] = b.inc_bc[q];
-        if (cl_window(z, b.idir, b.fen, a.w)) launch_bc_face(k, rop, a, c->stream);
+        if (cl_window(z, b.idir, b.fen, a.w)) launch_bc_face(k, rop, a, c->cur);
     }
     if (lcorner_out) *lcorner_out = lcorner;
 }
@@ -493,7 +506,7 @@ static void coins(fm_ctx* c, Zone* z, double* rop, const int* ind) {
     const KZ& k = z->k;
     L.i0 = ind[0] > 1 ? ind[0] : 1; L.j0 = ind[2] > 1 ? ind[2] : 1; L.k0 = ind[4] > 1 ? ind[4] : 1;
     L.i1 = ind[1] < k.ni ? ind[1] : k.ni; L.j1 = ind[3] < k.nj ? ind[3] : k.nj; L.k1 = ind[5] < k.nk ? ind[5] : k.nk;
-    launch_coins(k, rop, L, c->stream);
+    launch_coins(k, rop, L, c->cur);
 }
 
 // The pipelined LU schedule's exchange buffer, sized for the largest LU box of the zone (one
@@ -501,7 +514,7 @@ static void coins(fm_ctx* c, Zone* z, double* rop, const int* ind) {
 static int ensure_exchange(fm_ctx* c, Zone* z) {
     if (c->lu_schedule != LU_PIPELINE || z->lu_xch) return FM_OK;
     long n = lu_exchange_size(z->k.ni + 2, z->k.nj + 2, z->k.nk + 2);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMalloc(&z->lu_xch, sizeof(double) * n));
     return FM_OK;
 }
@@ -528,9 +541,9 @@ static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
     if (!z->lu_stx) {
         const KZ& k = z->k;
         long n = lu_stream_exchange_size(k.ni + 2, k.nj + 2, k.nk + 2);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMalloc(&z->lu_stx, sizeof(double) * n));
-        fill_pending(z->lu_stx, n, c->stream);
+        fill_pending(z->lu_stx, n, c->cur);
         HIPCHK(hipGetLastError());
     }
     la.stx = z->lu_stx;
@@ -543,9 +556,9 @@ static int ensure_flow_exchange(fm_ctx* c, Zone* z) {
     if (z->lu_fx) return FM_OK;
     const KZ& k = z->k;
     long n = lu_flow_exchange_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMalloc(&z->lu_fx, sizeof(double) * n));
-    fill_pending(z->lu_fx, n, c->stream);
+    fill_pending(z->lu_fx, n, c->cur);
     HIPCHK(hipGetLastError());
     return FM_OK;
 }
@@ -561,7 +574,7 @@ static int ensure_dataflow(fm_ctx* c, Zone* z, const Box& b, LuArgs& la) {
         std::vector<LuWork> work;
         std::vector<int> offs;
         lu_work_list(k.typ == 2, std::vector<Box>{b}, work, offs);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipDeviceSynchronize());
         if (z->flow_work) { HIPCHK(hipFree(z->flow_work)); z->flow_work = nullptr; }
         HIPCHK(hipMalloc(&z->flow_work, sizeof(LuWork) * (work.empty() ? 1 : work.size())));
         if (!work.empty())
@@ -637,7 +650,7 @@ static int wall_corrections(const Zone* z, RhsArgs& a) {
 static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_verif, int lexit_lu, int stop,
                    LuArgs* defer = nullptr) {
     KZ& k = z->k;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->cur;
     int implicit = k.itypcp <= 1;
     double* ro = z->ro[0]; double* ro_m1 = z->ro[1]; double* ro_p1 = z->ro[2];
     double *ro_ssiter, *ro_cl;
@@ -1118,6 +1131,37 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
     return FM_OK;
 }
 
+// Several zones: the per-zone work of a phase (right-hand side, boundary conditions) runs on up
+// to kSideStreams streams forked from the solver stream and joined back to it, so that the
+// kernels of small zones run side by side.  Graph capture records the fork and join as branches.
+constexpr size_t kSideStreams = 8;
+
+static int side_fork(fm_ctx* c, size_t ns, bool capture) {
+    if (c->side.size() < ns || !c->fork_ev) {
+        if (capture) return fail(FM_E_HIP, "graph capture: zone streams missing");
+        while (c->side.size() < ns) {
+            hipStream_t s2;
+            hipEvent_t e2;
+            HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+            c->side.push_back(s2); c->side_ev.push_back(e2);
+        }
+        if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+    for (size_t s = 0; s < ns; s++) HIPCHK(hipStreamWaitEvent(c->side[s], c->fork_ev, 0));
+    return FM_OK;
+}
+
+static int side_join(fm_ctx* c, size_t ns) {
+    c->cur = c->stream;
+    for (size_t s = 0; s < ns; s++) {
+        HIPCHK(hipEventRecord(c->side_ev[s], c->side[s]));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->side_ev[s], 0));
+    }
+    return FM_OK;
+}
+
 // Within a sub-iteration every zone first gets its right-hand side and LU-SGS update, then
 // the transfers fill the ghost cells shared with other zones, then every zone's physical
 // boundary conditions and corners are applied (gsdr3.cpp:205-444).
@@ -1132,12 +1176,15 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
                        (c->lu_schedule == LU_BLOCKS || c->lu_schedule == LU_DATAFLOW || c->lu_schedule == LU_STREAM);
     std::vector<int> lexit(nz);
     std::vector<double*> cl(nz);
+    const size_t ns = nz > 1 ? std::min(nz, kSideStreams) : 0;
     for (int nstep = n0; nstep <= n1; nstep++) {
         int any = 0;
         std::vector<Zone*> bz;
         std::vector<LuArgs> bl;
+        if (ns) { if (int rc = side_fork(c, ns, capture)) return rc; }
         for (size_t n = 0; n < nz; n++) {
             Zone* z = c->zones[n];
+            if (ns) c->cur = c->side[n % ns];
             int implicit = z->k.itypcp <= 1;
             int nssiter = z->k.nssiter;
             // souszones_list.cpp:44-84: a verification step computes residuals (and the
@@ -1152,11 +1199,12 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
             LuArgs la;
             la.i0 = 1; la.i1 = 0;
             int rc = substep(c, z, nstep, first_it, lssiter_verif, lexit_lu, 3, batch ? &la : nullptr);
-            if (rc) return rc;
+            if (rc) { c->cur = c->stream; return rc; }
             if (batch && implicit && lexit_lu == 0 && la.i1 >= la.i0) { bz.push_back(z); bl.push_back(la); }
             lexit[n] = lexit_lu;
             any |= lexit_lu == 0;
         }
+        if (ns) { if (int rc = side_join(c, ns)) return rc; }
         if (!bz.empty()) {
             int rc = lu_batched(c, bz, bl, ge, nstep, capture);
             if (rc) return rc;
@@ -1164,8 +1212,12 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
         if (!any) continue;
         int rc = transfer(c, cl.data());
         if (rc) return rc;
-        for (size_t n = 0; n < nz; n++)
+        if (ns) { if (int rc2 = side_fork(c, ns, capture)) return rc2; }
+        for (size_t n = 0; n < nz; n++) {
+            if (ns) c->cur = c->side[n % ns];
             if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
+        }
+        if (ns) { if (int rc2 = side_join(c, ns)) return rc2; }
         HIPCHK(hipGetLastError());
     }
     return FM_OK;
