@@ -57,6 +57,7 @@ struct Zone {
     double* adj_buf;         // face contributions of the transposed product (on first use)
     double* lu_fx;           // dataflow LU schedule: face exchange and block order (on first use)
     double* lu_stx;          // streaming LU schedule: pencil exchange (on first use)
+    double* lu_skw;          // streaming LU schedule: skewed copy of the sweep inputs (on first use)
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
@@ -168,6 +169,7 @@ static void free_zone(Zone* z) {
     if (z->adj_buf) (void)hipFree(z->adj_buf);
     if (z->lu_fx) (void)hipFree(z->lu_fx);
     if (z->lu_stx) (void)hipFree(z->lu_stx);
+    if (z->lu_skw) (void)hipFree(z->lu_skw);
     if (z->flow_work) (void)hipFree(z->flow_work);
     delete z;
 }
@@ -251,7 +253,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
-    z->lu_fx = nullptr; z->lu_stx = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
+    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
     z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
@@ -537,6 +539,7 @@ static bool uses_stream(const fm_ctx* c, const Zone* z) {
 
 static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
     la.stx = nullptr;
+    la.skw = nullptr;
     if (!uses_stream(c, z)) return FM_OK;
     if (!z->lu_stx) {
         const KZ& k = z->k;
@@ -546,7 +549,13 @@ static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
         fill_pending(z->lu_stx, n, c->cur);
         HIPCHK(hipGetLastError());
     }
+    if (!z->lu_skw) {
+        const KZ& k = z->k;
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMalloc(&z->lu_skw, sizeof(double) * lu_skew_size(k.ni + 2, k.nj + 2, k.nk + 2)));
+    }
     la.stx = z->lu_stx;
+    la.skw = z->lu_skw;
     return FM_OK;
 }
 

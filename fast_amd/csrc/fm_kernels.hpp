@@ -59,12 +59,16 @@ struct LuArgs {
     double* fx;
     int flow_step;                // dataflow: blocks take their halo products step by step (FP64)
     int flow_grid;                // dataflow: workgroups of a sweep (0: as many as are resident)
-    // streaming schedule: the pencil-to-pencil exchange (pending pattern when idle)
+    // streaming schedule: the pencil-to-pencil exchange (pending pattern when idle) and the
+    // skewed, pencil-major copy of the sweep inputs
     double* stx;
+    double* skw;
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
-// doubles of the streaming exchange of an LU box of ni x nj x nk cells
+// doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk
+// cells
 long lu_stream_exchange_size(int ni, int nj, int nk);
+long lu_skew_size(int ni, int nj, int nk);
 // fill n doubles of an exchange with the pending pattern (once when allocated, and after a wait
 // gave up: the sweeps give back every slot they read)
 void fill_pending(double* x, long n, hipStream_t st);

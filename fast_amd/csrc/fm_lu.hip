@@ -253,7 +253,7 @@ template <int CI> struct LuFace {
     }
 };
 
-template <bool CART, bool LOWER, int CI, typename T = double, bool FLOW = false>
+template <bool CART, bool LOWER, int CI, typename T = double, bool FLOW = false, bool STEP = false>
 __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, int bj, int bk,
                                          LuLds<CART, CI, T>& S, int nbi = 0, int nbj = 0, int nbk = 0) {
     constexpr int NV = LuLds<CART, CI, T>::NV;
@@ -302,8 +302,10 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             S.st[21][ii][L] = (T)tx; S.st[22][ii][L] = (T)ty; S.st[23][ii][L] = (T)tz;
         }
     }
+    // step-by-step halo hand-over (FP64 sweeps only)
+    constexpr bool FSTEP = FLOW && STEP && F64;
     // 1b. products of the halo cells (finished by earlier launches; dataflow: handed over by the
-    //     neighbour blocks through the face exchange).  With a.flow_step (FP64 sweeps) the block
+    //     neighbour blocks through the face exchange).  With FSTEP the block
     //     fetches what is there and the solver lane waits for a value still pending at the step
     //     that uses it, so a block starts while the blocks before it are still running; otherwise
     //     the block waits here for its whole halo.  Every slot read is stored back as pending (by
@@ -329,7 +331,7 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
             }
 #pragma unroll
             for (int v2 = 0; v2 < 5; v2++) v[v2] = lu_ldc(a.fx + base + (long)v2 * LuFace<CI>::N);
-            if (!F64 || !a.flow_step) {
+            if (!FSTEP) {
                 lu_settle(v, 5, a.fx + base, LuFace<CI>::N, a.sync + 1);
 #pragma unroll
                 for (int v2 = 0; v2 < 5; v2++) lu_stc(a.fx + base + (long)v2 * LuFace<CI>::N, __longlong_as_double(kLuPending));
@@ -430,6 +432,28 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
         if (wv == 0 && act) {
             const int oi = oi0 + ii;
             l = IDX(z, I(oi), j, k);
+            // every LDS operand of the solve is read before the first use (one round trip);
+            // neighbour terms in the reference order: lower 3dcart J,K,I; lower 3dfull and
+            // upper K,J,I
+            const T xal = S.st[5][ii][L];
+            const T diag = S.st[9][ii][L];
+#pragma unroll
+            for (int e = 0; e < 5; e++) x[e] = S.st[10 + e][ii][L];
+            T bn[3][5];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int D = (LOWER && CART) ? (q == 0 ? 1 : q == 1 ? 2 : 0) : (q == 0 ? 2 : q == 1 ? 1 : 0);
+                if (D == 0) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) bn[q][e] = ii == 0 ? S.hI[e][L] : S.px[0][e][L];
+                } else if (D == 1) {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) bn[q][e] = jj == 0 ? S.hJ[e][ii][kk] : S.px[1][e][L - 1];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 5; e++) bn[q][e] = kk == 0 ? S.hK[e][ii][jj] : S.px[2][e][L - 8];
+                }
+            }
             if (newton) {
                 // the Newton update works on the double state: the staged copy (FP64 sweeps)
                 // or global memory (mixed precision)
@@ -440,28 +464,13 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
                     r0 = rop[l]; u0 = rop[l + nd]; v0 = rop[l + 2 * nd]; w0 = rop[l + 3 * nd]; t0 = rop[l + 4 * nd];
                 }
             }
-            const T xal = S.st[5][ii][L];
-            T b[5];
-#pragma unroll
-            for (int e = 0; e < 5; e++) x[e] = S.st[10 + e][ii][L];
-            // neighbour terms in the reference order: lower 3dcart J,K,I; lower 3dfull and
-            // upper K,J,I
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 const int D = (LOWER && CART) ? (q == 0 ? 1 : q == 1 ? 2 : 0) : (q == 0 ? 2 : q == 1 ? 1 : 0);
                 const int o = D == 0 ? oi : D == 1 ? oj : ok;
                 if (o == 0) continue;
-                if (D == 0) {
-#pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = ii == 0 ? S.hI[e][L] : S.px[0][e][L];
-                } else if (D == 1) {
-#pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = jj == 0 ? S.hJ[e][ii][kk] : S.px[1][e][L - 1];
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 5; e++) b[e] = kk == 0 ? S.hK[e][ii][jj] : S.px[2][e][L - 8];
-                }
-                if (FLOW && F64 && a.flow_step && (D == 0 ? ii == 0 : D == 1 ? jj == 0 : kk == 0)) {
+                T* b = bn[q];
+                if (FSTEP && (D == 0 ? ii == 0 : D == 1 ? jj == 0 : kk == 0)) {
                     // a product of the previous block in D: wait for it if it was still pending
                     // when the block was staged, then give the slot back
                     const long nb = D == 0 ? bi - 1 + (long)nbi * (bj + (long)nbj * bk)
@@ -484,7 +493,6 @@ __device__ __forceinline__ void lu_block(const KZ& z, const LuArgs& a, int bi, i
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] + b[e] * xal;
             }
-            const T diag = S.st[9][ii][L];
 #pragma unroll
             for (int e = 0; e < 5; e++) {
                 x[e] = x[e] * diag; xd[e] = (double)x[e]; dw[l + e * nd] = xd[e]; S.xs[e][L] = x[e];
@@ -535,7 +543,7 @@ __global__ __launch_bounds__(256) void k_lu_block(KZ z, LuArgs a, int diag, int 
 // dispatch order), stage their own fields at once, read the products of their halo cells from
 // the face exchange as the neighbour blocks hand them over, and hand their own face products on.
 // Launch diagonals are no longer barriers: a block starts as soon as its three neighbours are done.
-template <bool CART, bool LOWER, int CI, typename T = double>
+template <bool CART, bool LOWER, int CI, typename T = double, bool STEP = false>
 __global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nbj, int nbk) {
     __shared__ LuLds<CART, CI, T> S;
     __shared__ int tk;
@@ -546,7 +554,7 @@ __global__ __launch_bounds__(256) void k_lu_flow(KZ z, LuArgs a, int nbi, int nb
         __syncthreads();
         if (t >= a.flow_n) break;
         const LuWork w = a.flow_work[t];
-        lu_block<CART, LOWER, CI, T, true>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
+        lu_block<CART, LOWER, CI, T, true, STEP>(z, a, w.bi, w.bj, w.bk, S, nbi, nbj, nbk);
         __syncthreads();
     }
 }
@@ -831,172 +839,239 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 // ---------------------------------------------------------------------------------------
 // Streaming schedule (LU_STREAM, Cartesian zones).  One wave per pencil of 8 x 8 (j, k) lines,
 // sweeping the whole i extent of the LU box as a skewed wavefront: lane (jj, kk) handles cell
-// ii = s - jj - kk of its line at step s, so the cell it needs from the line below in j (k) was
-// finished one step earlier by lane L - 1 (L - 8).
+// ii = s - jj - kk of its line at step s of the lower sweep, and ii = R - 1 - s - jj - kk at step
+// s of the upper sweep (R = ni + 14), so the cell it needs from the line before it in j (k) was
+// finished one step earlier by lane L -/+ 1 (L -/+ 8).
 //
 //  * The wave solves its 64 cells (the reference's neighbour terms and division) and forms the
 //    three products of each for its consumers in i, j and k: the i product stays in the lane's
-//    registers, the j product moves to lane L + 1 by a DPP row shift, the k product to lane L + 8
-//    by ds_bpermute.  No barrier: the wave runs the pencil alone, and four pencils share a CU.
-//  * Inputs reach LDS by direct loads (global_load_lds_dwordx4): every lane fetches the cells of
-//    its own line two at a time (16 contiguous bytes per array), into a ring of two pairs; a pair
-//    is refilled with the cells four steps ahead as soon as its two steps have read it.
-//  * Pencils hand their j-face (k-face) products to the next pencil in j (k) through a global
-//    exchange, one value per (pencil, lane, variable, cell), with agent-scope stores; the
+//    registers, the j product moves one lane by a DPP row shift, the k product eight lanes by
+//    ds_bpermute.  No barrier: the wave runs the pencil alone.
+//  * Inputs come from a skewed, pencil-major copy made before the sweeps (k_lu_pack): row r of a
+//    pencil holds, for each lane, the cell r - jj - kk of its line (right-hand side, state, the
+//    coefficients: 16 doubles), so a step reads whole 512-byte rows, one per array, and every byte
+//    fetched is used.  The wave loads the rows kLuStAhead steps ahead into registers.  The lower
+//    sweep writes dW* back into the right-hand-side rows; the upper sweep reads it from there and
+//    writes dW and the Newton-updated state into the rows it has read, which k_lu_unpack copies
+//    to the zone arrays.
+//  * Pencils hand their j-face (k-face) products to the next pencil in sweep order through a
+//    global exchange, one value per (pencil, lane, variable, cell), with agent-scope stores; the
 //    consumer lanes load them two steps ahead, re-read values that are still the pending pattern
 //    (bounded waits, error word) and store the pending pattern back once they have read a value,
 //    so the exchange is ready for the next sweep without a fill.
-//  * Pencils are taken from a counter in diagonal order: a pencil only waits on pencils handed
-//    out before it, so there is no deadlock whatever the dispatch order.
+//  * Pencils are taken from a counter in diagonal order (reversed for the upper sweep): a pencil
+//    only waits on pencils handed out before it, so there is no deadlock whatever the dispatch
+//    order.
 // Every cell runs the expressions of k_lu_block in the same order, so the results are the
 // reference's bit for bit.
-//
-// Requirements, checked by the host: a Cartesian zone, one ghost cell at least beyond the LU box
-// at both ends in i (a pair may straddle the box end by one cell).
 
-// lane-owned ring of two pairs: pair m holds, for lane L, the two cells of its line that it
-// handles at steps 2m and 2m + 1 of a window of four steps, in memory order; arrays 0-4 are the
-// right-hand side (dW* in the upper sweep), 5-9 the state, 10-14 coe
-struct LuStPair { double v[15][64][2]; };
-
+// doubles per row and lane of the skewed copy: right-hand side (dW* after the lower sweep) 0-4,
+// state 5-9, coe1..4 / 2 10-13 (the sweep's sign products up to the sign, formed by the same
+// multiplications), 1 / coe5 14, coe5 15
+constexpr int kLuSkF = 16;
+// rows in flight per wave
+constexpr int kLuStAhead = 8;
 // steps of slack a consumer lane builds up once it had to wait for the previous pencil
-constexpr int kLuStSlack = 6;
+constexpr int kLuStSlack = 3;
 
-// exchange slot of value e of cell ii of lane l8 on the j (face 0) or k (face 1) side of tile t
+__device__ __forceinline__ long lu_sk_row(int t, int r, int R) { return ((long)t * R + r) * kLuSkF * 64; }
+
+long lu_skew_size(int ni, int nj, int nk) {
+    long nt = (long)((nj + 7) / 8) * ((nk + 7) / 8);
+    return nt * (ni + 14) * kLuSkF * 64;
+}
+
+// exchange slot of value e of cell ii of lane l8 on the j (face 0) or k (face 1) side of the
+// consuming pencil t.  The eight lanes of a face handle cells with the same ii + l8 at one step
+// (in either sweep), so the slots are ordered by ii + l8, then value, then lane: a step reads and
+// writes 64 contiguous bytes per value.
 __device__ __forceinline__ long lu_st_slot(int face, int t, int l8, int e, int ii, int ntile, int ni) {
-    return (long)face * ntile * 40 * ni + ((long)(t * 8 + l8) * 5 + e) * ni + ii;
+    return ((((long)face * ntile + t) * (ni + 7) + ii + l8) * 5 + e) * 8 + l8;
 }
 
 long lu_stream_exchange_size(int ni, int nj, int nk) {
     long nt = (long)((nj + 7) / 8) * ((nk + 7) / 8);
-    return 2 * nt * 40 * (long)ni;
+    return 2 * nt * 40 * (long)(ni + 7);
 }
 
-// the value of lane L - 1 (row shift by one inside rows of 16 lanes: lanes 0 and 8 of a row
-// keep their own value, they take the j neighbour from the exchange)
-__device__ __forceinline__ double lu_from_jm1(double v) {
+// the value of the lane before this one in j in sweep order: L - 1 (lower) or L + 1 (upper), by
+// a row shift inside rows of 16 lanes (lanes on the first line of the pencil get a value they do
+// not use: they take the j neighbour from the exchange)
+template <bool LOWER>
+__device__ __forceinline__ double lu_from_jprev(double v) {
+    constexpr int ctl = LOWER ? 0x111 : 0x101;   // row_shr:1 / row_shl:1
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x111, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x111, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, ctl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), ctl, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-// the value of lane L - 8 (lanes 0..7 read their own value; they take the k neighbour from the
-// exchange)
-__device__ __forceinline__ double lu_from_km1(double v, int lane) {
+// the value of the lane before this one in k in sweep order: L - 8 (lower) or L + 8 (upper);
+// lanes on the first k line read their own value (they take the k neighbour from the exchange)
+template <bool LOWER>
+__device__ __forceinline__ double lu_from_kprev(double v, int lane) {
     const long long b = __double_as_longlong(v);
-    const int src = (lane >= 8 ? lane - 8 : lane) << 2;
+    const int src = (LOWER ? (lane >= 8 ? lane - 8 : lane) : (lane < 56 ? lane + 8 : lane)) << 2;
     const int lo = __builtin_amdgcn_ds_bpermute(src, (int)b);
     const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-typedef __attribute__((address_space(3))) void lds_void;
+// The skewed copy: workgroup (t, y, f) writes array f of rows [64 y, 64 y + 64) of pencil t.  The
+// cells those rows hold (78 per line) go through LDS, so both the natural-layout reads (along i)
+// and the row writes (along the lanes) are contiguous.
+__global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
+    __shared__ double T[64][79];
+    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
+    const int jt = t % ntj, kt = t / ntj;
+    const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
+    const long nd = z.nd;
+    const double* __restrict__ src = f < 5 ? a.dw + f * nd : f < 10 ? a.rop + (f - 5) * nd
+                                   : z.coe + (f < 14 ? f - 10 : 4) * nd;
+    for (int e = threadIdx.x; e < 64 * 78; e += 256) {
+        const int L = e / 78, x = e % 78, i = r0 - 14 + x;
+        const int oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
+        double v = 0.;
+        if (oj < nj && ok < nk && i >= 0 && i < ni) {
+            v = src[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)];
+            if (f >= 10 && f < 14) v = v * 0.5;
+            else if (f == 14) v = 1. / v;
+        }
+        T[L][x] = v;
+    }
+    __syncthreads();
+    double* __restrict__ dst = a.skw;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int rr = e >> 6, L = e & 63, r = r0 + rr;
+        if (r < R) dst[lu_sk_row(t, r, R) + f * 64 + L] = T[L][rr + 14 - (L & 7) - (L >> 3)];
+    }
+}
+
+// The results back to the zone arrays: workgroup (t, y, f) copies array f (dW 0-4, the updated
+// state 5-9) of rows [64 y, 64 y + 64) of pencil t, through LDS so that the row reads (along the
+// lanes) and the zone writes (along i) are contiguous.
+__global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj) {
+    __shared__ double T[64][65];
+    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
+    const int jt = t % ntj, kt = t / ntj;
+    const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
+    const long nd = z.nd;
+    const double* __restrict__ src = a.skw;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int rr = e >> 6, L = e & 63, r = r0 + rr;
+        T[L][rr] = r < R ? src[lu_sk_row(t, r, R) + f * 64 + L] : 0.;
+    }
+    __syncthreads();
+    double* __restrict__ dst = f < 5 ? a.dw + f * nd : a.out + (f - 5) * nd;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int L = e >> 6, x = e & 63, i = r0 + x - (L & 7) - (L >> 3);
+        const int oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
+        if (oj < nj && ok < nk && i >= 0 && i < ni && r0 + x < R) dst[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] = T[L][x];
+    }
+}
 
 template <bool LOWER>
 __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int ntk) {
-    __shared__ LuStPair P0, P1;
     const int L = threadIdx.x, jj = L & 7, kk = L >> 3;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    const int R = ni + 14;
     const int ntile = ntj * ntk;
-    const long nd = z.nd;
     const LUC<> c = lu_const<double>(z);
-    const double signe = LOWER ? 0.5 : -0.5;
-    const double* __restrict__ rop = a.rop;
-    const double* __restrict__ coe = z.coe;
-    double* __restrict__ dw = a.dw;
     double* __restrict__ X = a.stx;
+    double* __restrict__ SK = a.skw;
     int* err = a.sync + 1;
     const bool kalias = !LOWER && a.rop_k != a.rop;
     const bool newton = !LOWER && a.newton;
     const double tci = z.ti[0], tcj = z.tj[0], tck = z.tk[0];
     const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
     const double pending = __longlong_as_double(kLuPending);
+    // first and last line of a pencil in sweep order
+    const int first = LOWER ? 0 : 7, last = LOWER ? 7 : 0;
     for (;;) {
         int tk0 = 0;
         if (L == 0) tk0 = atomicAdd(a.sync, 1);
-        const int t = __shfl(tk0, 0, 64);
+        const int t = __builtin_amdgcn_readfirstlane(tk0);
         if (t >= ntile) break;
         int jt, kt;
         lu_tile_of(t, ntj, ntk, jt, kt);
+        if (!LOWER) { jt = ntj - 1 - jt; kt = ntk - 1 - kt; }
         const int tileid = jt + ntj * kt;
         const int oj = jt * 8 + jj, ok = kt * 8 + kk;
         const bool lane_ok = oj < nj && ok < nk;
-        const int j = LOWER ? a.j0 + oj : a.j1 - oj, k = LOWER ? a.k0 + ok : a.k1 - ok;
-        // memory index of the lane's cell ii: lb + ii (lower sweep) or lb - ii (upper sweep)
-        const long lb = lane_ok ? IDX(z, LOWER ? a.i0 : a.i1, j, k) : 0;
-        const int nsteps = ni + 14;
-        const int nwin = (nsteps + 3) / 4;
-        // the products of this lane's line feed the next pencil in j (k)
-        const bool put_j = jj == 7 && oj + 1 < nj && lane_ok;
-        const bool put_k = kk == 7 && ok + 1 < nk && lane_ok;
-        // the lanes of the first line in j (k) take the j (k) products of the previous pencil
-        const bool get_j = jj == 0 && jt > 0 && lane_ok;
-        const bool get_k = kk == 0 && kt > 0 && lane_ok;
-        double* hjp = X + lu_st_slot(0, get_j ? (jt - 1) + ntj * kt : 0, kk, 0, 0, ntile, ni);
-        double* hkp = X + lu_st_slot(1, get_k ? jt + ntj * (kt - 1) : 0, jj, 0, 0, ntile, ni);
+        // the lane's neighbours in j and k before it in sweep order, inside the LU box
+        const bool hjn = LOWER ? oj > 0 : oj + 1 < nj;
+        const bool hkn = LOWER ? ok > 0 : ok + 1 < nk;
+        // the lanes of the first line in j (k) take the products of the previous pencil; those of
+        // the last line hand theirs to the next pencil
+        const bool get_j = lane_ok && jj == first && hjn;
+        const bool get_k = lane_ok && kk == first && hkn;
+        const bool put_j = lane_ok && jj == last && (LOWER ? oj + 1 < nj : oj > 0);
+        const bool put_k = lane_ok && kk == last && (LOWER ? ok + 1 < nk : ok > 0);
+        const int tj_next = put_j ? (LOWER ? jt + 1 : jt - 1) + ntj * kt : 0;
+        const int tk_next = put_k ? jt + ntj * (LOWER ? kt + 1 : kt - 1) : 0;
+        // value e of cell ii on the lane's face: hjp + (ii * 5 + e) * 8 (hkp)
+        double* hjp = X + lu_st_slot(0, tileid, kk, 0, 0, ntile, ni);
+        double* hkp = X + lu_st_slot(1, tileid, jj, 0, 0, ntile, ni);
+        // the lane's cell at step s
+        auto cell = [&](int s) { return (LOWER ? s : R - 1 - s) - jj - kk; };
 
-        // pair m of window w: cells c0 = 4 w + 2 m - jj - kk and c0 + 1 of the lane's line; all
-        // lanes load (lanes without a cell there read the start of the arrays, never used)
-        auto dma = [&](int w, int m) {
-            const int c0 = 4 * w + 2 * m - jj - kk;
-            const bool any = lane_ok && c0 + 1 >= 0 && c0 < ni;
-            const long lm = any ? (LOWER ? lb + c0 : lb - (c0 + 1)) : 0;
-            LuStPair& Pm = m ? P1 : P0;
+        // the rows of steps s .. s + kLuStAhead - 1 in flight (slot s % kLuStAhead); the lower sweep
+        // does not read the density
+        double in[kLuStAhead][kLuSkF];
+        auto rows = [&](int s, int q) {
+            const int sc = s < R ? s : R - 1;
+            const double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? sc : R - 1 - sc, R) + L;
 #pragma unroll
-            for (int e = 0; e < 5; e++) {
-                __builtin_amdgcn_global_load_lds(dw + lm + e * nd, (lds_void*)&Pm.v[e][0][0], 16, 0, 0);
-                __builtin_amdgcn_global_load_lds(rop + lm + e * nd, (lds_void*)&Pm.v[5 + e][0][0], 16, 0, 0);
-                __builtin_amdgcn_global_load_lds(coe + lm + e * nd, (lds_void*)&Pm.v[10 + e][0][0], 16, 0, 0);
-            }
+            for (int f = 0; f < kLuSkF; f++)
+                if (!(LOWER && f == 5)) in[q][f] = p[f * 64];
         };
         // halo products of step s, two steps ahead (buffer s & 1), loaded by the lanes that take
         // them (the exchange is read by those lanes only: no other lane touches it)
         double hj[2][5], hk[2][5];
         auto hload = [&](int s, int bsel) {
-            const int ii = s - jj - kk;
-            const bool v = ii >= 0 && ii < ni;
+            const int ii = cell(s);
+            const bool v = s < R && ii >= 0 && ii < ni;
             if (get_j && v) {
 #pragma unroll
-                for (int e = 0; e < 5; e++) hj[bsel][e] = lu_ldc(hjp + e * ni + ii);
+                for (int e = 0; e < 5; e++) hj[bsel][e] = lu_ldc(hjp + (ii * 5 + e) * 8);
             }
             if (get_k && v) {
 #pragma unroll
-                for (int e = 0; e < 5; e++) hk[bsel][e] = lu_ldc(hkp + e * ni + ii);
+                for (int e = 0; e < 5; e++) hk[bsel][e] = lu_ldc(hkp + (ii * 5 + e) * 8);
             }
         };
         // the products of the previous step: i (own next cell), and the j and k products of the
-        // lanes below, already moved
+        // lanes before, already moved
         double pi[5], bj[5], bk[5];
 #pragma unroll
         for (int e = 0; e < 5; e++) { pi[e] = 0.; bj[e] = 0.; bk[e] = 0.; }
 
-        dma(0, 0);
-        dma(0, 1);
+#pragma unroll
+        for (int q = 0; q < kLuStAhead; q++) rows(q, q);
         hload(0, 0);
         hload(1, 1);
-        for (int win = 0; win < nwin; win++) {
+        for (int s0 = 0; s0 < R; s0 += kLuStAhead) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int s = win * 4 + q;
-                const int ii = s - jj - kk;
-                const bool act = s < nsteps && lane_ok && ii >= 0 && ii < ni;
+            for (int q = 0; q < kLuStAhead; q++) {
+                const int s = s0 + q;
+                const int ii = cell(s);
+                const bool act = s < R && lane_ok && ii >= 0 && ii < ni;
                 const int hb = q & 1;
-                // the inputs of the step (lanes without a cell read stale values, never used)
-                const LuStPair& Pm = q < 2 ? P0 : P1;
-                const int sl = LOWER ? (q & 1) : 1 - (q & 1);
-                double x[5], wv[5], cf[5];
+                // the inputs of the step (lanes without a cell read zeros or stale values, never
+                // used)
+                double x[5], wv[5];
 #pragma unroll
                 for (int e = 0; e < 5; e++) {
-                    x[e] = Pm.v[e][L][sl];
-                    wv[e] = Pm.v[5 + e][L][sl];
-                    cf[e] = Pm.v[10 + e][L][sl];
+                    x[e] = in[q][e];
+                    wv[e] = LOWER && e == 0 ? 0. : in[q][5 + e];
                 }
-                // the sweep's divisor, sign products and scaled right-hand side
-                const double xal = cf[0] * signe, diag = 1. / cf[4];
-                const double ci = cf[1] * signe, cj = cf[2] * signe, ck = cf[3] * signe;
+                // the sweep's divisor, sign products (coe * 0.5 negated for the upper sweep equals
+                // coe * -0.5) and scaled right-hand side
+                const double xal = LOWER ? in[q][10] : -in[q][10], diag = in[q][14];
+                const double ci = LOWER ? in[q][11] : -in[q][11], cj = LOWER ? in[q][12] : -in[q][12];
+                const double ck = LOWER ? in[q][13] : -in[q][13];
                 if (!LOWER) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) x[e] = x[e] * cf[4];
+                    for (int e = 0; e < 5; e++) x[e] = x[e] * in[q][15];
                 }
                 // halo products of the previous pencils: while any lane still holds the pending
                 // pattern, the wave re-reads its values (bounded)
@@ -1017,15 +1092,15 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                         // order), so that the next steps find their values in place instead of
                         // waiting one round trip each
                         {
-                            const int iia = min(ii + kLuStSlack, ni - 1);
-                            const double* pa = uj ? hjp + 4 * ni + iia : hkp + 4 * ni + iia;
+                            const int iia = LOWER ? min(ii + kLuStSlack, ni - 1) : max(ii - kLuStSlack, 0);
+                            const double* pa = (uj ? hjp : hkp) + (iia * 5 + 4) * 8;
                             const bool wa = uj || uk;
                             double va = wa ? lu_ldc(pa) : 0.;
                             while (__builtin_amdgcn_ballot_w64(wa && __double_as_longlong(va) == kLuPending)) {
                                 if (++spins > kLuSpinMax ||
                                     ((spins & 63) == 1 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
                                     break;
-                                __builtin_amdgcn_s_sleep(2);
+                                __builtin_amdgcn_s_sleep(16);
                                 if (wa) va = lu_ldc(pa);
                             }
                             hload(s + 1, hb ^ 1);
@@ -1036,7 +1111,7 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                                 if (L == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                 break;
                             }
-                            __builtin_amdgcn_s_sleep(1);
+                            __builtin_amdgcn_s_sleep(8);
                             hload(s, hb);
                             // every access of the wave done: the steps after this rare path keep
                             // exact memory counters
@@ -1046,10 +1121,10 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 }
 #pragma unroll
                 for (int e = 0; e < 5; e++) {
-                    bj[e] = jj == 0 ? hj[hb][e] : bj[e];
-                    bk[e] = kk == 0 ? hk[hb][e] : bk[e];
+                    bj[e] = jj == first ? hj[hb][e] : bj[e];
+                    bk[e] = kk == first ? hk[hb][e] : bk[e];
                 }
-                const bool hi = ii > 0, hjn = oj > 0, hkn = ok > 0;
+                const bool hi = LOWER ? ii > 0 : ii + 1 < ni;
                 // neighbour terms in the reference order: lower J, K, I; upper K, J, I
 #pragma unroll
                 for (int qq = 0; qq < 3; qq++) {
@@ -1065,12 +1140,12 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
 #pragma unroll
                 for (int e = 0; e < 5; e++) x[e] = x[e] * diag;
                 const double uu = wv[1], vv = wv[2], ww = wv[3], tt = wv[4];
-                // the i and j products (the j product moves to lane L + 1 by DPP)
+                // the i and j products (the j product moves one lane by DPP)
                 double pk[5], pj[5];
                 lu_prod<true, 0, double>(c, uu, vv, ww, tt, ci, tci, 0., 0., x, pi);
                 lu_prod<true, 1, double>(c, uu, vv, ww, tt, cj, tcj, 0., 0., x, pj);
 #pragma unroll
-                for (int e = 0; e < 5; e++) bj[e] = lu_from_jm1(pj[e]);
+                for (int e = 0; e < 5; e++) bj[e] = lu_from_jprev<LOWER>(pj[e]);
                 double o[5] = {0., 0., 0., 0., 0.};
                 if (newton) {
                     // mjr_newton.for: the Newton-updated state; in place (kalias) the k products
@@ -1083,44 +1158,45 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                     const double t1 = (roe_old + x[4]) * r_1 * cvinv - cvinv2 * (u1 * u1 + v1 * v1 + w1 * w1);
                     o[0] = r1; o[1] = u1; o[2] = v1; o[3] = w1; o[4] = t1;
                 }
-                // the k product (moves to lane L + 8 by ds_bpermute)
+                // the k product (moves eight lanes by ds_bpermute)
                 if (newton && kalias)
                     lu_prod<true, 2, double>(c, o[1], o[2], o[3], o[4], ck, tck, 0., 0., x, pk);
                 else
                     lu_prod<true, 2, double>(c, uu, vv, ww, tt, ck, tck, 0., 0., x, pk);
 #pragma unroll
-                for (int e = 0; e < 5; e++) bk[e] = lu_from_km1(pk[e], L);
+                for (int e = 0; e < 5; e++) bk[e] = lu_from_kprev<LOWER>(pk[e], L);
                 if (act) {
-                    const long l = LOWER ? lb + ii : lb - ii;
+                    // lower: dW* into the right-hand-side row the upper sweep reads; upper: dW and
+                    // the updated state into the row just read
+                    double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? s : R - 1 - s, R) + L;
 #pragma unroll
-                    for (int e = 0; e < 5; e++) dw[l + e * nd] = x[e];
+                    for (int e = 0; e < 5; e++) p[e * 64] = x[e];
                     if (newton) {
-                        double* __restrict__ O = a.out;
 #pragma unroll
-                        for (int e = 0; e < 5; e++) O[l + e * nd] = o[e];
+                        for (int e = 0; e < 5; e++) p[(5 + e) * 64] = o[e];
                     }
                 }
                 // the face products for the next pencils, and the consumed halo slots back to
                 // pending for the next sweep
                 if (put_j && act) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(0, tileid, kk, e, ii, ntile, ni), pj[e]);
+                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(0, tj_next, kk, e, ii, ntile, ni), pj[e]);
                 }
                 if (put_k && act) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(1, tileid, jj, e, ii, ntile, ni), pk[e]);
+                    for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(1, tk_next, jj, e, ii, ntile, ni), pk[e]);
                 }
                 if (uj) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(hjp + e * ni + ii, pending);
+                    for (int e = 0; e < 5; e++) lu_stc(hjp + (ii * 5 + e) * 8, pending);
                 }
                 if (uk) {
 #pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(hkp + e * ni + ii, pending);
+                    for (int e = 0; e < 5; e++) lu_stc(hkp + (ii * 5 + e) * 8, pending);
                 }
                 hload(s + 2, hb);
-                // both cells of the pair read: refill it with the cells four steps ahead
-                if ((q & 1) && win + 1 < nwin) dma(win + 1, q >> 1);
+                // the row kLuStAhead steps ahead into the slot just read
+                rows(s + kLuStAhead, q);
             }
         }
     }
@@ -1189,9 +1265,11 @@ void flow(const KZ& z, const LuArgs& a, hipStream_t st) {
     (void)nx;
     hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.fx, 0L, kLuPending, a.sync);
     if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, true, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    else if (a.flow_step) hipLaunchKernelGGL((k_lu_flow<CART, true, CI, double, true>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     else hipLaunchKernelGGL((k_lu_flow<CART, true, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.fx, 0L, kLuPending, a.sync);
     if (a.fp32) hipLaunchKernelGGL((k_lu_flow<CART, false, CI, float>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
+    else if (a.flow_step) hipLaunchKernelGGL((k_lu_flow<CART, false, CI, double, true>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
     else hipLaunchKernelGGL((k_lu_flow<CART, false, CI>), dim3(grid), dim3(256), 0, st, z, a, nbi, nbj, nbk);
 }
 
@@ -1199,12 +1277,14 @@ void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
 
-// Streaming sweeps of a Cartesian zone: the tile counter reset by a one-thread fill, then one
-// launch per sweep with as many workgroups as can be resident at once (at most one per pencil).
+// Streaming sweeps of a Cartesian zone: the skewed copy of the inputs, then for each sweep the tile
+// counter reset by a one-thread fill and one launch with as many workgroups as can be resident at
+// once (at most one per pencil).
 void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
-    (void)ni;
     int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)kLuSkF),
+                       dim3(256), 0, st, z, a, ntj);
     int cus = 0, dev = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
@@ -1216,6 +1296,8 @@ void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k_lu_stream<true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
     hipLaunchKernelGGL((k_lu_stream<false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
+    hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), a.newton ? 10u : 5u),
+                       dim3(256), 0, st, z, a, ntj);
 }
 
 template <bool CART>
@@ -1321,10 +1403,8 @@ long lu_flow_exchange_size(bool cart, int ni, int nj, int nk) {
 
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
-    if (a.schedule == LU_STREAM && z.typ == 2 && !a.fp32) {
-        // a pair of cells may straddle each end of the box in i by one cell
-        if (a.i0 - 1 >= 1 - z.nf && a.i1 + 1 <= z.ni + z.nf) stream(z, a, st);
-        else { sweep<true, true>(z, a, st); sweep<true, false>(z, a, st); }
+    if (a.schedule == LU_STREAM && z.typ == 2 && !a.fp32 && a.skw) {
+        stream(z, a, st);
         return;
     }
     if (a.schedule == LU_DATAFLOW || a.schedule == LU_STREAM) {
