@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal timing")
     ap.add_argument("--phase-reps", type=int, default=5)
-    ap.add_argument("--lu", choices=["pipeline", "blocks", "dataflow", "stream"], default="dataflow")
+    ap.add_argument("--lu", choices=["pipeline", "blocks", "dataflow", "stream"], default="stream")
     ap.add_argument("--residual", choices=["auto", "cells", "faces", "kcol"], default="auto")
     ap.add_argument("--lu-precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp32: mixed precision, LU-SGS sweep arithmetic in FP32 (state, residual FP64)")

@@ -138,7 +138,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     HIPCHK(hipSetDevice(device));
     fm_ctx* c = new fm_ctx();
     c->device = device;
-    c->lu_schedule = LU_DATAFLOW;
+    c->lu_schedule = LU_STREAM;
     c->res_schedule = RES_AUTO;
     c->lu_batch = 1;
     c->lu_fp32 = 0;

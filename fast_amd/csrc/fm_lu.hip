@@ -918,7 +918,8 @@ __device__ __forceinline__ double lu_from_kprev(double v, int lane) {
 
 // The skewed copy: workgroup (t, y, f) writes array f of rows [64 y, 64 y + 64) of pencil t.  The
 // cells those rows hold (78 per line) go through LDS, so both the natural-layout reads (along i)
-// and the row writes (along the lanes) are contiguous.
+// and the row writes (along the lanes) are contiguous; every thread issues all its loads before it
+// waits for one.
 __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     __shared__ double T[64][79];
     const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
@@ -927,22 +928,37 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     const long nd = z.nd;
     const double* __restrict__ src = f < 5 ? a.dw + f * nd : f < 10 ? a.rop + (f - 5) * nd
                                    : z.coe + (f < 14 ? f - 10 : 4) * nd;
-    for (int e = threadIdx.x; e < 64 * 78; e += 256) {
-        const int L = e / 78, x = e % 78, i = r0 - 14 + x;
-        const int oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
-        double v = 0.;
-        if (oj < nj && ok < nk && i >= 0 && i < ni) {
-            v = src[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)];
-            if (f >= 10 && f < 14) v = v * 0.5;
-            else if (f == 14) v = 1. / v;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // wave w reads lines w, w + 4, ...: lane x takes cells r0 - 14 + x and r0 + 50 + x (x < 14)
+    double v[16][2];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int L = w + 4 * m, oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
+        const bool line = oj < nj && ok < nk;
+        const long l0 = line ? IDX(z, a.i0, a.j0 + oj, a.k0 + ok) : 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = r0 - 14 + lane + 64 * h;
+            v[m][h] = line && (h == 0 || lane < 14) && i >= 0 && i < ni ? src[l0 + i] : 0.;
         }
-        T[L][x] = v;
+    }
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int L = w + 4 * m;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            double x = v[m][h];
+            if (f >= 10 && f < 14) x = x * 0.5;
+            else if (f == 14) x = 1. / x;
+            if (h == 0 || lane < 14) T[L][lane + 64 * h] = x;
+        }
     }
     __syncthreads();
-    double* __restrict__ dst = a.skw;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int rr = e >> 6, L = e & 63, r = r0 + rr;
-        if (r < R) dst[lu_sk_row(t, r, R) + f * 64 + L] = T[L][rr + 14 - (L & 7) - (L >> 3)];
+    double* __restrict__ dst = a.skw + f * 64 + lane;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int rr = w + 4 * m, r = r0 + rr;
+        if (r < R) dst[lu_sk_row(t, r, R)] = T[lane][rr + 14 - (lane & 7) - (lane >> 3)];
     }
 }
 
@@ -955,17 +971,24 @@ __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj) {
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const long nd = z.nd;
-    const double* __restrict__ src = a.skw;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int rr = e >> 6, L = e & 63, r = r0 + rr;
-        T[L][rr] = r < R ? src[lu_sk_row(t, r, R) + f * 64 + L] : 0.;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double* __restrict__ src = a.skw + f * 64 + lane;
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int r = r0 + w + 4 * m;
+        v[m] = r < R ? src[lu_sk_row(t, r, R)] : 0.;
     }
+#pragma unroll
+    for (int m = 0; m < 16; m++) T[lane][w + 4 * m] = v[m];
     __syncthreads();
     double* __restrict__ dst = f < 5 ? a.dw + f * nd : a.out + (f - 5) * nd;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int L = e >> 6, x = e & 63, i = r0 + x - (L & 7) - (L >> 3);
-        const int oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
-        if (oj < nj && ok < nk && i >= 0 && i < ni && r0 + x < R) dst[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] = T[L][x];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int L = w + 4 * m, oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
+        const int i = r0 + lane - (L & 7) - (L >> 3);
+        if (oj < nj && ok < nk && i >= 0 && i < ni && r0 + lane < R)
+            dst[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] = T[L][lane];
     }
 }
 

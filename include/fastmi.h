@@ -183,21 +183,21 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_OPT_LU_SCHEDULE 1
 #define FM_LU_BLOCKS 0
 #define FM_LU_PIPELINE 1
-/* FM_LU_DATAFLOW (default): one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
+/* FM_LU_DATAFLOW: one launch per sweep; the 8x8x8 blocks are handed out in diagonal order and
    each starts as soon as its three neighbour blocks have handed over their face products.  The
    waits are bounded: one that gives up sets an error word, the sweeps still run to their end
    (with wrong values), and the next call that waits for the stream -- fm_sync, fm_zone_substep,
    fm_zone_get_field, fm_residual_norms, fm_compute_timed, fm_time_phase -- returns FM_E_HIP and
    clears the word (fm_compute itself is asynchronous and returns FM_OK).  With several zones
-   and FM_OPT_ZONE_BATCH on, the sweeps run as the batched blocks schedule. */
+   and FM_OPT_ZONE_BATCH on, the sweeps of all zones run as one dataflow launch per sweep. */
 #define FM_LU_DATAFLOW 2
-/* FM_LU_STREAM: Cartesian zones; one wave per pencil of 8 x 8 (j, k) lines sweeps the whole i
-   extent as a skewed wavefront (neighbour products move between lanes by DPP and ds_bpermute,
-   inputs reach LDS by direct loads) and hands its face products to the next pencils through a
-   global exchange (bounded waits, reported as above).  Fastest on LU boxes of a few pencils
-   (about twice the dataflow schedule on one pencil); slower than the dataflow schedule on large
-   cross-sections, where its line-by-line memory accesses thrash the caches.  Curvilinear zones
-   and FP32 sweeps run the dataflow schedule. */
+/* FM_LU_STREAM (default): on Cartesian zones, one wave per pencil of 8 x 8 (j, k) lines sweeps
+   the whole i extent as a skewed wavefront (neighbour products move between lanes by DPP and
+   ds_bpermute) and hands its face products to the next pencils through a global exchange
+   (bounded waits, reported as above).  Its inputs are first copied into a skewed, pencil-major
+   layout (one contiguous row per step and array) and its results copied back; the zone needs
+   16 doubles per cell more device memory for that copy.  Curvilinear zones, FP32 sweeps and
+   batched zones run the dataflow schedule. */
 #define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
