@@ -870,9 +870,11 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 // multiplications), 1 / coe5 14, coe5 15
 constexpr int kLuSkF = 16;
 // rows in flight per wave
-constexpr int kLuStAhead = 8;
+constexpr int kLuStAhead = 4;
 // steps of slack a consumer lane builds up once it had to wait for the previous pencil
-constexpr int kLuStSlack = 3;
+constexpr int kLuStSlack = 1;
+// steps ahead at which a lane loads the products of the previous pencil (divides kLuStAhead)
+constexpr int kLuStHalo = 2;
 
 __device__ __forceinline__ long lu_sk_row(int t, int r, int R) { return ((long)t * R + r) * kLuSkF * 64; }
 
@@ -916,12 +918,12 @@ __device__ __forceinline__ double lu_from_kprev(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// The skewed copy: workgroup (t, y, f) writes array f of rows [64 y, 64 y + 64) of pencil t.  The
-// cells those rows hold (78 per line) go through LDS, so both the natural-layout reads (along i)
-// and the row writes (along the lanes) are contiguous; every thread issues all its loads before it
-// waits for one.
+// The skewed copy: workgroup (t, y, f) writes array f of rows [64 y, 64 y + 64) of pencil t.  Line
+// L of the pencil contributes its cells r0 - d .. r0 - d + 63 (d = jj + kk) to those rows: a
+// contiguous run per line, staged through LDS so that the row writes (along the lanes) are
+// contiguous too; every thread issues all its loads before it waits for one.
 __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
-    __shared__ double T[64][79];
+    __shared__ double T[64][65];
     const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
@@ -929,36 +931,27 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     const double* __restrict__ src = f < 5 ? a.dw + f * nd : f < 10 ? a.rop + (f - 5) * nd
                                    : z.coe + (f < 14 ? f - 10 : 4) * nd;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // wave w reads lines w, w + 4, ...: lane x takes cells r0 - 14 + x and r0 + 50 + x (x < 14)
-    double v[16][2];
+    // wave w reads lines w, w + 4, ...: lane x takes the cell of row r0 + x
+    double v[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         const int L = w + 4 * m, oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
-        const bool line = oj < nj && ok < nk;
-        const long l0 = line ? IDX(z, a.i0, a.j0 + oj, a.k0 + ok) : 0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int i = r0 - 14 + lane + 64 * h;
-            v[m][h] = line && (h == 0 || lane < 14) && i >= 0 && i < ni ? src[l0 + i] : 0.;
-        }
+        const int i = r0 + lane - (L & 7) - (L >> 3);
+        v[m] = oj < nj && ok < nk && i >= 0 && i < ni ? src[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] : 0.;
     }
 #pragma unroll
     for (int m = 0; m < 16; m++) {
-        const int L = w + 4 * m;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            double x = v[m][h];
-            if (f >= 10 && f < 14) x = x * 0.5;
-            else if (f == 14) x = 1. / x;
-            if (h == 0 || lane < 14) T[L][lane + 64 * h] = x;
-        }
+        double x = v[m];
+        if (f >= 10 && f < 14) x = x * 0.5;
+        else if (f == 14) x = 1. / x;
+        T[w + 4 * m][lane] = x;
     }
     __syncthreads();
     double* __restrict__ dst = a.skw + f * 64 + lane;
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         const int rr = w + 4 * m, r = r0 + rr;
-        if (r < R) dst[lu_sk_row(t, r, R)] = T[lane][rr + 14 - (lane & 7) - (lane >> 3)];
+        if (r < R) dst[lu_sk_row(t, r, R)] = T[lane][rr];
     }
 }
 
@@ -1047,9 +1040,9 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
             for (int f = 0; f < kLuSkF; f++)
                 if (!(LOWER && f == 5)) in[q][f] = p[f * 64];
         };
-        // halo products of step s, two steps ahead (buffer s & 1), loaded by the lanes that take
-        // them (the exchange is read by those lanes only: no other lane touches it)
-        double hj[2][5], hk[2][5];
+        // halo products of step s, kLuStHalo steps ahead (buffer s % kLuStHalo), loaded by the
+        // lanes that take them (the exchange is read by those lanes only: no other lane touches it)
+        double hj[kLuStHalo][5], hk[kLuStHalo][5];
         auto hload = [&](int s, int bsel) {
             const int ii = cell(s);
             const bool v = s < R && ii >= 0 && ii < ni;
@@ -1070,15 +1063,28 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
 
 #pragma unroll
         for (int q = 0; q < kLuStAhead; q++) rows(q, q);
-        hload(0, 0);
-        hload(1, 1);
+#pragma unroll
+        for (int q = 0; q < kLuStHalo; q++) hload(q, q);
+        // the B matrices of the three products of the next step, formed while the current step
+        // waits on its neighbours (they do not depend on the solution; with in-place
+        // sub-iterations the k matrix takes the updated state and is formed at its step)
+        LuB<double> B0, B1, B2;
+        auto coefs = [&](int qn) {
+            const double u = in[qn][6], v = in[qn][7], w = in[qn][8], t = in[qn][9];
+            const double ci = LOWER ? in[qn][11] : -in[qn][11], cj = LOWER ? in[qn][12] : -in[qn][12];
+            const double ck = LOWER ? in[qn][13] : -in[qn][13];
+            B0 = lu_coef<true, 0, double>(c, u, v, w, t, ci, tci, 0., 0.);
+            B1 = lu_coef<true, 1, double>(c, u, v, w, t, cj, tcj, 0., 0.);
+            if (!(newton && kalias)) B2 = lu_coef<true, 2, double>(c, u, v, w, t, ck, tck, 0., 0.);
+        };
+        coefs(0);
         for (int s0 = 0; s0 < R; s0 += kLuStAhead) {
 #pragma unroll
             for (int q = 0; q < kLuStAhead; q++) {
                 const int s = s0 + q;
                 const int ii = cell(s);
                 const bool act = s < R && lane_ok && ii >= 0 && ii < ni;
-                const int hb = q & 1;
+                const int hb = q % kLuStHalo;
                 // the inputs of the step (lanes without a cell read zeros or stale values, never
                 // used)
                 double x[5], wv[5];
@@ -1090,7 +1096,6 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 // the sweep's divisor, sign products (coe * 0.5 negated for the upper sweep equals
                 // coe * -0.5) and scaled right-hand side
                 const double xal = LOWER ? in[q][10] : -in[q][10], diag = in[q][14];
-                const double ci = LOWER ? in[q][11] : -in[q][11], cj = LOWER ? in[q][12] : -in[q][12];
                 const double ck = LOWER ? in[q][13] : -in[q][13];
                 if (!LOWER) {
 #pragma unroll
@@ -1126,7 +1131,8 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                                 __builtin_amdgcn_s_sleep(16);
                                 if (wa) va = lu_ldc(pa);
                             }
-                            hload(s + 1, hb ^ 1);
+#pragma unroll
+                            for (int d = 1; d < kLuStHalo; d++) hload(s + d, (hb + d) % kLuStHalo);
                         }
                         do {
                             if (++spins > kLuSpinMax ||
@@ -1165,8 +1171,8 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 const double uu = wv[1], vv = wv[2], ww = wv[3], tt = wv[4];
                 // the i and j products (the j product moves one lane by DPP)
                 double pk[5], pj[5];
-                lu_prod<true, 0, double>(c, uu, vv, ww, tt, ci, tci, 0., 0., x, pi);
-                lu_prod<true, 1, double>(c, uu, vv, ww, tt, cj, tcj, 0., 0., x, pj);
+                lu_apply<true, 0, double>(B0, x, pi);
+                lu_apply<true, 1, double>(B1, x, pj);
 #pragma unroll
                 for (int e = 0; e < 5; e++) bj[e] = lu_from_jprev<LOWER>(pj[e]);
                 double o[5] = {0., 0., 0., 0., 0.};
@@ -1185,7 +1191,7 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 if (newton && kalias)
                     lu_prod<true, 2, double>(c, o[1], o[2], o[3], o[4], ck, tck, 0., 0., x, pk);
                 else
-                    lu_prod<true, 2, double>(c, uu, vv, ww, tt, ck, tck, 0., 0., x, pk);
+                    lu_apply<true, 2, double>(B2, x, pk);
 #pragma unroll
                 for (int e = 0; e < 5; e++) bk[e] = lu_from_kprev<LOWER>(pk[e], L);
                 if (act) {
@@ -1217,9 +1223,10 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
 #pragma unroll
                     for (int e = 0; e < 5; e++) lu_stc(hkp + (ii * 5 + e) * 8, pending);
                 }
-                hload(s + 2, hb);
+                hload(s + kLuStHalo, hb);
                 // the row kLuStAhead steps ahead into the slot just read
                 rows(s + kLuStAhead, q);
+                coefs((q + 1) % kLuStAhead);
             }
         }
     }
