@@ -37,7 +37,8 @@ KERNELS = {"coe": "k_coe", "bc": "k_bc_face",
            "lu": {"blocks": "k_lu_block (lower + upper sweeps, one launch per block diagonal)",
                   "pipeline": "k_lu_pipe (lower + upper sweeps)",
                   "dataflow": "k_lu_flow (lower + upper sweeps, one launch each)",
-                  "stream": "k_lu_stream (lower + upper sweeps, one launch each)"}}
+                  "stream": "LU-SGS phase: k_lu_pack (skewed copy of the inputs), k_lu_stream lower + "
+                            "upper, k_lu_unpack (one launch each)"}}
 
 
 def parse():

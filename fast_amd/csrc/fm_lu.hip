@@ -1031,14 +1031,14 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
         auto cell = [&](int s) { return (LOWER ? s : R - 1 - s) - jj - kk; };
 
         // the rows of steps s .. s + kLuStAhead - 1 in flight (slot s % kLuStAhead); the lower sweep
-        // does not read the density
+        // reads neither the density nor coe5
         double in[kLuStAhead][kLuSkF];
         auto rows = [&](int s, int q) {
             const int sc = s < R ? s : R - 1;
             const double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? sc : R - 1 - sc, R) + L;
 #pragma unroll
             for (int f = 0; f < kLuSkF; f++)
-                if (!(LOWER && f == 5)) in[q][f] = p[f * 64];
+                if (!(LOWER && (f == 5 || f == 15))) in[q][f] = p[f * 64];
         };
         // halo products of step s, kLuStHalo steps ahead (buffer s % kLuStHalo), loaded by the
         // lanes that take them (the exchange is read by those lanes only: no other lane touches it)
