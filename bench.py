@@ -71,6 +71,10 @@ def parse():
                     help="1 replays captured HIP graphs of the time step (FM_OPT_GRAPH)")
     ap.add_argument("--zone-batch", type=int, default=1, choices=[0, 1],
                     help="zones64: 1 batches the LU-SGS of all zones per block diagonal, 0 loops")
+    ap.add_argument("--mixed-report", action="store_true",
+                    help="FP64 against FP32 LU-SGS sweeps: Newton convergence of one BDF step to 1e-10 on "
+                         "TGV 256^3 and the 513x257x65 channel (time, sub-iterations, state and KE errors)")
+    ap.add_argument("--mixed-nmax", type=int, default=200, help="sub-iteration limit of --mixed-report")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB MiB device buffer 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -341,6 +345,93 @@ def run_other(a):
     print(json.dumps(out))
 
 
+def _newton_to(ctx, zid, nmax, tol):
+    """Sub-iterations of one BDF step until the L2 norm of the Newton residual is tol times its
+    first value: (sub-iterations, seconds of GPU work including one norm read per sub-iteration,
+    L2 history relative to the first)."""
+    hist, first, secs = [], None, 0.
+    for it in range(1, nmax + 1):
+        t0 = time.perf_counter()
+        ctx.substep(zid, it, first_it=0, lssiter_verif=1)
+        r = ctx.residual_norms(zid)[:5]
+        secs += time.perf_counter() - t0
+        l2 = float(np.sqrt(np.sum(r ** 2)))
+        first = l2 if first is None else first
+        hist.append(l2 / first)
+        if l2 <= tol * first:
+            return it, secs, hist
+    return None, secs, hist
+
+
+def run_mixed(a):
+    """FP64 sweeps against FP32 sweeps (state, Newton residual and update FP64) on one BDF step
+    driven to a 1e-10 relative Newton residual: time, sub-iterations, and the converged state's
+    error.  The FP64 GPU path equals the oracle bit for bit (the parity tests), so the error
+    against the FP64 run is the error against the oracle."""
+    from fast_amd import _lib, cases, params as P
+    tol = 1e-10
+    out = {"metric": "time to a 1e-10 Newton residual, FP64 vs FP32 LU-SGS sweeps", "configs": []}
+    cfgs = [("TGV 256^3, ausmpred o3 laminar, BDF, global time step CFL 2", "tgv"),
+            ("channel with a bump 513x257x65 (3dfull), ausmpred o3 laminar, BDF, local time step CFL 10",
+             "channel")]
+    runs = (("fp64 (default schedule)", None, _lib.LU_FP64), ("fp64 (dataflow)", _lib.LU_DATAFLOW, _lib.LU_FP64),
+            ("fp32 sweeps (dataflow)", _lib.LU_DATAFLOW, _lib.LU_FP32))
+    for desc, kind in cfgs:
+        res = {"workload": desc, "tolerance": tol, "runs": {}}
+        states = {}
+        for name, sched, prec in runs:
+            if kind == "tgv":
+                num = P.Numerics(scheme="ausmpred", slope="o3", temporal_scheme="implicit",
+                                 ss_iteration=a.mixed_nmax, time_step_nature="global", time_step=1.0)
+                z = cases.tgv_zone((256, 256, 256), num)
+                z.param_real[P.DTC] = cases.tgv_dt(z, cfl=2.0)
+            else:
+                num = P.Numerics(scheme="ausmpred", slope="o3", temporal_scheme="implicit",
+                                 ss_iteration=a.mixed_nmax, time_step_nature="local", cfl=10.0)
+                z = cases.channel_zone(513, 257, 65, numerics=num)
+            ctx = _lib.Context(0)
+            ctx.set_option(_lib.OPT_GRAPH, 0)
+            if sched is not None:
+                ctx.set_option(_lib.OPT_LU_SCHEDULE, sched)
+            ctx.set_option(_lib.OPT_LU_PRECISION, prec)
+            zid = ctx.add_zone(z)
+            ctx.apply_bc(zid, 0)
+            buf = np.zeros_like(z.ro)
+            ctx.get(zid, _lib.FIELD_RO, buf)
+            ctx.set(zid, _lib.FIELD_RO_M1, buf)
+            ctx.set(zid, _lib.FIELD_RO_P1, buf)
+            # one untimed warm-up sub-iteration sequence on a copy of the start (first-use
+            # allocations), then the measured one from the same start
+            _newton_to(ctx, zid, 2, 0.)
+            ctx.set(zid, _lib.FIELD_RO_P1, buf)
+            n, secs, hist = _newton_to(ctx, zid, a.mixed_nmax, tol)
+            w = np.zeros_like(z.ro)
+            ctx.get(zid, _lib.FIELD_RO_P1, w)
+            states[name] = (z.interior(w), ctx.tke(zid, 2)[0])
+            ctx.close()
+            del z
+            print("mixed-report: %s, %s: %s sub-iterations, %.3f s" % (kind, name, n, secs), file=sys.stderr, flush=True)
+            res["runs"][name] = {"subiterations": n, "seconds": round(secs, 4),
+                                 "ms_per_subiteration": round(secs / len(hist) * 1e3, 3),
+                                 "final_relative_l2": hist[-1],
+                                 "subiterations_to": {"%g" % t: next((q + 1 for q, h in enumerate(hist) if h <= t), None)
+                                                      for t in (1e-4, 1e-6, 1e-8, 1e-9, 1e-10)}}
+        ref, ke_ref = states["fp64 (default schedule)"]
+        for name, (w, ke) in states.items():
+            d = w - ref
+            scale = np.abs(ref).reshape(5, -1).max(axis=1)
+            res["runs"][name]["state_error_vs_fp64"] = {
+                "l2_rel": [float(np.sqrt(np.mean(d[e] ** 2)) / scale[e]) for e in range(5)],
+                "linf_rel": [float(np.abs(d[e]).max() / scale[e]) for e in range(5)],
+                "ke_rel": float(abs(ke - ke_ref) / abs(ke_ref))}
+        f64, f32 = res["runs"]["fp64 (default schedule)"], res["runs"]["fp32 sweeps (dataflow)"]
+        if f64["subiterations"] and f32["subiterations"]:
+            res["speedup_fp32_vs_fp64_default"] = round(f64["seconds"] / f32["seconds"], 3)
+            res["speedup_fp32_vs_fp64_dataflow"] = round(res["runs"]["fp64 (dataflow)"]["seconds"] / f32["seconds"], 3)
+        out["configs"].append(res)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -349,6 +440,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.mixed_report:
+        run_mixed(a)
         return
     if a.workload != "tgv256":
         run_other(a)
