@@ -729,7 +729,9 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             {
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
                 if (int rc = ensure_dataflow(c, z, lb, la)) return rc;
-                if (int rc = ensure_stream(c, z, la)) return rc;
+                // a batched zone runs the dataflow list: no streaming buffers for it
+                if (defer) { la.stx = nullptr; la.skw = nullptr; }
+                else if (int rc = ensure_stream(c, z, la)) return rc;
             }
             la.rop = ro_ssiter;
             if (ro_ssiter == ro_p1) {
