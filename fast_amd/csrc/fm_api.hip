@@ -58,6 +58,8 @@ struct Zone {
     double* lu_fx;           // dataflow LU schedule: face exchange and block order (on first use)
     double* lu_stx;          // streaming LU schedule: pencil exchange (on first use)
     double* lu_skw;          // streaming LU schedule: skewed copy of the sweep inputs (on first use)
+    int dw_pending;          // dW of the last streaming LU-SGS is still in its skewed rows (dw_la)
+    LuArgs dw_la;
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
@@ -95,8 +97,12 @@ struct fm_ctx {
     int res_kcol_n;          // tile shapes: k-column length of RES_KCOL (0: built-in)
     int lu_flow_grid;        //   workgroups of a dataflow sweep (0: all that are resident)
     // time steps captured as HIP graphs (FM_OPT_GRAPH): one per distinct configuration
+    // inside fm_compute: the streaming LU-SGS leaves dW in its skewed rows (copied back once at
+    // the end: within a time step nothing reads it before the next residual overwrites it)
+    int defer_dw;
     struct GraphEntry {
         std::vector<const void*> ptrs;  // every zone's three level pointers
+        std::vector<int> dw_pending;    // per zone, after the time step
         int first_it, verif, n0, n1;
         long epoch;
         int state;                      // 1: ran once eagerly, 2: captured
@@ -145,6 +151,7 @@ extern "C" int fm_ctx_create(fm_ctx** out, int device) {
     c->graph_on = 1;
     c->cur = nullptr;
     c->fork_ev = nullptr;
+    c->defer_dw = 0;
     c->res_kcol_n = 0;
     c->lu_flow_grid = 0;
     if (int rc = resolve_scratch(device)) { delete c; return rc; }
@@ -253,7 +260,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
-    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
+    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->dw_pending = 0; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
     z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
@@ -707,6 +714,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
     else { a.rk_out = ro; a.rk_in = ro_p1; }
     if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
     HIPCHK(hipGetLastError());
+    z->dw_pending = 0;
     if (stop == 1) return FM_OK;
     if (implicit) {
         int shift_lu[6], lcorner = 1;
@@ -741,9 +749,11 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             } else {
                 la.rop_k = ro_ssiter; la.out = ro_p1;
             }
+            la.keep_dw = !c->defer_dw;
             if (defer) { *defer = la; return FM_OK; }
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
+            if (!la.keep_dw && la.newton && lu_streams(k, la)) { z->dw_pending = 1; z->dw_la = la; }
             if (la.out == z->ro_scratch) {
                 // copy the updated cells back (the LU box, ghost layers it extends into included)
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
@@ -1245,7 +1255,8 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
 // first_it, verification, sub-iteration range) appears, captured into a HIP graph the second
 // time, and replayed from then on.  The levels rotate with period 3, so a run settles on a
 // few graphs.
-static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
+static int compute_steps(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif,
+                         int first_it) {
     for (int it = 0; it < nit_c; it++) {
         int nitrun_loc = nitrun * nit_c + it;
         int verif = modulo_verif > 0 && (nitrun_loc % modulo_verif == 0);
@@ -1269,6 +1280,8 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
                 int rc = time_step(c, first_it, verif, nstep_deb, nstep_fin, ge, false);
                 if (rc) return rc;
                 ge->state = 1;
+                ge->dw_pending.clear();
+                for (Zone* z : c->zones) ge->dw_pending.push_back(z->dw_pending);
             } else {
                 if (ge->state == 1) {
                     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1284,6 +1297,9 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
                     ge->state = 2;
                 }
                 HIPCHK(hipGraphLaunch(ge->exec, c->stream));
+                // the host state a replayed step leaves is the one the eager step left
+                for (size_t n = 0; n < c->zones.size() && n < ge->dw_pending.size(); n++)
+                    c->zones[n]->dw_pending = ge->dw_pending[n];
             }
         }
         first_it = 1;
@@ -1295,6 +1311,21 @@ static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_
             double* m1 = z->ro[1]; z->ro[1] = z->ro[0]; z->ro[0] = z->ro[2]; z->ro[2] = m1;
         }
     }
+    return FM_OK;
+}
+
+static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
+    c->defer_dw = 1;
+    int rc = compute_steps(c, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it);
+    c->defer_dw = 0;
+    // dW of the last LU-SGS back to drodm, where the eager steps leave it
+    for (Zone* z : c->zones) {
+        if (!z->dw_pending) continue;
+        launch_lu_unpack_dw(z->k, z->dw_la, c->stream);
+        z->dw_pending = 0;
+    }
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
     return FM_OK;
 }
 
@@ -1487,6 +1518,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     if (int rc = ensure_exchange(c, z)) return rc;
     la.xch = z->lu_xch;
     la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
+    la.keep_dw = 1;
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;

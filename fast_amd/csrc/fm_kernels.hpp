@@ -63,6 +63,7 @@ struct LuArgs {
     // skewed, pencil-major copy of the sweep inputs
     double* stx;
     double* skw;
+    int keep_dw;                  // streaming: copy dW back to dw (else it stays in the skewed rows)
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
 // doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk
@@ -119,6 +120,9 @@ void launch_coe(const KZ& z, const double* rop, int first_it, int mode, Box b, i
 void launch_wiggle(const KZ& z, const double* rop, hipStream_t st);
 int launch_residual(const KZ& z, const RhsArgs& a, hipStream_t st);
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st);
+// the streaming schedule runs for these arguments; the dW its last run left in the skewed rows
+bool lu_streams(const KZ& z, const LuArgs& a);
+void launch_lu_unpack_dw(const KZ& z, const LuArgs& a, hipStream_t st);
 // diagonal-batched LU-SGS (blocks schedule) over several zones of the same mesh kind
 struct LuZone { KZ z; LuArgs a; };
 struct LuWork { int zone, bi, bj, bk; };

@@ -958,9 +958,9 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
 // The results back to the zone arrays: workgroup (t, y, f) copies array f (dW 0-4, the updated
 // state 5-9) of rows [64 y, 64 y + 64) of pencil t, through LDS so that the row reads (along the
 // lanes) and the zone writes (along i) are contiguous.
-__global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj) {
+__global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int f0) {
     __shared__ double T[64][65];
-    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
+    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = f0 + blockIdx.z;
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const long nd = z.nd;
@@ -1326,8 +1326,10 @@ void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k_lu_stream<true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
     hipLaunchKernelGGL((k_lu_stream<false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
-    hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), a.newton ? 10u : 5u),
-                       dim3(256), 0, st, z, a, ntj);
+    // dW back only if asked (inside fm_compute it is copied once, after the last time step)
+    const int f0 = a.newton && !a.keep_dw ? 5 : 0, f1 = a.newton ? 10 : 5;
+    hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(f1 - f0)),
+                       dim3(256), 0, st, z, a, ntj, f0);
 }
 
 template <bool CART>
@@ -1431,9 +1433,22 @@ long lu_flow_exchange_size(bool cart, int ni, int nj, int nk) {
     return nb * 5 * (64 + 16 * CI);
 }
 
+bool lu_streams(const KZ& z, const LuArgs& a) {
+    return a.schedule == LU_STREAM && z.typ == 2 && !a.fp32 && a.skw;
+}
+
+// dW of the last streaming LU-SGS, left in its skewed rows, back to the zone array
+void launch_lu_unpack_dw(const KZ& z, const LuArgs& a, hipStream_t st) {
+    if (!lu_streams(z, a) || a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
+    int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
+    int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), 5u),
+                       dim3(256), 0, st, z, a, ntj, 0);
+}
+
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
-    if (a.schedule == LU_STREAM && z.typ == 2 && !a.fp32 && a.skw) {
+    if (lu_streams(z, a)) {
         stream(z, a, st);
         return;
     }

@@ -260,6 +260,33 @@ def test_compute_loop_matches_oracle(ctxmod, orc):
     assert np.array_equal(zo.ro, zg.ro)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_compute_leaves_dw_like_eager_sweeps(ctxmod, graph):
+    """Inside fm_compute the streaming LU-SGS leaves dW in its skewed rows and copies it to drodm
+    once, after the last time step (graph replays included): every field, drodm too, equals the
+    dataflow schedule's, which writes dW at every sweep."""
+    num = H.numerics("ausmpred", "bdf")
+    out = {}
+    for sched in (ctxmod.LU_STREAM, ctxmod.LU_DATAFLOW):
+        zg = cases.tgv_zone((20, 17, 11), num, perturb=0.02)
+        ctx = ctxmod.Context(0)
+        ctx.set_option(ctxmod.OPT_LU_SCHEDULE, sched)
+        ctx.set_option(ctxmod.OPT_GRAPH, graph)
+        zid = ctx.add_zone(zg)
+        ctx.apply_bc(zid, 0)
+        t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
+        ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
+        ctx.compute(0, 1, 1, nit_c=5, modulo_verif=0, first_it=0)
+        ctx.compute(5, 1, 1, nit_c=3, modulo_verif=2, first_it=1)
+        ctx.sync()
+        ctx.pull(zid, zg)
+        ctx.close()
+        out[sched] = zg
+    a, b = out[ctxmod.LU_STREAM], out[ctxmod.LU_DATAFLOW]
+    for f in ("ro", "ro_m1", "ro_p1", "drodm"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
 @pytest.mark.parametrize("mesh", ["cart", "full"])
 @pytest.mark.parametrize("flow", ["lamin", "euler"])
 def test_cfl_monitor(ctxmod, orc, mesh, flow):
