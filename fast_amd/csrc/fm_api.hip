@@ -60,6 +60,10 @@ struct Zone {
     double* lu_skw;          // streaming LU schedule: skewed copy of the sweep inputs (on first use)
     int dw_pending;          // dW of the last streaming LU-SGS is still in its skewed rows (dw_la)
     LuArgs dw_la;
+    // the state level whose LU-box values the state rows of the skewed copy hold (written there by
+    // the last upper sweep; nullptr: none), and that box
+    const double* w_rows;
+    Box w_box;
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
@@ -103,6 +107,8 @@ struct fm_ctx {
     struct GraphEntry {
         std::vector<const void*> ptrs;  // every zone's three level pointers
         std::vector<int> dw_pending;    // per zone, after the time step
+        std::vector<const double*> w_rows;        // per zone, before (part of the key) and after
+        std::vector<const double*> w_rows_after;
         int first_it, verif, n0, n1;
         long epoch;
         int state;                      // 1: ran once eagerly, 2: captured
@@ -260,7 +266,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
-    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->dw_pending = 0; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
+    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->dw_pending = 0; z->w_rows = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
     z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
@@ -420,6 +426,7 @@ extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* h
     if (!z || !host) return fail(FM_E_ARG, "fm_zone_set_field: bad arguments");
     long sz; double* d = field_ptr(z, field, &sz);
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
+    z->w_rows = nullptr;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -715,6 +722,7 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
     if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
     HIPCHK(hipGetLastError());
     z->dw_pending = 0;
+    if (!implicit) z->w_rows = nullptr;       // the stage update rewrites the state
     if (stop == 1) return FM_OK;
     if (implicit) {
         int shift_lu[6], lcorner = 1;
@@ -750,10 +758,23 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
                 la.rop_k = ro_ssiter; la.out = ro_p1;
             }
             la.keep_dw = !c->defer_dw;
-            if (defer) { *defer = la; return FM_OK; }
+            // the state rows already hold rop if the last upper sweep wrote them for this box and
+            // the box is interior (the ghost fill does not touch it)
+            const Box lbx = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+            const bool interior = la.i0 >= 1 && la.i1 <= k.ni && la.j0 >= 1 && la.j1 <= k.nj && la.k0 >= 1 && la.k1 <= k.nk;
+            const Box& wb = z->w_box;
+            la.w_ready = !defer && lu_streams(k, la) && interior && z->w_rows == la.rop && wb.i0 == lbx.i0 &&
+                         wb.i1 == lbx.i1 && wb.j0 == lbx.j0 && wb.j1 == lbx.j1 && wb.k0 == lbx.k0 && wb.k1 == lbx.k1;
+            if (defer) { z->w_rows = nullptr; *defer = la; return FM_OK; }
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
             if (!la.keep_dw && la.newton && lu_streams(k, la)) { z->dw_pending = 1; z->dw_la = la; }
+            if (la.newton && interior && lu_streams(k, la)) {
+                z->w_rows = la.out == z->ro_scratch ? ro_p1 : la.out;
+                z->w_box = lbx;
+            } else {
+                z->w_rows = nullptr;
+            }
             if (la.out == z->ro_scratch) {
                 // copy the updated cells back (the LU box, ghost layers it extends into included)
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
@@ -1265,14 +1286,18 @@ static int compute_steps(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, in
             if (rc) return rc;
         } else {
             std::vector<const void*> ptrs;
-            for (Zone* z : c->zones) for (int q = 0; q < 3; q++) ptrs.push_back(z->ro[q]);
+            std::vector<const double*> wr;
+            for (Zone* z : c->zones) {
+                for (int q = 0; q < 3; q++) ptrs.push_back(z->ro[q]);
+                wr.push_back(z->w_rows);
+            }
             fm_ctx::GraphEntry* ge = nullptr;
             for (auto* g : c->graphs)
-                if (g->ptrs == ptrs && g->first_it == first_it && g->verif == verif && g->n0 == nstep_deb &&
-                    g->n1 == nstep_fin && g->epoch == c->graph_epoch) { ge = g; break; }
+                if (g->ptrs == ptrs && g->w_rows == wr && g->first_it == first_it && g->verif == verif &&
+                    g->n0 == nstep_deb && g->n1 == nstep_fin && g->epoch == c->graph_epoch) { ge = g; break; }
             if (!ge) {
                 ge = new fm_ctx::GraphEntry();
-                ge->ptrs = ptrs; ge->first_it = first_it; ge->verif = verif; ge->n0 = nstep_deb;
+                ge->ptrs = ptrs; ge->w_rows = wr; ge->first_it = first_it; ge->verif = verif; ge->n0 = nstep_deb;
                 ge->n1 = nstep_fin; ge->epoch = c->graph_epoch; ge->state = 0;
                 c->graphs.push_back(ge);
             }
@@ -1281,7 +1306,8 @@ static int compute_steps(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, in
                 if (rc) return rc;
                 ge->state = 1;
                 ge->dw_pending.clear();
-                for (Zone* z : c->zones) ge->dw_pending.push_back(z->dw_pending);
+                ge->w_rows_after.clear();
+                for (Zone* z : c->zones) { ge->dw_pending.push_back(z->dw_pending); ge->w_rows_after.push_back(z->w_rows); }
             } else {
                 if (ge->state == 1) {
                     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1298,8 +1324,10 @@ static int compute_steps(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, in
                 }
                 HIPCHK(hipGraphLaunch(ge->exec, c->stream));
                 // the host state a replayed step leaves is the one the eager step left
-                for (size_t n = 0; n < c->zones.size() && n < ge->dw_pending.size(); n++)
+                for (size_t n = 0; n < c->zones.size() && n < ge->dw_pending.size(); n++) {
                     c->zones[n]->dw_pending = ge->dw_pending[n];
+                    c->zones[n]->w_rows = ge->w_rows_after[n];
+                }
             }
         }
         first_it = 1;
@@ -1519,6 +1547,8 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.xch = z->lu_xch;
     la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
     la.keep_dw = 1;
+    la.w_ready = 0;
+    z->w_rows = nullptr;                      // its sweeps write the scratch state into the rows
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;

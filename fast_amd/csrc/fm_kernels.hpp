@@ -64,6 +64,8 @@ struct LuArgs {
     double* stx;
     double* skw;
     int keep_dw;                  // streaming: copy dW back to dw (else it stays in the skewed rows)
+    int w_ready;                  // streaming: the state rows hold rop already (the last upper sweep
+                                  // wrote them and nothing changed the state since)
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
 // doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk

@@ -924,7 +924,8 @@ __device__ __forceinline__ double lu_from_kprev(double v, int lane) {
 // contiguous too; every thread issues all its loads before it waits for one.
 __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     __shared__ double T[64][65];
-    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = blockIdx.z;
+    // with a.w_ready the state rows hold the state already: arrays 0-4 and 10-15 only
+    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = a.w_ready && blockIdx.z >= 5 ? blockIdx.z + 5 : blockIdx.z;
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const long nd = z.nd;
@@ -1313,7 +1314,7 @@ void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
 void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
-    hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)kLuSkF),
+    hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(a.w_ready ? kLuSkF - 5 : kLuSkF)),
                        dim3(256), 0, st, z, a, ntj);
     int cus = 0, dev = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||

@@ -263,8 +263,9 @@ def test_compute_loop_matches_oracle(ctxmod, orc):
 @pytest.mark.parametrize("graph", [0, 1])
 def test_compute_leaves_dw_like_eager_sweeps(ctxmod, graph):
     """Inside fm_compute the streaming LU-SGS leaves dW in its skewed rows and copies it to drodm
-    once, after the last time step (graph replays included): every field, drodm too, equals the
-    dataflow schedule's, which writes dW at every sweep."""
+    once, after the last time step, and reuses the state rows its upper sweep wrote (graph replays
+    included; an edit of the state in between invalidates them): every field, drodm too, equals the
+    dataflow schedule's, which writes dW at every sweep and copies nothing."""
     num = H.numerics("ausmpred", "bdf")
     out = {}
     for sched in (ctxmod.LU_STREAM, ctxmod.LU_DATAFLOW):
@@ -277,7 +278,13 @@ def test_compute_leaves_dw_like_eager_sweeps(ctxmod, graph):
         t = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, t)
         ctx.set(zid, ctxmod.FIELD_RO_M1, t); ctx.set(zid, ctxmod.FIELD_RO_P1, t)
         ctx.compute(0, 1, 1, nit_c=5, modulo_verif=0, first_it=0)
+        # an edit of the state between two calls: the rows the streaming sweeps kept of it are
+        # stale and must be copied again
+        w = np.zeros_like(zg.ro); ctx.get(zid, ctxmod.FIELD_RO, w)
+        w *= 1.0 + 1e-3 * np.sin(np.arange(w.size))
+        ctx.set(zid, ctxmod.FIELD_RO, w)
         ctx.compute(5, 1, 1, nit_c=3, modulo_verif=2, first_it=1)
+        ctx.compute(8, 1, 1, nit_c=4, modulo_verif=0, first_it=1)
         ctx.sync()
         ctx.pull(zid, zg)
         ctx.close()
