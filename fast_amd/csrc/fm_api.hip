@@ -406,6 +406,8 @@ extern "C" int fm_zone_metric(fm_ctx* c, int zone) {
     return FM_OK;
 }
 
+static void resolve_dw(fm_ctx* c, Zone* z);
+
 static double* field_ptr(Zone* z, int field, long* n) {
     long nd = z->k.nd;
     switch (field) {
@@ -427,6 +429,7 @@ extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* h
     long sz; double* d = field_ptr(z, field, &sz);
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
     z->w_rows = nullptr;
+    if (field == FM_FIELD_DRODM) z->dw_pending = 0;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -441,6 +444,7 @@ extern "C" int fm_zone_get_field(fm_ctx* c, int zone, int field, double* host, l
     long sz; double* d = field_ptr(z, field, &sz);
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
     HIPCHK(hipSetDevice(c->device));
+    if (field == FM_FIELD_DRODM) resolve_dw(c, z);
     HIPCHK(hipStreamSynchronize(c->stream));
     // a sweep whose bounded wait gave up leaves wrong values: report it before handing them out
     if (int rc = check_lu(c)) return rc;
@@ -974,6 +978,7 @@ extern "C" int fm_flux_balance(fm_ctx* c, int zone, int level, double* drodm) {
     RhsArgs a;
     if (int rc = balance_args(z, rop, a, res_schedule_of(c, z))) return rc;
     a.kcol_n = c->res_kcol_n;
+    z->dw_pending = 0;
     HIPCHK(hipMemsetAsync(k.drodm, 0, sizeof(double) * 5 * (size_t)k.nd, st));
     if (launch_residual(k, a, st)) return fail(FM_E_UNSUPPORTED, "no residual kernel for this scheme");
     HIPCHK(hipGetLastError());
@@ -1342,16 +1347,18 @@ static int compute_steps(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, in
     return FM_OK;
 }
 
+// dW of the last streaming LU-SGS inside fm_compute is still in its skewed rows: copy it to drodm
+// before anything reads drodm (the next residual overwrites it instead)
+static void resolve_dw(fm_ctx* c, Zone* z) {
+    if (!z->dw_pending) return;
+    launch_lu_unpack_dw(z->k, z->dw_la, c->stream);
+    z->dw_pending = 0;
+}
+
 static int compute(fm_ctx* c, int nitrun, int nstep_deb, int nstep_fin, int nit_c, int modulo_verif, int first_it) {
     c->defer_dw = 1;
     int rc = compute_steps(c, nitrun, nstep_deb, nstep_fin, nit_c, modulo_verif, first_it);
     c->defer_dw = 0;
-    // dW of the last LU-SGS back to drodm, where the eager steps leave it
-    for (Zone* z : c->zones) {
-        if (!z->dw_pending) continue;
-        launch_lu_unpack_dw(z->k, z->dw_la, c->stream);
-        z->dw_pending = 0;
-    }
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     return FM_OK;
@@ -1522,6 +1529,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     Zone* z = get_zone(c, zone);
     if (!z || reps < 1 || !ms) return fail(FM_E_ARG, "fm_time_phase: bad arguments");
     HIPCHK(hipSetDevice(c->device));
+    resolve_dw(c, z);
     KZ& k = z->k;
     hipStream_t st = c->stream;
     int implicit = k.itypcp <= 1;
