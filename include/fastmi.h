@@ -198,8 +198,8 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    layout (one contiguous row per step and array) and its results copied back; the zone needs
    16 doubles per cell more device memory for that copy.  Inside fm_compute the rows are kept from
    one sweep to the next where nothing else changed them (the state rows the upper sweep writes;
-   fm_zone_set_field invalidates them) and dW is copied back to drodm once, after the last time
-   step.  Curvilinear zones, FP32 sweeps and batched zones run the dataflow schedule. */
+   fm_zone_set_field invalidates them) and dW is copied back to drodm when drodm is next read
+   (fm_zone_get_field, fm_time_phase).  Curvilinear zones, FP32 sweeps and batched zones run the dataflow schedule. */
 #define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
