@@ -856,9 +856,10 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 //    to the zone arrays.
 //  * Pencils hand their j-face (k-face) products to the next pencil in sweep order through a
 //    global exchange, one value per (pencil, lane, variable, cell), with agent-scope stores; the
-//    consumer lanes load them two steps ahead, re-read values that are still the pending pattern
-//    (bounded waits, error word) and store the pending pattern back once they have read a value,
-//    so the exchange is ready for the next sweep without a fill.
+//    consumer lanes load them two steps ahead and re-read values that are still the pending
+//    pattern (bounded waits, error word).  The exchange is filled with the pending pattern before
+//    each sweep (one fill kernel: cheaper than agent-scope stores giving every slot back, whose
+//    write-through holds up the loads issued after them).
 //  * Pencils are taken from a counter in diagonal order (reversed for the upper sweep): a pencil
 //    only waits on pencils handed out before it, so there is no deadlock whatever the dispatch
 //    order.
@@ -1000,7 +1001,6 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
     const bool newton = !LOWER && a.newton;
     const double tci = z.ti[0], tcj = z.tj[0], tck = z.tk[0];
     const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
-    const double pending = __longlong_as_double(kLuPending);
     // first and last line of a pencil in sweep order
     const int first = LOWER ? 0 : 7, last = LOWER ? 7 : 0;
     for (;;) {
@@ -1206,8 +1206,7 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                         for (int e = 0; e < 5; e++) p[(5 + e) * 64] = o[e];
                     }
                 }
-                // the face products for the next pencils, and the consumed halo slots back to
-                // pending for the next sweep
+                // the face products for the next pencils
                 if (put_j && act) {
 #pragma unroll
                     for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(0, tj_next, kk, e, ii, ntile, ni), pj[e]);
@@ -1215,14 +1214,6 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 if (put_k && act) {
 #pragma unroll
                     for (int e = 0; e < 5; e++) lu_stc(X + lu_st_slot(1, tk_next, jj, e, ii, ntile, ni), pk[e]);
-                }
-                if (uj) {
-#pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(hjp + (ii * 5 + e) * 8, pending);
-                }
-                if (uk) {
-#pragma unroll
-                    for (int e = 0; e < 5; e++) lu_stc(hkp + (ii * 5 + e) * 8, pending);
                 }
                 hload(s + kLuStHalo, hb);
                 // the row kLuStAhead steps ahead into the slot just read
@@ -1308,12 +1299,13 @@ void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
 
-// Streaming sweeps of a Cartesian zone: the skewed copy of the inputs, then for each sweep the tile
-// counter reset by a one-thread fill and one launch with as many workgroups as can be resident at
-// once (at most one per pencil).
+// Streaming sweeps of a Cartesian zone: the skewed copy of the inputs, then for each sweep the
+// exchange filled with the pending pattern (and the tile counter reset) and one launch with as many
+// workgroups as can be resident at once (at most one per pencil).
 void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
+    const long nx = lu_stream_exchange_size(ni, nj, nk);
     hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(a.w_ready ? kLuSkF - 5 : kLuSkF)),
                        dim3(256), 0, st, z, a, ntj);
     int cus = 0, dev = 0, per = 0;
@@ -1323,9 +1315,9 @@ void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
         per = 1;
     long cap = (long)per * cus;
     unsigned grid = (unsigned)((long)ntj * ntk < cap ? (long)ntj * ntk : cap);
-    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.stx, nx, kLuPending, a.sync);
     hipLaunchKernelGGL((k_lu_stream<true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
-    hipLaunchKernelGGL(k_fill_lin, dim3(1), dim3(64), 0, st, a.stx, 0L, kLuPending, a.sync);
+    hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.stx, nx, kLuPending, a.sync);
     hipLaunchKernelGGL((k_lu_stream<false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     // dW back only if asked (inside fm_compute it is copied once, after the last time step)
     const int f0 = a.newton && !a.keep_dw ? 5 : 0, f1 = a.newton ? 10 : 5;
