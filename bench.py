@@ -291,6 +291,8 @@ def run_other(a):
     ctx.set_option(_lib.OPT_ZONE_BATCH, a.zone_batch)
     ctx.set_option(_lib.OPT_GRAPH, a.graph)
     ctx.set_option(_lib.OPT_LU_PRECISION, _lib.LU_FP32 if a.lu_precision == "fp32" else _lib.LU_FP64)
+    ctx.set_option(_lib.OPT_LU_SCHEDULE, {"blocks": _lib.LU_BLOCKS, "pipeline": _lib.LU_PIPELINE,
+                                          "dataflow": _lib.LU_DATAFLOW, "stream": _lib.LU_STREAM}[a.lu])
     if a.workload == "channel":
         zones, racs = [cases.channel_zone(257, 129, 33)], []
         desc = ("channel with a bump 257x129x33 (3dfull, jittered), laminar AUSM+ o3, implicit LU-SGS, "
@@ -328,7 +330,7 @@ def run_other(a):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if a.lu_precision == "fp64" else "f64 (LU-SGS sweeps f32)",
            "data": "synthetic", "config": {"workload": desc, "cells": ncell, "zones": len(ids),
-                                           "zone_batch": a.zone_batch, "graph": a.graph},
+                                           "zone_batch": a.zone_batch, "graph": a.graph, "lu_schedule": a.lu},
            "event_ms_per_step": round(ms_ev / a.steps, 3),
            "iteration_hbm_frac": round(value * 1e6 * bpc["iteration"] / 1e9 / HBM_PEAK_GBS, 4)}
     if len(ids) == 1:
@@ -338,7 +340,8 @@ def run_other(a):
             phases[name] = ctx.time_phase(ids[0], ph, reps=a.phase_reps)
         out["phase_ms"] = {k: round(v, 4) for k, v in phases.items()}
         ach = bpc["lu"] * ncell / (phases["lu"] * 1e-3) / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "k_lu_block (lower + upper sweeps)", "achieved": round(ach, 1),
+        kname = KERNELS["lu"][a.lu]
+        out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": None, "bytes_per_cell": bpc["lu"]}
     ctx.close()

@@ -64,6 +64,9 @@ struct Zone {
     // the last upper sweep; nullptr: none), and that box
     const double* w_rows;
     Box w_box;
+    // 3dfull: the box whose mean face normals the normal rows hold (n_known)
+    int n_known;
+    Box n_box;
     LuWork* flow_work; int flow_n; Box flow_box;
     double* norm_part; double* norm_out; int norm_blocks;
     double h_norms[10]; int have_norms, have_cfl;
@@ -266,7 +269,7 @@ extern "C" int fm_zone_add(fm_ctx* c, const int* p, int n_int, const double* pr,
     z->pr.assign(pr, pr + n_real);
     z->ro[0] = z->ro[1] = z->ro[2] = nullptr; z->ro_scratch = nullptr;
     z->lu_dws = nullptr; z->lu_sync = nullptr; z->lu_xch = nullptr; z->adj_buf = nullptr;
-    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->dw_pending = 0; z->w_rows = nullptr; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
+    z->lu_fx = nullptr; z->lu_stx = nullptr; z->lu_skw = nullptr; z->dw_pending = 0; z->w_rows = nullptr; z->n_known = 0; z->flow_work = nullptr; z->flow_n = 0; z->flow_box = Box{0, -1, 0, -1, 0, -1};
     z->norm_part = z->norm_out = z->norm_hist = nullptr; z->have_norms = 0; z->have_cfl = 0;
     z->rec_lo = 1; z->rec_hi = 0;
     KZ& k = z->k;
@@ -430,6 +433,7 @@ extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* h
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
     z->w_rows = nullptr;
     if (field == FM_FIELD_DRODM) z->dw_pending = 0;
+    if (field == FM_FIELD_METRIC) z->n_known = 0;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -549,10 +553,11 @@ static int res_schedule_of(const fm_ctx* c, const Zone* z) {
     return z->k.typ == 2 ? RES_CELLS : RES_KCOL;
 }
 
-// Streaming LU schedule of a Cartesian zone: the pencil exchange, sized for the largest LU box and
+// Streaming LU schedule of a zone: the pencil exchange, sized for the largest LU box and
 // filled with the pending pattern once (the sweeps leave it that way).
 static bool uses_stream(const fm_ctx* c, const Zone* z) {
-    return c->lu_schedule == LU_STREAM && z->k.typ == 2 && !c->lu_fp32;
+    (void)z;
+    return c->lu_schedule == LU_STREAM && !c->lu_fp32;
 }
 
 static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
@@ -570,7 +575,7 @@ static int ensure_stream(fm_ctx* c, Zone* z, LuArgs& la) {
     if (!z->lu_skw) {
         const KZ& k = z->k;
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMalloc(&z->lu_skw, sizeof(double) * lu_skew_size(k.ni + 2, k.nj + 2, k.nk + 2)));
+        HIPCHK(hipMalloc(&z->lu_skw, sizeof(double) * lu_skew_size(k.typ == 2, k.ni + 2, k.nj + 2, k.nk + 2)));
     }
     la.stx = z->lu_stx;
     la.skw = z->lu_skw;
@@ -769,10 +774,15 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             const Box& wb = z->w_box;
             la.w_ready = !defer && lu_streams(k, la) && interior && z->w_rows == la.rop && wb.i0 == lbx.i0 &&
                          wb.i1 == lbx.i1 && wb.j0 == lbx.j0 && wb.j1 == lbx.j1 && wb.k0 == lbx.k0 && wb.k1 == lbx.k1;
+            // 3dfull: the normal rows, written by an earlier pack of the same box
+            const Box& nb = z->n_box;
+            la.n_ready = !defer && lu_streams(k, la) && k.typ != 2 && z->n_known && nb.i0 == lbx.i0 &&
+                         nb.i1 == lbx.i1 && nb.j0 == lbx.j0 && nb.j1 == lbx.j1 && nb.k0 == lbx.k0 && nb.k1 == lbx.k1;
             if (defer) { z->w_rows = nullptr; *defer = la; return FM_OK; }
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
             if (!la.keep_dw && la.newton && lu_streams(k, la)) { z->dw_pending = 1; z->dw_la = la; }
+            if (lu_streams(k, la) && k.typ != 2) { z->n_known = 1; z->n_box = lbx; }
             if (la.newton && interior && lu_streams(k, la)) {
                 z->w_rows = la.out == z->ro_scratch ? ro_p1 : la.out;
                 z->w_box = lbx;
@@ -1556,7 +1566,9 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.fx = nullptr; la.flow_work = nullptr; la.flow_n = 0; la.flow_step = 0;
     la.keep_dw = 1;
     la.w_ready = 0;
+    la.n_ready = 0;
     z->w_rows = nullptr;                      // its sweeps write the scratch state into the rows
+    z->n_known = 0;                           // and copy the normals of its own box
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;

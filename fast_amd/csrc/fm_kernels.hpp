@@ -66,12 +66,13 @@ struct LuArgs {
     int keep_dw;                  // streaming: copy dW back to dw (else it stays in the skewed rows)
     int w_ready;                  // streaming: the state rows hold rop already (the last upper sweep
                                   // wrote them and nothing changed the state since)
+    int n_ready;                  // streaming, 3dfull: the normal rows of this box are in place
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
 // doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk
 // cells
 long lu_stream_exchange_size(int ni, int nj, int nk);
-long lu_skew_size(int ni, int nj, int nk);
+long lu_skew_size(bool cart, int ni, int nj, int nk);
 // fill n doubles of an exchange with the pending pattern (once when allocated, and after a wait
 // gave up: the sweeps give back every slot they read)
 void fill_pending(double* x, long n, hipStream_t st);

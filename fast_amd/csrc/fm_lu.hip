@@ -868,8 +868,10 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 
 // doubles per row and lane of the skewed copy: right-hand side (dW* after the lower sweep) 0-4,
 // state 5-9, coe1..4 / 2 10-13 (the sweep's sign products up to the sign, formed by the same
-// multiplications), 1 / coe5 14, coe5 15
-constexpr int kLuSkF = 16;
+// multiplications), 1 / coe5 14, coe5 15; on 3dfull zones the mean face normals of the i, j and
+// k products 16-24 (lu_normal)
+template <bool CART> constexpr int lu_skf() { return CART ? 16 : 25; }
+__device__ __forceinline__ int lu_skf_of(const KZ& z) { return z.typ == 2 ? 16 : 25; }
 // rows in flight per wave
 constexpr int kLuStAhead = 4;
 // steps of slack a consumer lane builds up once it had to wait for the previous pencil
@@ -877,11 +879,17 @@ constexpr int kLuStSlack = 1;
 // steps ahead at which a lane loads the products of the previous pencil (divides kLuStAhead)
 constexpr int kLuStHalo = 2;
 
-__device__ __forceinline__ long lu_sk_row(int t, int r, int R) { return ((long)t * R + r) * kLuSkF * 64; }
+__device__ __forceinline__ long lu_sk_row(int t, int r, int R, int nfld) { return ((long)t * R + r) * nfld * 64; }
 
-long lu_skew_size(int ni, int nj, int nk) {
+long lu_skew_size(bool cart, int ni, int nj, int nk) {
     long nt = (long)((nj + 7) / 8) * ((nk + 7) / 8);
-    return nt * (ni + 14) * kLuSkF * 64;
+    return nt * (ni + 14) * (cart ? 16 : 25) * 64;
+}
+
+// arrays the pack copies, in launch order: the right-hand side, the state (unless a.w_ready), the
+// coefficients, the normals (3dfull, unless a.n_ready)
+__device__ __host__ __forceinline__ int lu_pack_count(bool cart, const LuArgs& a) {
+    return (cart ? 16 : 25) - (a.w_ready ? 5 : 0) - (!cart && a.n_ready ? 9 : 0);
 }
 
 // exchange slot of value e of cell ii of lane l8 on the j (face 0) or k (face 1) side of the
@@ -925,10 +933,11 @@ __device__ __forceinline__ double lu_from_kprev(double v, int lane) {
 // contiguous too; every thread issues all its loads before it waits for one.
 __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     __shared__ double T[64][65];
-    // with a.w_ready the state rows hold the state already: arrays 0-4 and 10-15 only
+    // with a.w_ready the state rows hold the state already (arrays 5-9 skipped)
     const int t = blockIdx.x, r0 = blockIdx.y * 64, f = a.w_ready && blockIdx.z >= 5 ? blockIdx.z + 5 : blockIdx.z;
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
+    const int nfld = lu_skf_of(z);
     const long nd = z.nd;
     const double* __restrict__ src = f < 5 ? a.dw + f * nd : f < 10 ? a.rop + (f - 5) * nd
                                    : z.coe + (f < 14 ? f - 10 : 4) * nd;
@@ -939,7 +948,22 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     for (int m = 0; m < 16; m++) {
         const int L = w + 4 * m, oj = jt * 8 + (L & 7), ok = kt * 8 + (L >> 3);
         const int i = r0 + lane - (L & 7) - (L >> 3);
-        v[m] = oj < nj && ok < nk && i >= 0 && i < ni ? src[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] : 0.;
+        const bool in_box = oj < nj && ok < nk && i >= 0 && i < ni;
+        if (f < 16) {
+            v[m] = in_box ? src[IDX(z, a.i0 + i, a.j0 + oj, a.k0 + ok)] : 0.;
+        } else {
+            // 3dfull: component f of the mean face normal of the product in direction D
+            double n3[3] = {0., 0., 0.};
+            if (in_box) {
+                const long lt = MTR(z, a.i0 + i, a.j0 + oj, a.k0 + ok);
+                const int D = (f - 16) / 3;
+                if (D == 0) lu_normal<false, 0>(z, lt, n3[0], n3[1], n3[2]);
+                else if (D == 1) lu_normal<false, 1>(z, lt, n3[0], n3[1], n3[2]);
+                else lu_normal<false, 2>(z, lt, n3[0], n3[1], n3[2]);
+            }
+            const int cpt = (f - 16) % 3;
+            v[m] = cpt == 0 ? n3[0] : cpt == 1 ? n3[1] : n3[2];
+        }
     }
 #pragma unroll
     for (int m = 0; m < 16; m++) {
@@ -953,7 +977,7 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         const int rr = w + 4 * m, r = r0 + rr;
-        if (r < R) dst[lu_sk_row(t, r, R)] = T[lane][rr];
+        if (r < R) dst[lu_sk_row(t, r, R, nfld)] = T[lane][rr];
     }
 }
 
@@ -967,12 +991,13 @@ __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int 
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const long nd = z.nd;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nfld = lu_skf_of(z);
     const double* __restrict__ src = a.skw + f * 64 + lane;
     double v[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         const int r = r0 + w + 4 * m;
-        v[m] = r < R ? src[lu_sk_row(t, r, R)] : 0.;
+        v[m] = r < R ? src[lu_sk_row(t, r, R, nfld)] : 0.;
     }
 #pragma unroll
     for (int m = 0; m < 16; m++) T[lane][w + 4 * m] = v[m];
@@ -987,8 +1012,9 @@ __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int 
     }
 }
 
-template <bool LOWER>
+template <bool CART, bool LOWER>
 __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int ntk) {
+    constexpr int NF = lu_skf<CART>();
     const int L = threadIdx.x, jj = L & 7, kk = L >> 3;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     const int R = ni + 14;
@@ -1033,12 +1059,12 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
 
         // the rows of steps s .. s + kLuStAhead - 1 in flight (slot s % kLuStAhead); the lower sweep
         // reads neither the density nor coe5
-        double in[kLuStAhead][kLuSkF];
+        double in[kLuStAhead][NF];
         auto rows = [&](int s, int q) {
             const int sc = s < R ? s : R - 1;
-            const double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? sc : R - 1 - sc, R) + L;
+            const double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? sc : R - 1 - sc, R, NF) + L;
 #pragma unroll
-            for (int f = 0; f < kLuSkF; f++)
+            for (int f = 0; f < NF; f++)
                 if (!(LOWER && (f == 5 || f == 15))) in[q][f] = p[f * 64];
         };
         // halo products of step s, kLuStHalo steps ahead (buffer s % kLuStHalo), loaded by the
@@ -1074,9 +1100,12 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
             const double u = in[qn][6], v = in[qn][7], w = in[qn][8], t = in[qn][9];
             const double ci = LOWER ? in[qn][11] : -in[qn][11], cj = LOWER ? in[qn][12] : -in[qn][12];
             const double ck = LOWER ? in[qn][13] : -in[qn][13];
-            B0 = lu_coef<true, 0, double>(c, u, v, w, t, ci, tci, 0., 0.);
-            B1 = lu_coef<true, 1, double>(c, u, v, w, t, cj, tcj, 0., 0.);
-            if (!(newton && kalias)) B2 = lu_coef<true, 2, double>(c, u, v, w, t, ck, tck, 0., 0.);
+            if (CART) {
+                B0 = lu_coef<true, 0, double>(c, u, v, w, t, ci, tci, 0., 0.);
+                B1 = lu_coef<true, 1, double>(c, u, v, w, t, cj, tcj, 0., 0.);
+                if (!(newton && kalias)) B2 = lu_coef<true, 2, double>(c, u, v, w, t, ck, tck, 0., 0.);
+            }
+            // (3dfull: the full 5 x 5 matrices are formed at their step, registers being short)
         };
         coefs(0);
         for (int s0 = 0; s0 < R; s0 += kLuStAhead) {
@@ -1155,10 +1184,11 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                     bk[e] = kk == first ? hk[hb][e] : bk[e];
                 }
                 const bool hi = LOWER ? ii > 0 : ii + 1 < ni;
-                // neighbour terms in the reference order: lower J, K, I; upper K, J, I
+                // neighbour terms in the reference order: lower 3dcart J, K, I; lower 3dfull and
+                // upper K, J, I
 #pragma unroll
                 for (int qq = 0; qq < 3; qq++) {
-                    const int D = LOWER ? (qq == 0 ? 1 : qq == 1 ? 2 : 0) : (qq == 0 ? 2 : qq == 1 ? 1 : 0);
+                    const int D = (LOWER && CART) ? (qq == 0 ? 1 : qq == 1 ? 2 : 0) : (qq == 0 ? 2 : qq == 1 ? 1 : 0);
                     const bool has = D == 0 ? hi : D == 1 ? hjn : hkn;
 #pragma unroll
                     for (int e = 0; e < 5; e++) {
@@ -1172,8 +1202,14 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                 const double uu = wv[1], vv = wv[2], ww = wv[3], tt = wv[4];
                 // the i and j products (the j product moves one lane by DPP)
                 double pk[5], pj[5];
-                lu_apply<true, 0, double>(B0, x, pi);
-                lu_apply<true, 1, double>(B1, x, pj);
+                if (CART) {
+                    lu_apply<true, 0, double>(B0, x, pi);
+                    lu_apply<true, 1, double>(B1, x, pj);
+                } else {
+                    const double ci = LOWER ? in[q][11] : -in[q][11], cj = LOWER ? in[q][12] : -in[q][12];
+                    lu_prod<false, 0, double>(c, uu, vv, ww, tt, ci, in[q][16], in[q][17], in[q][18], x, pi);
+                    lu_prod<false, 1, double>(c, uu, vv, ww, tt, cj, in[q][19], in[q][20], in[q][21], x, pj);
+                }
 #pragma unroll
                 for (int e = 0; e < 5; e++) bj[e] = lu_from_jprev<LOWER>(pj[e]);
                 double o[5] = {0., 0., 0., 0., 0.};
@@ -1189,16 +1225,20 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
                     o[0] = r1; o[1] = u1; o[2] = v1; o[3] = w1; o[4] = t1;
                 }
                 // the k product (moves eight lanes by ds_bpermute)
-                if (newton && kalias)
-                    lu_prod<true, 2, double>(c, o[1], o[2], o[3], o[4], ck, tck, 0., 0., x, pk);
-                else
+                if (newton && kalias) {
+                    if (CART) lu_prod<true, 2, double>(c, o[1], o[2], o[3], o[4], ck, tck, 0., 0., x, pk);
+                    else lu_prod<false, 2, double>(c, o[1], o[2], o[3], o[4], ck, in[q][22], in[q][23], in[q][24], x, pk);
+                } else if (CART) {
                     lu_apply<true, 2, double>(B2, x, pk);
+                } else {
+                    lu_prod<false, 2, double>(c, uu, vv, ww, tt, ck, in[q][22], in[q][23], in[q][24], x, pk);
+                }
 #pragma unroll
                 for (int e = 0; e < 5; e++) bk[e] = lu_from_kprev<LOWER>(pk[e], L);
                 if (act) {
                     // lower: dW* into the right-hand-side row the upper sweep reads; upper: dW and
                     // the updated state into the row just read
-                    double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? s : R - 1 - s, R) + L;
+                    double* __restrict__ p = SK + lu_sk_row(tileid, LOWER ? s : R - 1 - s, R, NF) + L;
 #pragma unroll
                     for (int e = 0; e < 5; e++) p[e * 64] = x[e];
                     if (newton) {
@@ -1299,26 +1339,27 @@ void fill_exchange(const LuArgs& a, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.xch, n, kLuPending, a.sync);
 }
 
-// Streaming sweeps of a Cartesian zone: the skewed copy of the inputs, then for each sweep the
-// exchange filled with the pending pattern (and the tile counter reset) and one launch with as many
-// workgroups as can be resident at once (at most one per pencil).
+// Streaming sweeps of a zone: the skewed copy of the inputs, then for each sweep the exchange filled
+// with the pending pattern (and the tile counter reset) and one launch with as many workgroups as
+// can be resident at once (at most one per pencil).
+template <bool CART>
 void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
     int ntj = (nj + 7) / 8, ntk = (nk + 7) / 8;
     const long nx = lu_stream_exchange_size(ni, nj, nk);
-    hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(a.w_ready ? kLuSkF - 5 : kLuSkF)),
+    hipLaunchKernelGGL(k_lu_pack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)lu_pack_count(CART, a)),
                        dim3(256), 0, st, z, a, ntj);
     int cus = 0, dev = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_lu_stream<true>, 64, 0) != hipSuccess || per < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_lu_stream<CART, true>, 64, 0) != hipSuccess || per < 1)
         per = 1;
     long cap = (long)per * cus;
     unsigned grid = (unsigned)((long)ntj * ntk < cap ? (long)ntj * ntk : cap);
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.stx, nx, kLuPending, a.sync);
-    hipLaunchKernelGGL((k_lu_stream<true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
+    hipLaunchKernelGGL((k_lu_stream<CART, true>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.stx, nx, kLuPending, a.sync);
-    hipLaunchKernelGGL((k_lu_stream<false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
+    hipLaunchKernelGGL((k_lu_stream<CART, false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     // dW back only if asked (inside fm_compute it is copied once, after the last time step)
     const int f0 = a.newton && !a.keep_dw ? 5 : 0, f1 = a.newton ? 10 : 5;
     hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(f1 - f0)),
@@ -1427,7 +1468,8 @@ long lu_flow_exchange_size(bool cart, int ni, int nj, int nk) {
 }
 
 bool lu_streams(const KZ& z, const LuArgs& a) {
-    return a.schedule == LU_STREAM && z.typ == 2 && !a.fp32 && a.skw;
+    (void)z;
+    return a.schedule == LU_STREAM && !a.fp32 && a.skw;
 }
 
 // dW of the last streaming LU-SGS, left in its skewed rows, back to the zone array
@@ -1442,7 +1484,7 @@ void launch_lu_unpack_dw(const KZ& z, const LuArgs& a, hipStream_t st) {
 void launch_lu(const KZ& z, const LuArgs& a, hipStream_t st) {
     if (a.i1 < a.i0 || a.j1 < a.j0 || a.k1 < a.k0) return;
     if (lu_streams(z, a)) {
-        stream(z, a, st);
+        if (z.typ == 2) stream<true>(z, a, st); else stream<false>(z, a, st);
         return;
     }
     if (a.schedule == LU_DATAFLOW || a.schedule == LU_STREAM) {
