@@ -191,15 +191,16 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    clears the word (fm_compute itself is asynchronous and returns FM_OK).  With several zones
    and FM_OPT_ZONE_BATCH on, the sweeps of all zones run as one dataflow launch per sweep. */
 #define FM_LU_DATAFLOW 2
-/* FM_LU_STREAM (default): on Cartesian zones, one wave per pencil of 8 x 8 (j, k) lines sweeps
-   the whole i extent as a skewed wavefront (neighbour products move between lanes by DPP and
-   ds_bpermute) and hands its face products to the next pencils through a global exchange
-   (bounded waits, reported as above).  Its inputs are first copied into a skewed, pencil-major
-   layout (one contiguous row per step and array) and its results copied back; the zone needs
-   16 doubles per cell more device memory for that copy.  Inside fm_compute the rows are kept from
-   one sweep to the next where nothing else changed them (the state rows the upper sweep writes;
-   fm_zone_set_field invalidates them) and dW is copied back to drodm when drodm is next read
-   (fm_zone_get_field, fm_time_phase).  Curvilinear zones, FP32 sweeps and batched zones run the dataflow schedule. */
+/* FM_LU_STREAM (default): one wave per pencil of 8 x 8 (j, k) lines sweeps the whole i extent as a
+   skewed wavefront (neighbour products move between lanes by DPP and ds_bpermute) and hands its
+   face products to the next pencils through a global exchange (bounded waits, reported as above).
+   Its inputs are first copied into a skewed, pencil-major layout (one contiguous row per step and
+   array; on curvilinear zones with the mean face normals) and its results copied back; the zone
+   needs 16 (3dcart) or 25 (3dfull) doubles per cell more device memory for that copy.  Inside
+   fm_compute the rows are kept from one sweep to the next where nothing else changed them (the
+   state rows the upper sweep writes, the normals; fm_zone_set_field invalidates them) and dW is
+   copied back to drodm when drodm is next read (fm_zone_get_field, fm_time_phase).  FP32 sweeps
+   and batched zones run the dataflow schedule. */
 #define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
