@@ -88,7 +88,7 @@ struct fm_ctx {
     long tr_n;
     // diagonal-batched LU-SGS over several zones: cached block lists (per set of LU boxes)
     // and the per-zone argument table, staged through two pinned host buffers
-    struct LuList { bool cart; std::vector<Box> boxes; LuWork* work; std::vector<int> offs; };
+    struct LuList { bool cart; bool stream; std::vector<Box> boxes; LuWork* work; std::vector<int> offs; int maxrb; };
     std::vector<LuList> lu_lists;
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
@@ -1087,10 +1087,12 @@ extern "C" int fm_apply_bc(fm_ctx* c, int zone, int level, int nstep) {
 static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector<LuArgs>& las,
                       fm_ctx::GraphEntry* ge = nullptr, int nstep = 0, bool capture = false) {
     hipStream_t st = c->stream;
-    // FP64 dataflow: one launch per sweep over the blocks of all zones, each zone with its own face
-    // exchange; otherwise (blocks schedule, FP32 sweeps) one launch per block diagonal
-    const bool flow = (c->lu_schedule == LU_DATAFLOW || c->lu_schedule == LU_STREAM) && !c->lu_fp32;
-    if (flow && !c->flow_ctr) {
+    // FP64 streaming: one launch per sweep over the pencils of all zones; FP64 dataflow: one launch
+    // per sweep over the blocks of all zones, each zone with its own face exchange; otherwise (blocks
+    // schedule, FP32 sweeps) one launch per block diagonal
+    const bool strm = c->lu_schedule == LU_STREAM && !c->lu_fp32;
+    const bool flow = c->lu_schedule == LU_DATAFLOW && !c->lu_fp32;
+    if ((flow || strm) && !c->flow_ctr) {
         if (capture) return fail(FM_E_HIP, "graph capture: dataflow counter missing");
         HIPCHK(hipStreamSynchronize(st));
         HIPCHK(hipMalloc(&c->flow_ctr, 4 * sizeof(int)));
@@ -1107,13 +1109,18 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
                 if (int rc = ensure_flow_exchange(c, zs[n])) return rc;
                 a.fx = zs[n]->lu_fx; a.flow_step = 0; a.schedule = LU_DATAFLOW;
             }
+            if (strm) {
+                if ((!zs[n]->lu_skw || !zs[n]->lu_stx) && capture) return fail(FM_E_HIP, "graph capture: LU rows missing");
+                if (int rc = ensure_stream(c, zs[n], a)) return rc;
+                a.schedule = LU_STREAM; a.keep_dw = 1; a.w_ready = 0; a.n_ready = 0;
+            }
             boxes.push_back(Box{a.i0, a.i1, a.j0, a.j1, a.k0, a.k1});
             tab.push_back(LuZone{zs[n]->k, a});
         }
         if (tab.empty()) continue;
         fm_ctx::LuList* L = nullptr;
         for (auto& l : c->lu_lists) {
-            if (l.cart != (cart == 1) || l.boxes.size() != boxes.size()) continue;
+            if (l.cart != (cart == 1) || l.stream != strm || l.boxes.size() != boxes.size()) continue;
             bool same = true;
             for (size_t q = 0; q < boxes.size() && same; q++)
                 same = !memcmp(&l.boxes[q], &boxes[q], sizeof(Box));
@@ -1121,9 +1128,15 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         }
         if (!L) {
             fm_ctx::LuList nl;
-            nl.cart = cart == 1; nl.boxes = boxes; nl.work = nullptr;
+            nl.cart = cart == 1; nl.stream = strm; nl.boxes = boxes; nl.work = nullptr; nl.maxrb = 0;
             std::vector<LuWork> w;
-            lu_work_list(cart == 1, boxes, w, nl.offs);
+            if (strm) {
+                lu_pencil_list(boxes, w, nl.maxrb);
+                nl.offs.assign(1, 0);
+                nl.offs.push_back((int)w.size());
+            } else {
+                lu_work_list(cart == 1, boxes, w, nl.offs);
+            }
             if (!w.empty()) {
                 HIPCHK(hipMalloc(&nl.work, w.size() * sizeof(LuWork)));
                 HIPCHK(hipMemcpy(nl.work, w.data(), w.size() * sizeof(LuWork), hipMemcpyHostToDevice));
@@ -1143,7 +1156,9 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
                 ge->tabs.push_back({key, d});
             }
             if (!capture) HIPCHK(hipMemcpy(d, tab.data(), tab.size() * sizeof(LuZone), hipMemcpyHostToDevice));
-            if (flow) {
+            if (strm) {
+                launch_lu_stream_list(cart == 1, d, (int)tab.size(), L->work, L->offs.back(), L->maxrb, c->flow_ctr, st);
+            } else if (flow) {
                 launch_lu_flow_list(cart == 1, d, L->work, L->offs.back(), c->flow_ctr, st);
             } else {
                 launch_lu_list(cart == 1, true, c->lu_fp32, d, L->work, L->offs, st);
@@ -1169,7 +1184,9 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         memcpy(c->lu_host[q], tab.data(), tab.size() * sizeof(LuZone));
         HIPCHK(hipMemcpyAsync(c->lu_tab, c->lu_host[q], tab.size() * sizeof(LuZone), hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(c->lu_ev[q], st));
-        if (flow) {
+        if (strm) {
+            launch_lu_stream_list(cart == 1, c->lu_tab, (int)tab.size(), L->work, L->offs.back(), L->maxrb, c->flow_ctr, st);
+        } else if (flow) {
             launch_lu_flow_list(cart == 1, c->lu_tab, L->work, L->offs.back(), c->flow_ctr, st);
         } else {
             launch_lu_list(cart == 1, true, c->lu_fp32, c->lu_tab, L->work, L->offs, st);

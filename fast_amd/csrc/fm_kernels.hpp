@@ -71,7 +71,7 @@ struct LuArgs {
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
 // doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk
 // cells
-long lu_stream_exchange_size(int ni, int nj, int nk);
+__host__ __device__ long lu_stream_exchange_size(int ni, int nj, int nk);
 long lu_skew_size(bool cart, int ni, int nj, int nk);
 // fill n doubles of an exchange with the pending pattern (once when allocated, and after a wait
 // gave up: the sweeps give back every slot they read)
@@ -134,6 +134,12 @@ void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const
                     const std::vector<int>& offs, hipStream_t st);
 // dataflow sweeps (lower then upper, FP64) of several zones of one mesh kind in one launch each:
 // the work list in diagonal order, every zone's LuArgs with its face exchange (fx) and error word
+// streaming sweeps of several zones of one mesh kind: pencils (zone, 0, jt, kt) in diagonal order,
+// every zone's LuArgs with its rows (skw), exchange (stx) and error word; maxrb = the largest
+// number of 64-row blocks of a zone
+void lu_pencil_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, int& maxrb);
+void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork* work, int nwork, int maxrb,
+                           int* counter, hipStream_t st);
 void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,
                          hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);

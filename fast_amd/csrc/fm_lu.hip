@@ -36,6 +36,7 @@ template <bool CART> constexpr int lu_ci() { return CART ? FM_LU_CI_CART : FM_LU
 #ifndef FM_LU_FLOW_GRID
 #define FM_LU_FLOW_GRID 2048
 #endif
+#include <algorithm>
 #include <array>
 #include <vector>
 
@@ -900,7 +901,7 @@ __device__ __forceinline__ long lu_st_slot(int face, int t, int l8, int e, int i
     return ((((long)face * ntile + t) * (ni + 7) + ii + l8) * 5 + e) * 8 + l8;
 }
 
-long lu_stream_exchange_size(int ni, int nj, int nk) {
+__host__ __device__ long lu_stream_exchange_size(int ni, int nj, int nk) {
     long nt = (long)((nj + 7) / 8) * ((nk + 7) / 8);
     return 2 * nt * 40 * (long)(ni + 7);
 }
@@ -931,10 +932,8 @@ __device__ __forceinline__ double lu_from_kprev(double v, int lane) {
 // L of the pencil contributes its cells r0 - d .. r0 - d + 63 (d = jj + kk) to those rows: a
 // contiguous run per line, staged through LDS so that the row writes (along the lanes) are
 // contiguous too; every thread issues all its loads before it waits for one.
-__global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
-    __shared__ double T[64][65];
-    // with a.w_ready the state rows hold the state already (arrays 5-9 skipped)
-    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = a.w_ready && blockIdx.z >= 5 ? blockIdx.z + 5 : blockIdx.z;
+__device__ __forceinline__ void lu_pack_rows(const KZ& z, const LuArgs& a, int ntj, int t, int r0, int f,
+                                             double (*T)[65]) {
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const int nfld = lu_skf_of(z);
@@ -981,12 +980,31 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     }
 }
 
+
+__global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
+    __shared__ double T[64][65];
+    // with a.w_ready the state rows hold the state already (arrays 5-9 skipped)
+    const int f = a.w_ready && blockIdx.z >= 5 ? blockIdx.z + 5 : blockIdx.z;
+    lu_pack_rows(z, a, ntj, blockIdx.x, blockIdx.y * 64, f, T);
+}
+
+// The skewed copies of several zones: workgroup (w, y, f) packs array f of rows [64 y, 64 y + 64)
+// of pencil work[w] (every array: the batched sweeps keep nothing between calls).
+__global__ __launch_bounds__(256) void k_lu_pack_list(const LuZone* __restrict__ zones, const LuWork* __restrict__ work) {
+    __shared__ double T[64][65];
+    const LuWork w = work[blockIdx.x];
+    const KZ& z = zones[w.zone].z;
+    const LuArgs& a = zones[w.zone].a;
+    const int ntj = (a.j1 - a.j0 + 8) / 8, R = a.i1 - a.i0 + 15, r0 = blockIdx.y * 64;
+    if (r0 >= R || (int)blockIdx.z >= lu_skf_of(z)) return;
+    lu_pack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, blockIdx.z, T);
+}
+
 // The results back to the zone arrays: workgroup (t, y, f) copies array f (dW 0-4, the updated
 // state 5-9) of rows [64 y, 64 y + 64) of pencil t, through LDS so that the row reads (along the
 // lanes) and the zone writes (along i) are contiguous.
-__global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int f0) {
-    __shared__ double T[64][65];
-    const int t = blockIdx.x, r0 = blockIdx.y * 64, f = f0 + blockIdx.z;
+__device__ __forceinline__ void lu_unpack_rows(const KZ& z, const LuArgs& a, int ntj, int t, int r0, int f,
+                                               double (*T)[65]) {
     const int jt = t % ntj, kt = t / ntj;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1, R = ni + 14;
     const long nd = z.nd;
@@ -1012,8 +1030,37 @@ __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int 
     }
 }
 
+
+__global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int f0) {
+    __shared__ double T[64][65];
+    lu_unpack_rows(z, a, ntj, blockIdx.x, blockIdx.y * 64, f0 + blockIdx.z, T);
+}
+
+// The results of several zones back: workgroup (w, y, f) copies array f (dW 0-4, the updated state
+// 5-9) of rows [64 y, 64 y + 64) of pencil work[w].
+__global__ __launch_bounds__(256) void k_lu_unpack_list(const LuZone* __restrict__ zones, const LuWork* __restrict__ work) {
+    __shared__ double T[64][65];
+    const LuWork w = work[blockIdx.x];
+    const KZ& z = zones[w.zone].z;
+    const LuArgs& a = zones[w.zone].a;
+    const int ntj = (a.j1 - a.j0 + 8) / 8, R = a.i1 - a.i0 + 15, r0 = blockIdx.y * 64;
+    if (r0 >= R || ((int)blockIdx.z >= 5 && !a.newton)) return;
+    lu_unpack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, blockIdx.z, T);
+}
+
+// The pencil exchanges of several zones filled with the pending pattern (and the shared pencil
+// counter reset): workgroup (n, c) fills chunk c of zone n's exchange.
+__global__ __launch_bounds__(256) void k_lu_fill_list(const LuZone* __restrict__ zones, int* counter) {
+    const LuArgs& a = zones[blockIdx.x].a;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *counter = 0;
+    const long n = lu_stream_exchange_size(a.i1 - a.i0 + 1, a.j1 - a.j0 + 1, a.k1 - a.k0 + 1);
+    for (long e = (long)blockIdx.y * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.y * blockDim.x)
+        a.stx[e] = __longlong_as_double(kLuPending);
+}
+
+// One pencil (jt, kt) of a sweep, in sweep-order tile coordinates.
 template <bool CART, bool LOWER>
-__global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int ntk) {
+__device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, int kt, int ntj, int ntk) {
     constexpr int NF = lu_skf<CART>();
     const int L = threadIdx.x, jj = L & 7, kk = L >> 3;
     const int ni = a.i1 - a.i0 + 1, nj = a.j1 - a.j0 + 1, nk = a.k1 - a.k0 + 1;
@@ -1029,14 +1076,7 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
     const double cvinv = 1. / z.cv, cvinv2 = 0.5 * cvinv;
     // first and last line of a pencil in sweep order
     const int first = LOWER ? 0 : 7, last = LOWER ? 7 : 0;
-    for (;;) {
-        int tk0 = 0;
-        if (L == 0) tk0 = atomicAdd(a.sync, 1);
-        const int t = __builtin_amdgcn_readfirstlane(tk0);
-        if (t >= ntile) break;
-        int jt, kt;
-        lu_tile_of(t, ntj, ntk, jt, kt);
-        if (!LOWER) { jt = ntj - 1 - jt; kt = ntk - 1 - kt; }
+    {
         const int tileid = jt + ntj * kt;
         const int oj = jt * 8 + jj, ok = kt * 8 + kk;
         const bool lane_ok = oj < nj && ok < nk;
@@ -1264,6 +1304,44 @@ __global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int n
     }
 }
 
+
+template <bool CART, bool LOWER>
+__global__ __launch_bounds__(64) void k_lu_stream(KZ z, LuArgs a, int ntj, int ntk) {
+    const int ntile = ntj * ntk;
+    for (;;) {
+        int tk0 = 0;
+        if (threadIdx.x == 0) tk0 = atomicAdd(a.sync, 1);
+        const int t = __builtin_amdgcn_readfirstlane(tk0);
+        if (t >= ntile) break;
+        int jt, kt;
+        lu_tile_of(t, ntj, ntk, jt, kt);
+        if (!LOWER) { jt = ntj - 1 - jt; kt = ntk - 1 - kt; }
+        lu_pencil<CART, LOWER>(z, a, jt, kt, ntj, ntk);
+    }
+}
+
+// Streaming sweeps of several zones in one launch: the pencils of every zone in one list ordered by
+// pencil diagonal (jt + kt), handed out from one counter; the upper sweep takes each zone's
+// pencils mirrored, so in either sweep a pencil only waits on pencils of its zone handed out before
+// it.  Each zone has its own rows, exchange and error word.
+template <bool CART, bool LOWER>
+__global__ __launch_bounds__(64) void k_lu_stream_list(const LuZone* __restrict__ zones,
+                                                       const LuWork* __restrict__ work, int nwork, int* counter) {
+    for (;;) {
+        int tk0 = 0;
+        if (threadIdx.x == 0) tk0 = atomicAdd(counter, 1);
+        const int t = __builtin_amdgcn_readfirstlane(tk0);
+        if (t >= nwork) break;
+        const LuWork w = work[t];
+        const KZ& z = zones[w.zone].z;
+        const LuArgs& a = zones[w.zone].a;
+        const int ntj = (a.j1 - a.j0 + 8) / 8, ntk = (a.k1 - a.k0 + 8) / 8;
+        int jt = w.bj, kt = w.bk;
+        if (!LOWER) { jt = ntj - 1 - jt; kt = ntk - 1 - kt; }
+        lu_pencil<CART, LOWER>(z, a, jt, kt, ntj, ntk);
+    }
+}
+
 // fill n doubles with a 64-bit pattern and zero the tile counter
 __global__ void k_fill_lin(double* __restrict__ p, long n, long long bits, int* sync) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1436,6 +1514,57 @@ void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const
             else hipLaunchKernelGGL((k_lu_block_list<false, false, CF>), dim3(n), dim3(256), 0, st, zones, w);
         }
     }
+}
+
+// pencils (zone, 0, jt, kt) of several LU boxes, in pencil-diagonal order
+void lu_pencil_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, int& maxrb) {
+    int nd = 0;
+    maxrb = 0;
+    std::vector<std::array<int, 2>> nt(boxes.size());
+    for (size_t n = 0; n < boxes.size(); n++) {
+        const Box& b = boxes[n];
+        const bool empty = b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0;
+        nt[n] = {empty ? 0 : (b.j1 - b.j0 + 1 + 7) / 8, empty ? 0 : (b.k1 - b.k0 + 1 + 7) / 8};
+        if (!empty) {
+            nd = std::max(nd, nt[n][0] + nt[n][1] - 1);
+            maxrb = std::max(maxrb, (b.i1 - b.i0 + 1 + 14 + 63) / 64);
+        }
+    }
+    work.clear();
+    for (int d = 0; d < nd; d++)
+        for (size_t n = 0; n < boxes.size(); n++)
+            for (int kt = 0; kt < nt[n][1]; kt++) {
+                const int jt = d - kt;
+                if (jt >= 0 && jt < nt[n][0]) work.push_back(LuWork{(int)n, 0, jt, kt});
+            }
+}
+
+// Streaming LU-SGS of several zones of one mesh kind: the skewed copies, then for each sweep the
+// exchanges filled (and the pencil counter reset) and one launch over the pencil list, then the
+// results back.
+void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork* work, int nwork, int maxrb,
+                           int* counter, hipStream_t st) {
+    if (nwork <= 0 || nz <= 0) return;
+    int cus = 0, dev = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    const void* kfn = cart ? (const void*)k_lu_stream_list<true, true> : (const void*)k_lu_stream_list<false, true>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, 64, 0) != hipSuccess || per < 1) per = 1;
+    const long cap = (long)per * cus;
+    const unsigned grid = (unsigned)(nwork < cap ? nwork : cap);
+    hipLaunchKernelGGL(k_lu_pack_list, dim3((unsigned)nwork, (unsigned)maxrb, cart ? 16u : 25u), dim3(256), 0, st,
+                       zones, work);
+    for (int lower = 1; lower >= 0; lower--) {
+        hipLaunchKernelGGL(k_lu_fill_list, dim3((unsigned)nz, 64), dim3(256), 0, st, zones, counter);
+        if (cart) {
+            if (lower) hipLaunchKernelGGL((k_lu_stream_list<true, true>), dim3(grid), dim3(64), 0, st, zones, work, nwork, counter);
+            else hipLaunchKernelGGL((k_lu_stream_list<true, false>), dim3(grid), dim3(64), 0, st, zones, work, nwork, counter);
+        } else {
+            if (lower) hipLaunchKernelGGL((k_lu_stream_list<false, true>), dim3(grid), dim3(64), 0, st, zones, work, nwork, counter);
+            else hipLaunchKernelGGL((k_lu_stream_list<false, false>), dim3(grid), dim3(64), 0, st, zones, work, nwork, counter);
+        }
+    }
+    hipLaunchKernelGGL(k_lu_unpack_list, dim3((unsigned)nwork, (unsigned)maxrb, 10u), dim3(256), 0, st, zones, work);
 }
 
 void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,
