@@ -199,8 +199,9 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
    needs 16 (3dcart) or 25 (3dfull) doubles per cell more device memory for that copy.  Inside
    fm_compute the rows are kept from one sweep to the next where nothing else changed them (the
    state rows the upper sweep writes, the normals; fm_zone_set_field invalidates them) and dW is
-   copied back to drodm when drodm is next read (fm_zone_get_field, fm_time_phase).  FP32 sweeps
-   and batched zones run the dataflow schedule. */
+   copied back to drodm when drodm is next read (fm_zone_get_field, fm_time_phase).  With several
+   zones and FM_OPT_ZONE_BATCH on, the pencils of all zones go through one launch per sweep.  FP32
+   sweeps run the dataflow schedule. */
 #define FM_LU_STREAM 3
 /* FM_OPT_RESIDUAL_SCHEDULE: FM_RES_CELLS evaluates the six faces of every cell;
    FM_RES_FACES evaluates every face flux once and sums them per cell in the reference order;
