@@ -876,7 +876,17 @@ __device__ __forceinline__ int lu_skf_of(const KZ& z) { return z.typ == 2 ? 16 :
 // rows in flight per wave
 constexpr int kLuStAhead = 4;
 // steps of slack a consumer lane builds up once it had to wait for the previous pencil
-constexpr int kLuStSlack = 1;
+#ifndef FM_LU_ST_SLACK
+#define FM_LU_ST_SLACK 1
+#endif
+constexpr int kLuStSlack = FM_LU_ST_SLACK;
+// s_sleep arguments (units of 64 cycles) of the wait for the slack step and of the re-reads
+#ifndef FM_LU_ST_SLEEP_SLACK
+#define FM_LU_ST_SLEEP_SLACK 4
+#endif
+#ifndef FM_LU_ST_SLEEP_RETRY
+#define FM_LU_ST_SLEEP_RETRY 2
+#endif
 // steps ahead at which a lane loads the products of the previous pencil (divides kLuStAhead)
 constexpr int kLuStHalo = 2;
 
@@ -981,11 +991,34 @@ __device__ __forceinline__ void lu_pack_rows(const KZ& z, const LuArgs& a, int n
 }
 
 
+// Pencil t and row block y of workgroup (x, y) of a (pencil, row block) grid.
+// Consecutive workgroups go to the eight XCDs in turn; this order gives each XCD a contiguous
+// range of (t, y) with y fastest (XCD c the c-th of eight near-equal parts), so that the row
+// blocks of a pencil run one after another on one XCD and the cache lines a block's runs share
+// with the next block's (the runs of a line start at r0 - jj - kk, not on a line boundary) are
+// served from that XCD's L2.
+#ifndef FM_LU_XCD_ORDER
+#define FM_LU_XCD_ORDER 1
+#endif
+__device__ __forceinline__ void lu_copy_tile(int& t, int& y) {
+    const int nx = gridDim.x, ny = gridDim.y, n = nx * ny;
+    if (!FM_LU_XCD_ORDER) {
+        t = blockIdx.x; y = blockIdx.y;
+        return;
+    }
+    // XCD c holds the ids id = c (mod 8): n / 8 of them, one more for the first n % 8 XCDs
+    const int id = blockIdx.x + nx * blockIdx.y, c = id % 8, per = n / 8, rem = n % 8;
+    const int w = c * per + (c < rem ? c : rem) + id / 8;
+    t = w / ny; y = w % ny;
+}
+
 __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     __shared__ double T[64][65];
     // with a.w_ready the state rows hold the state already (arrays 5-9 skipped)
     const int f = a.w_ready && blockIdx.z >= 5 ? blockIdx.z + 5 : blockIdx.z;
-    lu_pack_rows(z, a, ntj, blockIdx.x, blockIdx.y * 64, f, T);
+    int t, y;
+    lu_copy_tile(t, y);
+    lu_pack_rows(z, a, ntj, t, y * 64, f, T);
 }
 
 // The skewed copies of several zones: workgroup (w, y, f) packs array f of rows [64 y, 64 y + 64)
@@ -1033,7 +1066,9 @@ __device__ __forceinline__ void lu_unpack_rows(const KZ& z, const LuArgs& a, int
 
 __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int f0) {
     __shared__ double T[64][65];
-    lu_unpack_rows(z, a, ntj, blockIdx.x, blockIdx.y * 64, f0 + blockIdx.z, T);
+    int t, y;
+    lu_copy_tile(t, y);
+    lu_unpack_rows(z, a, ntj, t, y * 64, f0 + blockIdx.z, T);
 }
 
 // The results of several zones back: workgroup (w, y, f) copies array f (dW 0-4, the updated state
@@ -1198,7 +1233,7 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
                                 if (++spins > kLuSpinMax ||
                                     ((spins & 63) == 1 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
                                     break;
-                                __builtin_amdgcn_s_sleep(16);
+                                __builtin_amdgcn_s_sleep(FM_LU_ST_SLEEP_SLACK);
                                 if (wa) va = lu_ldc(pa);
                             }
 #pragma unroll
@@ -1210,7 +1245,7 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
                                 if (L == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                 break;
                             }
-                            __builtin_amdgcn_s_sleep(8);
+                            __builtin_amdgcn_s_sleep(FM_LU_ST_SLEEP_RETRY);
                             hload(s, hb);
                             // every access of the wave done: the steps after this rare path keep
                             // exact memory counters
