@@ -874,7 +874,15 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 template <bool CART> constexpr int lu_skf() { return CART ? 16 : 25; }
 __device__ __forceinline__ int lu_skf_of(const KZ& z) { return z.typ == 2 ? 16 : 25; }
 // rows in flight per wave
-constexpr int kLuStAhead = 4;
+// 1: the B matrices of a step are formed one step ahead (while the step before waits), 0: at the
+// step (as fast at 256^3, and fewer instructions)
+#ifndef FM_LU_ST_COEF_AHEAD
+#define FM_LU_ST_COEF_AHEAD 0
+#endif
+#ifndef FM_LU_ST_AHEAD
+#define FM_LU_ST_AHEAD 4
+#endif
+constexpr int kLuStAhead = FM_LU_ST_AHEAD;
 // steps of slack a consumer lane builds up once it had to wait for the previous pencil
 #ifndef FM_LU_ST_SLACK
 #define FM_LU_ST_SLACK 1
@@ -1167,9 +1175,9 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
         for (int q = 0; q < kLuStAhead; q++) rows(q, q);
 #pragma unroll
         for (int q = 0; q < kLuStHalo; q++) hload(q, q);
-        // the B matrices of the three products of the next step, formed while the current step
-        // waits on its neighbours (they do not depend on the solution; with in-place
-        // sub-iterations the k matrix takes the updated state and is formed at its step)
+        // the B matrices of the three products of a step, formed before it waits on its
+        // neighbours (they do not depend on the solution; with in-place sub-iterations the k
+        // matrix takes the updated state and is formed with the product)
         LuB<double> B0, B1, B2;
         auto coefs = [&](int qn) {
             const double u = in[qn][6], v = in[qn][7], w = in[qn][8], t = in[qn][9];
@@ -1182,7 +1190,9 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
             }
             // (3dfull: the full 5 x 5 matrices are formed at their step, registers being short)
         };
+#if FM_LU_ST_COEF_AHEAD
         coefs(0);
+#endif
         for (int s0 = 0; s0 < R; s0 += kLuStAhead) {
 #pragma unroll
             for (int q = 0; q < kLuStAhead; q++) {
@@ -1190,6 +1200,9 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
                 const int ii = cell(s);
                 const bool act = s < R && lane_ok && ii >= 0 && ii < ni;
                 const int hb = q % kLuStHalo;
+#if !FM_LU_ST_COEF_AHEAD
+                coefs(q);
+#endif
                 // the inputs of the step (lanes without a cell read zeros or stale values, never
                 // used)
                 double x[5], wv[5];
@@ -1333,7 +1346,9 @@ __device__ __forceinline__ void lu_pencil(const KZ& z, const LuArgs& a, int jt, 
                 hload(s + kLuStHalo, hb);
                 // the row kLuStAhead steps ahead into the slot just read
                 rows(s + kLuStAhead, q);
+#if FM_LU_ST_COEF_AHEAD
                 coefs((q + 1) % kLuStAhead);
+#endif
             }
         }
     }
