@@ -213,9 +213,10 @@ int fm_compute_timed(fm_ctx* ctx, int nitrun, int nstep_deb, int nstep_fin, int 
 #define FM_RES_AUTO 3
 #define FM_RES_FACES 1
 #define FM_RES_KCOL 2
-/* FM_OPT_ZONE_BATCH: 1 (default) runs the LU-SGS sweeps of all zones together, one launch per
-   block diagonal over a work list of (zone, block) pairs (the blocks schedule); 0 sweeps the
-   zones one after the other.  Same results. */
+/* FM_OPT_ZONE_BATCH: 1 (default) runs the LU-SGS sweeps of all zones together over one work list:
+   one launch per sweep over (zone, pencil) pairs (stream schedule) or (zone, block) pairs
+   (dataflow), one launch per block diagonal (blocks); 0 sweeps the zones one after the other.
+   Same results. */
 #define FM_OPT_ZONE_BATCH 3
 /* FM_OPT_GRAPH: 1 (default) captures each distinct time-step configuration (level pointers, first
    step, verification, sub-iteration range) into a HIP graph the second time it runs and replays it
