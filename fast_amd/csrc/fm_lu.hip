@@ -873,12 +873,12 @@ __global__ __launch_bounds__(64) void k_lu_pipe(KZ z, LuArgs a, const double* __
 // k products 16-24 (lu_normal)
 template <bool CART> constexpr int lu_skf() { return CART ? 16 : 25; }
 __device__ __forceinline__ int lu_skf_of(const KZ& z) { return z.typ == 2 ? 16 : 25; }
-// rows in flight per wave
 // 1: the B matrices of a step are formed one step ahead (while the step before waits), 0: at the
 // step (as fast at 256^3, and fewer instructions)
 #ifndef FM_LU_ST_COEF_AHEAD
 #define FM_LU_ST_COEF_AHEAD 0
 #endif
+// rows in flight per wave
 #ifndef FM_LU_ST_AHEAD
 #define FM_LU_ST_AHEAD 4
 #endif
