@@ -433,7 +433,10 @@ extern "C" int fm_zone_set_field(fm_ctx* c, int zone, int field, const double* h
     if (!d || n != sz) return fail(FM_E_ARG, "field %d: size %ld expected %ld", field, n, sz);
     z->w_rows = nullptr;
     if (field == FM_FIELD_DRODM) z->dw_pending = 0;
-    if (field == FM_FIELD_METRIC) z->n_known = 0;
+    if (field == FM_FIELD_METRIC) {
+        z->n_known = 0;
+        c->graph_epoch++;     // captured steps hold whether the normal rows were in place
+    }
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -678,6 +681,41 @@ static int wall_corrections(const Zone* z, RhsArgs& a) {
     return FM_OK;
 }
 
+static bool same_box(const Box& a, const Box& b) {
+    return a.i0 == b.i0 && a.i1 == b.i1 && a.j0 == b.j0 && a.j1 == b.j1 && a.k0 == b.k0 && a.k1 == b.k1;
+}
+static bool lu_interior(const KZ& k, const LuArgs& la) {
+    return la.i0 >= 1 && la.i1 <= k.ni && la.j0 >= 1 && la.j1 <= k.nj && la.k0 >= 1 && la.k1 <= k.nk;
+}
+
+// The copies a streaming LU-SGS of zone z can skip: dW stays in the rows inside fm_compute (copied
+// back when drodm is read); the state rows already hold rop if the last upper sweep wrote them for
+// this box and the box is interior (the ghost fill does not touch it); 3dfull: the normal rows,
+// written by an earlier pack of the same box.  la carries the zone's rows (la.skw) here.
+static void lu_stream_flags(const fm_ctx* c, const Zone* z, LuArgs& la) {
+    const KZ& k = z->k;
+    const Box lbx = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+    const bool st = lu_streams(k, la);
+    la.keep_dw = !c->defer_dw;
+    la.w_ready = st && lu_interior(k, la) && z->w_rows == la.rop && same_box(z->w_box, lbx);
+    la.n_ready = st && k.typ != 2 && z->n_known && same_box(z->n_box, lbx);
+}
+
+// What the rows of zone z hold after its LU-SGS with la (any schedule: the others leave none).
+static void lu_stream_after(Zone* z, const LuArgs& la) {
+    const KZ& k = z->k;
+    const Box lbx = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
+    const bool st = lu_streams(k, la);
+    if (!la.keep_dw && la.newton && st) { z->dw_pending = 1; z->dw_la = la; }
+    if (st && k.typ != 2) { z->n_known = 1; z->n_box = lbx; }
+    if (la.newton && lu_interior(k, la) && st) {
+        z->w_rows = la.out == z->ro_scratch ? z->ro[2] : la.out;
+        z->w_box = lbx;
+    } else {
+        z->w_rows = nullptr;
+    }
+}
+
 // One sub-iteration of one zone (gsdr3.cpp:152-444 with navier_stokes_struct.for and lhs.cpp).
 static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_verif, int lexit_lu, int stop,
                    LuArgs* defer = nullptr) {
@@ -766,29 +804,12 @@ static int substep(fm_ctx* c, Zone* z, int nitcfg, int first_it, int lssiter_ver
             } else {
                 la.rop_k = ro_ssiter; la.out = ro_p1;
             }
-            la.keep_dw = !c->defer_dw;
-            // the state rows already hold rop if the last upper sweep wrote them for this box and
-            // the box is interior (the ghost fill does not touch it)
-            const Box lbx = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
-            const bool interior = la.i0 >= 1 && la.i1 <= k.ni && la.j0 >= 1 && la.j1 <= k.nj && la.k0 >= 1 && la.k1 <= k.nk;
-            const Box& wb = z->w_box;
-            la.w_ready = !defer && lu_streams(k, la) && interior && z->w_rows == la.rop && wb.i0 == lbx.i0 &&
-                         wb.i1 == lbx.i1 && wb.j0 == lbx.j0 && wb.j1 == lbx.j1 && wb.k0 == lbx.k0 && wb.k1 == lbx.k1;
-            // 3dfull: the normal rows, written by an earlier pack of the same box
-            const Box& nb = z->n_box;
-            la.n_ready = !defer && lu_streams(k, la) && k.typ != 2 && z->n_known && nb.i0 == lbx.i0 &&
-                         nb.i1 == lbx.i1 && nb.j0 == lbx.j0 && nb.j1 == lbx.j1 && nb.k0 == lbx.k0 && nb.k1 == lbx.k1;
-            if (defer) { z->w_rows = nullptr; *defer = la; return FM_OK; }
+            // (a batched zone: lu_batched sets the copies it skips, with its rows)
+            lu_stream_flags(c, z, la);
+            if (defer) { *defer = la; return FM_OK; }
             launch_lu(k, la, st);
             HIPCHK(hipGetLastError());
-            if (!la.keep_dw && la.newton && lu_streams(k, la)) { z->dw_pending = 1; z->dw_la = la; }
-            if (lu_streams(k, la) && k.typ != 2) { z->n_known = 1; z->n_box = lbx; }
-            if (la.newton && interior && lu_streams(k, la)) {
-                z->w_rows = la.out == z->ro_scratch ? ro_p1 : la.out;
-                z->w_box = lbx;
-            } else {
-                z->w_rows = nullptr;
-            }
+            lu_stream_after(z, la);
             if (la.out == z->ro_scratch) {
                 // copy the updated cells back (the LU box, ghost layers it extends into included)
                 Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
@@ -1098,9 +1119,12 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         HIPCHK(hipMalloc(&c->flow_ctr, 4 * sizeof(int)));
         HIPCHK(hipMemset(c->flow_ctr, 0, 4 * sizeof(int)));
     }
+    // each zone's LuArgs as launched, for the record of what its rows hold
+    std::vector<LuArgs> done(las);
     for (int cart = 0; cart < 2; cart++) {
         std::vector<Box> boxes;
         std::vector<LuZone> tab;
+        int npack = 0, nunpack = 0;
         for (size_t n = 0; n < zs.size(); n++) {
             if ((zs[n]->k.typ == 2) != (cart == 1)) continue;
             LuArgs a = las[n];
@@ -1112,10 +1136,14 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
             if (strm) {
                 if ((!zs[n]->lu_skw || !zs[n]->lu_stx) && capture) return fail(FM_E_HIP, "graph capture: LU rows missing");
                 if (int rc = ensure_stream(c, zs[n], a)) return rc;
-                a.schedule = LU_STREAM; a.keep_dw = 1; a.w_ready = 0; a.n_ready = 0;
+                a.schedule = LU_STREAM;
+                lu_stream_flags(c, zs[n], a);
+                npack = std::max(npack, lu_pack_count(cart == 1, a));
+                nunpack = std::max(nunpack, lu_unpack_end(a) - lu_unpack_first(a));
             }
             boxes.push_back(Box{a.i0, a.i1, a.j0, a.j1, a.k0, a.k1});
             tab.push_back(LuZone{zs[n]->k, a});
+            done[n] = a;
         }
         if (tab.empty()) continue;
         fm_ctx::LuList* L = nullptr;
@@ -1157,7 +1185,8 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
             }
             if (!capture) HIPCHK(hipMemcpy(d, tab.data(), tab.size() * sizeof(LuZone), hipMemcpyHostToDevice));
             if (strm) {
-                launch_lu_stream_list(cart == 1, d, (int)tab.size(), L->work, L->offs.back(), L->maxrb, c->flow_ctr, st);
+                launch_lu_stream_list(cart == 1, d, (int)tab.size(), L->work, L->offs.back(), L->maxrb, npack, nunpack,
+                                      c->flow_ctr, st);
             } else if (flow) {
                 launch_lu_flow_list(cart == 1, d, L->work, L->offs.back(), c->flow_ctr, st);
             } else {
@@ -1185,7 +1214,8 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
         HIPCHK(hipMemcpyAsync(c->lu_tab, c->lu_host[q], tab.size() * sizeof(LuZone), hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(c->lu_ev[q], st));
         if (strm) {
-            launch_lu_stream_list(cart == 1, c->lu_tab, (int)tab.size(), L->work, L->offs.back(), L->maxrb, c->flow_ctr, st);
+            launch_lu_stream_list(cart == 1, c->lu_tab, (int)tab.size(), L->work, L->offs.back(), L->maxrb, npack,
+                                  nunpack, c->flow_ctr, st);
         } else if (flow) {
             launch_lu_flow_list(cart == 1, c->lu_tab, L->work, L->offs.back(), c->flow_ctr, st);
         } else {
@@ -1196,6 +1226,7 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
     }
     for (size_t n = 0; n < zs.size(); n++) {
         const LuArgs& la = las[n];
+        lu_stream_after(zs[n], done[n]);
         if (la.out == zs[n]->ro_scratch) {
             Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
             launch_copy_box(zs[n]->k, zs[n]->ro_scratch, zs[n]->ro[2], lb, st);
@@ -1586,6 +1617,7 @@ extern "C" int fm_time_phase(fm_ctx* c, int zone, int phase, int reps, float* ms
     la.n_ready = 0;
     z->w_rows = nullptr;                      // its sweeps write the scratch state into the rows
     z->n_known = 0;                           // and copy the normals of its own box
+    c->graph_epoch++;                         // (captured steps hold whether they were in place)
     {
         Box lb = {la.i0, la.i1, la.j0, la.j1, la.k0, la.k1};
         if (int rc = ensure_dataflow(c, z, lb, la)) return rc;

@@ -69,6 +69,14 @@ struct LuArgs {
     int n_ready;                  // streaming, 3dfull: the normal rows of this box are in place
 };
 enum { LU_BLOCKS = 0, LU_PIPELINE = 1, LU_DATAFLOW = 2, LU_STREAM = 3 };
+// streaming: the arrays copied into the rows (16 on 3dcart, 25 on 3dfull; the state rows are
+// skipped with w_ready, the normal rows with n_ready), and the arrays [first, end) copied back:
+// dW (0-4) unless it stays in the rows (keep_dw off), the updated state (5-9) with the Newton update
+__device__ __host__ __forceinline__ int lu_pack_count(bool cart, const LuArgs& a) {
+    return (cart ? 16 : 25) - (a.w_ready ? 5 : 0) - (!cart && a.n_ready ? 9 : 0);
+}
+__device__ __host__ __forceinline__ int lu_unpack_first(const LuArgs& a) { return a.newton && !a.keep_dw ? 5 : 0; }
+__device__ __host__ __forceinline__ int lu_unpack_end(const LuArgs& a) { return a.newton ? 10 : 5; }
 // doubles of the streaming exchange, and of the skewed input copy, of an LU box of ni x nj x nk
 // cells
 __host__ __device__ long lu_stream_exchange_size(int ni, int nj, int nk);
@@ -136,10 +144,11 @@ void launch_lu_list(bool cart, bool lower, bool fp32, const LuZone* zones, const
 // the work list in diagonal order, every zone's LuArgs with its face exchange (fx) and error word
 // streaming sweeps of several zones of one mesh kind: pencils (zone, 0, jt, kt) in diagonal order,
 // every zone's LuArgs with its rows (skw), exchange (stx) and error word; maxrb = the largest
-// number of 64-row blocks of a zone
+// number of 64-row blocks of a zone; npack and nunpack = the largest lu_pack_count and
+// lu_unpack_end - lu_unpack_first over the zones
 void lu_pencil_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, int& maxrb);
 void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork* work, int nwork, int maxrb,
-                           int* counter, hipStream_t st);
+                           int npack, int nunpack, int* counter, hipStream_t st);
 void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,
                          hipStream_t st);
 int lu_pipeline_failed(const LuArgs& a, hipStream_t st);

@@ -907,9 +907,6 @@ long lu_skew_size(bool cart, int ni, int nj, int nk) {
 
 // arrays the pack copies, in launch order: the right-hand side, the state (unless a.w_ready), the
 // coefficients, the normals (3dfull, unless a.n_ready)
-__device__ __host__ __forceinline__ int lu_pack_count(bool cart, const LuArgs& a) {
-    return (cart ? 16 : 25) - (a.w_ready ? 5 : 0) - (!cart && a.n_ready ? 9 : 0);
-}
 
 // exchange slot of value e of cell ii of lane l8 on the j (face 0) or k (face 1) side of the
 // consuming pencil t.  The eight lanes of a face handle cells with the same ii + l8 at one step
@@ -1029,16 +1026,18 @@ __global__ __launch_bounds__(256) void k_lu_pack(KZ z, LuArgs a, int ntj) {
     lu_pack_rows(z, a, ntj, t, y * 64, f, T);
 }
 
-// The skewed copies of several zones: workgroup (w, y, f) packs array f of rows [64 y, 64 y + 64)
-// of pencil work[w] (every array: the batched sweeps keep nothing between calls).
+// The skewed copies of several zones: workgroup (w, y, c) packs the c-th array its zone needs
+// (as in k_lu_pack: the state rows are skipped with a.w_ready, the normal rows with a.n_ready) of
+// rows [64 y, 64 y + 64) of pencil work[w].
 __global__ __launch_bounds__(256) void k_lu_pack_list(const LuZone* __restrict__ zones, const LuWork* __restrict__ work) {
     __shared__ double T[64][65];
     const LuWork w = work[blockIdx.x];
     const KZ& z = zones[w.zone].z;
     const LuArgs& a = zones[w.zone].a;
     const int ntj = (a.j1 - a.j0 + 8) / 8, R = a.i1 - a.i0 + 15, r0 = blockIdx.y * 64;
-    if (r0 >= R || (int)blockIdx.z >= lu_skf_of(z)) return;
-    lu_pack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, blockIdx.z, T);
+    const int c = blockIdx.z;
+    if (r0 >= R || c >= lu_pack_count(z.typ == 2, a)) return;
+    lu_pack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, a.w_ready && c >= 5 ? c + 5 : c, T);
 }
 
 // The results back to the zone arrays: workgroup (t, y, f) copies array f (dW 0-4, the updated
@@ -1079,16 +1078,18 @@ __global__ __launch_bounds__(256) void k_lu_unpack(KZ z, LuArgs a, int ntj, int 
     lu_unpack_rows(z, a, ntj, t, y * 64, f0 + blockIdx.z, T);
 }
 
-// The results of several zones back: workgroup (w, y, f) copies array f (dW 0-4, the updated state
-// 5-9) of rows [64 y, 64 y + 64) of pencil work[w].
+// The results of several zones back: workgroup (w, y, c) copies array f0 + c (dW 0-4, the updated
+// state 5-9) of rows [64 y, 64 y + 64) of pencil work[w]; as in k_lu_unpack, dW stays in the rows
+// when a.keep_dw is off.
 __global__ __launch_bounds__(256) void k_lu_unpack_list(const LuZone* __restrict__ zones, const LuWork* __restrict__ work) {
     __shared__ double T[64][65];
     const LuWork w = work[blockIdx.x];
     const KZ& z = zones[w.zone].z;
     const LuArgs& a = zones[w.zone].a;
     const int ntj = (a.j1 - a.j0 + 8) / 8, R = a.i1 - a.i0 + 15, r0 = blockIdx.y * 64;
-    if (r0 >= R || ((int)blockIdx.z >= 5 && !a.newton)) return;
-    lu_unpack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, blockIdx.z, T);
+    const int f = lu_unpack_first(a) + (int)blockIdx.z;
+    if (r0 >= R || f >= lu_unpack_end(a)) return;
+    lu_unpack_rows(z, a, ntj, w.bj + ntj * w.bk, r0, f, T);
 }
 
 // The pencil exchanges of several zones filled with the pending pattern (and the shared pencil
@@ -1489,7 +1490,7 @@ void stream(const KZ& z, const LuArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_lin, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, a.stx, nx, kLuPending, a.sync);
     hipLaunchKernelGGL((k_lu_stream<CART, false>), dim3(grid), dim3(64), 0, st, z, a, ntj, ntk);
     // dW back only if asked (inside fm_compute it is copied once, after the last time step)
-    const int f0 = a.newton && !a.keep_dw ? 5 : 0, f1 = a.newton ? 10 : 5;
+    const int f0 = lu_unpack_first(a), f1 = lu_unpack_end(a);
     hipLaunchKernelGGL(k_lu_unpack, dim3((unsigned)(ntj * ntk), (unsigned)((ni + 14 + 63) / 64), (unsigned)(f1 - f0)),
                        dim3(256), 0, st, z, a, ntj, f0);
 }
@@ -1593,7 +1594,7 @@ void lu_pencil_list(const std::vector<Box>& boxes, std::vector<LuWork>& work, in
 // exchanges filled (and the pencil counter reset) and one launch over the pencil list, then the
 // results back.
 void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork* work, int nwork, int maxrb,
-                           int* counter, hipStream_t st) {
+                           int npack, int nunpack, int* counter, hipStream_t st) {
     if (nwork <= 0 || nz <= 0) return;
     int cus = 0, dev = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1602,8 +1603,9 @@ void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork*
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, 64, 0) != hipSuccess || per < 1) per = 1;
     const long cap = (long)per * cus;
     const unsigned grid = (unsigned)(nwork < cap ? nwork : cap);
-    hipLaunchKernelGGL(k_lu_pack_list, dim3((unsigned)nwork, (unsigned)maxrb, cart ? 16u : 25u), dim3(256), 0, st,
-                       zones, work);
+    if (npack > 0)
+        hipLaunchKernelGGL(k_lu_pack_list, dim3((unsigned)nwork, (unsigned)maxrb, (unsigned)npack), dim3(256), 0, st,
+                           zones, work);
     for (int lower = 1; lower >= 0; lower--) {
         hipLaunchKernelGGL(k_lu_fill_list, dim3((unsigned)nz, 64), dim3(256), 0, st, zones, counter);
         if (cart) {
@@ -1614,7 +1616,9 @@ void launch_lu_stream_list(bool cart, const LuZone* zones, int nz, const LuWork*
             else hipLaunchKernelGGL((k_lu_stream_list<false, false>), dim3(grid), dim3(64), 0, st, zones, work, nwork, counter);
         }
     }
-    hipLaunchKernelGGL(k_lu_unpack_list, dim3((unsigned)nwork, (unsigned)maxrb, 10u), dim3(256), 0, st, zones, work);
+    if (nunpack > 0)
+        hipLaunchKernelGGL(k_lu_unpack_list, dim3((unsigned)nwork, (unsigned)maxrb, (unsigned)nunpack), dim3(256), 0, st,
+                           zones, work);
 }
 
 void launch_lu_flow_list(bool cart, const LuZone* zones, const LuWork* work, int nwork, int* counter,

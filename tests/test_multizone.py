@@ -169,8 +169,11 @@ def test_gpu_tiles_implicit_walls(orc, mode):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["bdf", "bdf_ss2", "local"])
 def test_gpu_zone_batch_equals_zone_loop(mode):
-    """The diagonal-batched LU-SGS over all zones (a work list of (zone, block) pairs) gives the
-    per-zone sweeps bit for bit, on uneven blocks and on unconnected zones of both mesh kinds."""
+    """The batched LU-SGS over all zones (one work list of (zone, pencil) pairs, the default
+    streaming schedule) gives the per-zone sweeps bit for bit, on uneven blocks and on unconnected
+    zones of both mesh kinds.  Compared: the three time levels, drodm, mu and coe after seven time
+    steps (run eagerly, then captured, then replayed; dW stays in the rows inside fm_compute and is
+    copied back when drodm is read), a state edit of one zone and three more steps."""
     from fast_amd import _lib
     num = H.numerics("ausmpred", mode)
 
@@ -188,7 +191,12 @@ def test_gpu_zone_batch_equals_zone_loop(mode):
             ctx.get(i, _lib.FIELD_RO, t)
             ctx.set(i, _lib.FIELD_RO_M1, t); ctx.set(i, _lib.FIELD_RO_P1, t)
         nsub = int(zones[0].param_int[P.NSSITER])
-        ctx.compute(0, 1, nsub, nit_c=2, modulo_verif=2, first_it=0)
+        ctx.compute(0, 1, nsub, nit_c=7, modulo_verif=2, first_it=0)
+        # an edit of one zone's state between calls: its rows no longer hold it
+        t = np.zeros_like(zones[0].ro)
+        ctx.get(ids[0], _lib.FIELD_RO, t)
+        ctx.set(ids[0], _lib.FIELD_RO, t * (1.0 + 1e-3))
+        ctx.compute(1, 1, nsub, nit_c=3, modulo_verif=2, first_it=0)
         ctx.sync()
         for i, z in zip(ids, zones):
             ctx.pull(i, z)
@@ -210,8 +218,8 @@ def test_gpu_zone_batch_equals_zone_loop(mode):
         a = run(0, make)
         b = run(1, make)
         for za, zb in zip(a, b):
-            assert np.array_equal(za.ro, zb.ro), za.name
-            assert np.array_equal(za.ro_m1, zb.ro_m1), za.name
+            for f in ("ro", "ro_m1", "ro_p1", "drodm", "xmut", "coe"):
+                assert np.array_equal(getattr(za, f), getattr(zb, f)), (za.name, f)
 
 
 @pytest.mark.gpu
