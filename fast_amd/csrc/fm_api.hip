@@ -92,6 +92,9 @@ struct fm_ctx {
     std::vector<LuList> lu_lists;
     LuZone* lu_tab; int lu_tab_cap;
     LuZone* lu_host[2]; hipEvent_t lu_ev[2]; int lu_slot;
+    // the edge and corner table of several zones, outside graphs (staged as the LU table)
+    CoinsZone* co_tab; int co_cap;
+    CoinsZone* co_host[2]; hipEvent_t co_ev[2]; int co_slot;
     int* flow_ctr;           // block counter of the batched dataflow sweeps
     // several zones: each zone's right-hand side and boundary conditions run on a stream of its
     // own (forked from and joined back to `stream`); `cur` is the stream the zone work goes to
@@ -116,7 +119,9 @@ struct fm_ctx {
         long epoch;
         int state;                      // 1: ran once eagerly, 2: captured
         hipGraphExec_t exec;
-        std::vector<std::pair<long, LuZone*>> tabs;   // LU tables per (sub-iteration, mesh kind)
+        // device tables a captured step reads: LU (key 2 sub-iteration + mesh kind), edges and
+        // corners (key -1 - sub-iteration)
+        std::vector<std::pair<long, void*>> tabs;
     };
     int graph_on;
     long graph_epoch;                   // bumped by anything a captured kernel argument depends on
@@ -203,6 +208,7 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
         delete g;
     }
     if (c->lu_tab) (void)hipFree(c->lu_tab);
+    if (c->co_tab) (void)hipFree(c->co_tab);
     if (c->flow_ctr) (void)hipFree(c->flow_ctr);
     for (hipStream_t s2 : c->side) (void)hipStreamDestroy(s2);
     for (hipEvent_t e2 : c->side_ev) (void)hipEventDestroy(e2);
@@ -210,6 +216,8 @@ extern "C" int fm_ctx_destroy(fm_ctx* c) {
     for (int q = 0; q < 2; q++) {
         if (c->lu_host[q]) (void)hipHostFree(c->lu_host[q]);
         if (c->lu_ev[q]) (void)hipEventDestroy(c->lu_ev[q]);
+        if (c->co_host[q]) (void)hipHostFree(c->co_host[q]);
+        if (c->co_ev[q]) (void)hipEventDestroy(c->co_ev[q]);
     }
     (void)hipEventDestroy(c->e0);
     (void)hipEventDestroy(c->e1);
@@ -1177,7 +1185,7 @@ static int lu_batched(fm_ctx* c, const std::vector<Zone*>& zs, const std::vector
             // a captured step reads a table of its own, written once before its capture
             long key = 2L * nstep + cart;
             LuZone* d = nullptr;
-            for (auto& t : ge->tabs) if (t.first == key) d = t.second;
+            for (auto& t : ge->tabs) if (t.first == key) d = (LuZone*)t.second;
             if (!d) {
                 if (capture) return fail(FM_E_HIP, "graph capture: LU table missing");
                 HIPCHK(hipMalloc(&d, tab.size() * sizeof(LuZone)));
@@ -1267,6 +1275,49 @@ static int side_join(fm_ctx* c, size_t ns) {
     return FM_OK;
 }
 
+// The edges and corners of several zones in one launch, after their physical boundary conditions
+// (per zone, gsdr3.cpp:436-439): a zone's edges and corners read only its own face ghosts.  A
+// captured step reads a table of its own (written once before its capture); otherwise the table
+// goes through a pinned buffer that the previous copy from it has released.
+static int coins_batched(fm_ctx* c, const std::vector<CoinsZone>& co, int nblk, fm_ctx::GraphEntry* ge,
+                         int nstep, bool capture) {
+    if (co.empty()) return FM_OK;
+    hipStream_t st = c->stream;
+    const size_t bytes = co.size() * sizeof(CoinsZone);
+    CoinsZone* d = nullptr;
+    if (ge) {
+        const long key = -1L - nstep;
+        for (auto& t : ge->tabs) if (t.first == key) d = (CoinsZone*)t.second;
+        if (!d) {
+            if (capture) return fail(FM_E_HIP, "graph capture: corner table missing");
+            HIPCHK(hipMalloc(&d, bytes));
+            ge->tabs.push_back({key, d});
+        }
+        if (!capture) HIPCHK(hipMemcpy(d, co.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        if ((int)co.size() > c->co_cap) {
+            HIPCHK(hipStreamSynchronize(st));
+            if (c->co_tab) HIPCHK(hipFree(c->co_tab));
+            for (int q = 0; q < 2; q++) {
+                if (c->co_host[q]) HIPCHK(hipHostFree(c->co_host[q]));
+                HIPCHK(hipHostMalloc(&c->co_host[q], bytes));
+                if (!c->co_ev[q]) HIPCHK(hipEventCreateWithFlags(&c->co_ev[q], hipEventDisableTiming));
+            }
+            HIPCHK(hipMalloc(&c->co_tab, bytes));
+            c->co_cap = (int)co.size();
+        }
+        int q = c->co_slot; c->co_slot ^= 1;
+        HIPCHK(hipEventSynchronize(c->co_ev[q]));
+        memcpy(c->co_host[q], co.data(), bytes);
+        HIPCHK(hipMemcpyAsync(c->co_tab, c->co_host[q], bytes, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(c->co_ev[q], st));
+        d = c->co_tab;
+    }
+    launch_coins_list(d, (int)co.size(), nblk, st);
+    HIPCHK(hipGetLastError());
+    return FM_OK;
+}
+
 // Within a sub-iteration every zone first gets its right-hand side and LU-SGS update, then
 // the transfers fill the ghost cells shared with other zones, then every zone's physical
 // boundary conditions and corners are applied (gsdr3.cpp:205-444).
@@ -1317,12 +1368,28 @@ static int time_step(fm_ctx* c, int first_it, int verif, int n0, int n1, fm_ctx:
         if (!any) continue;
         int rc = transfer(c, cl.data());
         if (rc) return rc;
-        if (ns) { if (int rc2 = side_fork(c, ns, capture)) return rc2; }
-        for (size_t n = 0; n < nz; n++) {
-            if (ns) c->cur = c->side[n % ns];
-            if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
+        if (ns) {
+            // several zones: their physical boundary conditions side by side, then the edges and
+            // corners of all of them in one launch
+            if (int rc2 = side_fork(c, ns, capture)) return rc2;
+            std::vector<CoinsZone> co;
+            int nblk = 0;
+            for (size_t n = 0; n < nz; n++) {
+                if (lexit[n] != 0) continue;
+                Zone* z = c->zones[n];
+                c->cur = c->side[n % ns];
+                int shift[6];
+                bczone(c, z, 0, cl[n], shift, nullptr);
+                const CoinsZone e = {z->k, cl[n], Box{1, z->k.ni, 1, z->k.nj, 1, z->k.nk}};
+                co.push_back(e);
+                nblk = std::max(nblk, coins_blocks(e.z, e.L));
+            }
+            if (int rc2 = side_join(c, ns)) return rc2;
+            if (int rc2 = coins_batched(c, co, nblk, ge, nstep, capture)) return rc2;
+        } else {
+            for (size_t n = 0; n < nz; n++)
+                if (lexit[n] == 0) fill_ghosts(c, c->zones[n], cl[n]);
         }
-        if (ns) { if (int rc2 = side_join(c, ns)) return rc2; }
         HIPCHK(hipGetLastError());
     }
     return FM_OK;

@@ -750,10 +750,9 @@ __global__ void k_bc_column(KZ z, double* __restrict__ rop, BcArgs a) {
     }
 }
 
-// Edges: 12 bands of nf x nf ghosts along the zone edges; mean of the two face ghosts.
-__global__ void k_coins_edges(KZ z, double* __restrict__ rop, Box L) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int band = blockIdx.y;       // 0..11
+// Edges: 12 bands of nf x nf ghosts along the zone edges; mean of the two face ghosts (which no
+// edge or corner writes).  Thread t of band 0..11.
+__device__ __forceinline__ void coins_edge(const KZ& z, double* __restrict__ rop, const Box& L, int band, int t) {
     int nf = z.nf, nfk = z.nfk;
     long nd = z.nd;
     int a, b, len;
@@ -798,10 +797,13 @@ __global__ void k_coins_edges(KZ z, double* __restrict__ rop, Box L) {
     }
     (void)a; (void)b;
 }
+__global__ void k_coins_edges(KZ z, double* __restrict__ rop, Box L) {
+    coins_edge(z, rop, L, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
-__global__ void k_coins_corners(KZ z, double* __restrict__ rop, Box L) {
-    int corner = blockIdx.x;   // 0..7: bit0 i, bit1 j, bit2 k
-    int t = threadIdx.x;
+// Corners: nf x nf x nfk ghosts, the mean of three face ghosts.  Thread t of corner 0..7 (bit0
+// i, bit1 j, bit2 k).
+__device__ __forceinline__ void coins_corner(const KZ& z, double* __restrict__ rop, const Box& L, int corner, int t) {
     int nf = z.nf, nfk = z.nfk;
     if (t >= nf * nf * nfk) return;
     int ic = corner & 1, jc = (corner >> 1) & 1, kc = (corner >> 2) & 1;
@@ -816,6 +818,19 @@ __global__ void k_coins_corners(KZ z, double* __restrict__ rop, Box L) {
     double c3 = 1. / 3.;
 #pragma unroll
     for (int e = 0; e < 5; e++) rop[l + e * nd] = (rop[l11 + e * nd] + rop[l12 + e * nd] + rop[l13 + e * nd]) * c3;
+}
+__global__ void k_coins_corners(KZ z, double* __restrict__ rop, Box L) {
+    coins_corner(z, rop, L, blockIdx.x, threadIdx.x);
+}
+
+// Edges and corners of several zones in one launch: workgroup (x, y, n) covers threads
+// 256 x .. 256 x + 255 of band y (0..11) of zone n, or with y = 12 corner x (x < 8).  Edges and
+// corners read face ghosts only and write disjoint cells, so one launch gives what the two
+// kernels give in turn.
+__global__ __launch_bounds__(256) void k_coins_list(const CoinsZone* __restrict__ zones) {
+    const CoinsZone& c = zones[blockIdx.z];
+    if (blockIdx.y < 12) coins_edge(c.z, c.rop, c.L, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+    else if (blockIdx.x < 8) coins_corner(c.z, c.rop, c.L, blockIdx.x, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1127,6 +1142,20 @@ void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st) {
     int nt = mx * z.nf * (z.nf > z.nfk ? z.nf : z.nfk);
     hipLaunchKernelGGL(k_coins_edges, dim3((nt + 255) / 256, 12), dim3(256), 0, st, z, rop, L);
     hipLaunchKernelGGL(k_coins_corners, dim3(8), dim3(64), 0, st, z, rop, L);
+}
+
+int coins_blocks(const KZ& z, const Box& L) {
+    int mx = L.i1 - L.i0 + 1;
+    if (L.j1 - L.j0 + 1 > mx) mx = L.j1 - L.j0 + 1;
+    if (L.k1 - L.k0 + 1 > mx) mx = L.k1 - L.k0 + 1;
+    const int nt = mx * z.nf * (z.nf > z.nfk ? z.nf : z.nfk);
+    const int nb = (nt + 255) / 256;
+    return nb > 8 ? nb : 8;
+}
+
+void launch_coins_list(const CoinsZone* zones, int nz, int nblk, hipStream_t st) {
+    if (nz <= 0) return;
+    hipLaunchKernelGGL(k_coins_list, dim3((unsigned)nblk, 13, (unsigned)nz), dim3(256), 0, st, zones);
 }
 
 void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double* out, hipStream_t st) {

@@ -157,6 +157,10 @@ long lu_exchange_size(int ni, int nj, int nk);
 void launch_copy_box(const KZ& z, const double* src, double* dst, Box b, hipStream_t st);
 void launch_bc_face(const KZ& z, double* rop, const BcArgs& a, hipStream_t st);
 void launch_coins(const KZ& z, double* rop, Box L, hipStream_t st);
+// edges and corners of several zones in one launch (each zone's level and interior box)
+struct CoinsZone { KZ z; double* rop; Box L; };
+int coins_blocks(const KZ& z, const Box& L);     // workgroups per band a zone needs
+void launch_coins_list(const CoinsZone* zones, int nz, int nblk, hipStream_t st);
 void launch_norms(const KZ& z, const double* dr, double* part, int nblk, double* out, hipStream_t st);
 void launch_cfl(const KZ& z, const double* rop, double* part, int nblk, double* out, hipStream_t st);
 void launch_tke(const KZ& z, const double* ro, double* part, int nblk, hipStream_t st);
