@@ -342,21 +342,29 @@ def warmup(t, tc=None, graph=None, infos_ale=None, Adjoint=False, tmy=None, list
            Padding=None, verbose=0):
     """Build the solver state: metrics, time levels _M1/_P1, ghost cells.  -> (t, tc, metrics)
 
-    Zones are joined by the GridConnectivity1to1 nodes of t (injection transfers of the ghost
-    cells, as the reference's tc of a matching mesh); ``tc`` may instead be a list of
-    fast_amd.connect.Raccord.  A Cassiopee connectivity tree is not accepted (it needs
-    Connector)."""
-    from . import _lib
-    if tc is not None and not (isinstance(tc, list) and all(hasattr(r, "rcv") for r in tc)):
-        raise NotImplementedError("Cassiopee connectivity trees are not supported: give the matches as "
-                                  "GridConnectivity1to1 nodes of t or a list of fast_amd.connect.Raccord")
+    Zones are joined by the injection raccords of ``tc``, a Cassiopee connectivity tree (the
+    ``ID_*`` subregions of its donor zones, PyTree.py:492-499), or, without ``tc``, by the
+    GridConnectivity1to1 nodes of t; ``tc`` may also be a list of fast_amd.connect.Raccord.
+    As in the reference, HOOK['param_int_tc'] / HOOK['param_real_tc'] then hold the raccords
+    in their flattened form."""
+    from . import _lib, connect
+    racs_given = tc is not None and isinstance(tc, list) and all(hasattr(r, "rcv") for r in tc)
+    if tc is not None and not racs_given and not Internal.getZones(tc):
+        raise TypeError("tc must be a connectivity tree or a list of fast_amd.connect.Raccord")
     HOOK.clear()
     metrics = allocate_metric(t, verbose)
     _init_metric(t, metrics, HOOK)
     ctx = HOOK["ctx"]
-    racs = list(tc) if tc is not None else _matches(HOOK["zones"])
+    if tc is None:
+        racs = _matches(HOOK["zones"])
+    elif racs_given:
+        racs = list(tc)
+    else:
+        index = {z[0]: n for n, (z, hz) in enumerate(HOOK["zones"])}
+        racs = connect.from_tc(tc, index, [hz.nd for z, hz in HOOK["zones"]])
     ctx.set_transfers(racs)
     HOOK["raccords"] = racs
+    HOOK["param_int_tc"], HOOK["param_real_tc"] = connect.pack_tc(racs)
     first_it = 1
     has_m1s = []
     for (z, hz), zid in zip(HOOK["zones"], HOOK["ids"]):

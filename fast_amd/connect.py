@@ -3,9 +3,11 @@
 The reference fills the ghost cells of a zone face that abuts another zone from the donor
 zone's interior cells (FastC/Com/setInterpTransfersFast.cpp:440-760, setInterpTransfersIntra:
 for each raccord, rcv[rcvPts[n]] = sum coef * dnr[donorPts[...]]; a 1-to-1 match is an
-injection with coefficient 1, interpolation type 0).  The raccords themselves are built by
-Cassiopee's Connector (not part of this build); here they are built directly from the face
-adjacency of the zones.
+injection: interpolation type 1 with coefficient 1, as the reference's hand-built connectivity
+tree of Fast/FastS/test/tgvMpi.py:135-140 writes it).  Raccords come from the face adjacency of
+the zones, from the GridConnectivity1to1 nodes of a tree, from the ``ID_*`` subregions of a
+Cassiopee connectivity tree (``tc``), or from its flattened form ``param_int_tc`` /
+``param_real_tc``; Connector, which builds ``tc`` trees from meshes, is not part of this build.
 
 A raccord is (receiver zone, donor zone, receiver cell indices, donor cell indices), the indices
 being 0-based linear indices into the zones' field arrays (``inddm``).  The ghost slab of the
@@ -129,3 +131,157 @@ def apply(zones, raccords, levels):
         ndr, ndd = zones[r.zr].nd, zones[r.zd].nd
         for e in range(5):
             levels[r.zr][r.rcv + e * ndr] = levels[r.zd][r.dnr + e * ndd]
+
+
+# ---------------------------------------------------------------------------------------------
+# Cassiopee connectivity trees (tc) and their flattened form param_int_tc / param_real_tc
+
+INJECTION = 1          # InterpolantsType of an injection point (tgvMpi.py:138-139)
+NTAB = 18              # integer tables per raccord (ntab_int, setInterpTransfersFast.cpp:497)
+NEXPLOC = 27           # integers per raccord after the tables, read for local explicit steps
+# table numbers read by setInterpTransfersIntra (setInterpTransfersFast.cpp:640-760); table t of
+# raccord r sits at ech + 4 + 2 * timelevel + r + nrac * t
+T_NPTS, T_NINTERP, T_IBCTYPE, T_NOD, T_DONOR, T_NTYPE, T_COEF = 0, 1, 3, 5, 6, 7, 8
+T_NRCV, T_NOR, T_RCV, T_NVARS, T_ROT, T_NFLU = 10, 11, 12, 13, 14, 16
+
+
+def from_tc(tc, index, sizes):
+    """Injection raccords of a Cassiopee connectivity tree.
+
+    The tree holds one zone per donor zone (same names as in t).  Each ``ID_<name>``
+    ZoneSubRegion_t of a donor zone carries the receiver zone's name as its value, ZoneRole
+    'Donor', PointList = donor cells and PointListDonor = receiver cells (storage on the donor
+    side, setInterpTransfersFast.cpp:753-756), both 0-based linear indices into the zones'
+    field arrays, and InterpolantsType / InterpolantsDonor, the type and the coefficients of
+    each point (Fast/FastS/test/tgvMpi.py:132-195 builds such a tree by hand).  ``index`` maps
+    zone names to zone numbers, ``sizes[n]`` is the field-array length of zone n.  Immersed-
+    boundary subregions (``IBCD_*``) and interpolation types other than injection with
+    coefficient 1 raise NotImplementedError."""
+    from . import Internal
+    racs = []
+    for zd in Internal.getZones(tc):
+        if zd[0] not in index:
+            raise ValueError("connectivity tree zone %r is not a zone of t" % zd[0])
+        d = index[zd[0]]
+        for sr in Internal.getNodesFromType1(zd, "ZoneSubRegion_t"):
+            if sr[0].startswith("IBCD_"):
+                raise NotImplementedError("immersed-boundary raccord %s of zone %s is not supported"
+                                          % (sr[0], zd[0]))
+            if not sr[0].startswith("ID_"):
+                continue
+            role = Internal.getNodeFromName1(sr, "ZoneRole")
+            if role is not None and Internal.getValue(role) != "Donor":
+                continue
+            rname = Internal.getValue(sr)
+            if rname not in index:
+                raise ValueError("raccord %s of zone %s: unknown receiver zone %r" % (sr[0], zd[0], rname))
+            r = index[rname]
+            get = lambda name: Internal.getNodeFromName1(sr, name)
+            if get("PointList") is None or get("PointListDonor") is None:
+                raise ValueError("raccord %s of zone %s has no PointList / PointListDonor" % (sr[0], zd[0]))
+            dnr = np.asarray(get("PointList")[1]).ravel().astype(np.int64)
+            rcv = np.asarray(get("PointListDonor")[1]).ravel().astype(np.int64)
+            typ = get("InterpolantsType")
+            typ = np.full(dnr.size, INJECTION) if typ is None else np.asarray(typ[1]).ravel()
+            coef = get("InterpolantsDonor")
+            coef = np.ones(dnr.size) if coef is None else np.asarray(coef[1], dtype=np.float64).ravel()
+            _check_injection("raccord %s of zone %s" % (sr[0], zd[0]), rcv, dnr, typ, coef,
+                             sizes[r], sizes[d])
+            racs.append(Raccord(r, d, rcv, dnr))
+    return racs
+
+
+def _check_injection(what, rcv, dnr, typ, coef, nd_r, nd_d):
+    if rcv.size != dnr.size or typ.size != dnr.size:
+        raise ValueError("%s: %d receiver points, %d donor points, %d types"
+                         % (what, rcv.size, dnr.size, typ.size))
+    bad = typ[typ != INJECTION]
+    if bad.size:
+        raise NotImplementedError("%s: interpolation type %d (only injection, type %d, is supported)"
+                                  % (what, int(bad[0]), INJECTION))
+    if coef.size != dnr.size or not np.all(coef == 1.0):
+        raise NotImplementedError("%s: injection coefficients other than 1" % what)
+    if rcv.size and (rcv.min() < 0 or rcv.max() >= nd_r or dnr.min() < 0 or dnr.max() >= nd_d):
+        raise ValueError("%s: point index outside the zone's field arrays" % what)
+
+
+def pack_tc(raccords):
+    """(param_int_tc, param_real_tc) of injection raccords, laid out where
+    setInterpTransfersIntra reads them (setInterpTransfersFast.cpp:440-760): no immersed-boundary
+    or MPI part (sizecomIBC = sizecomID = 0), the intra exchange header at ech = param_int_tc[4]
+    (nrac, no unsteady raccord, timelevel 0), then NTAB tables of nrac entries, NEXPLOC zeros per
+    raccord, and per raccord its type groups [1, n, types...], donor and receiver points; the
+    coefficients (1 per point) in param_real_tc.  Entries the reader does not use are 0."""
+    nrac = len(raccords)
+    ech = 5
+    tab0 = ech + 4
+    data = tab0 + NTAB * nrac + NEXPLOC * nrac
+    size = data + sum(2 + 3 * r.rcv.size for r in raccords)
+    pit = np.zeros(size, dtype=np.int64)
+    prt = np.zeros(sum(r.rcv.size for r in raccords), dtype=np.float64)
+    pit[4] = ech
+    pit[ech + 1] = nrac
+    pit[ech + 2] = 0
+    pit[ech + 3] = 0
+    pos, rpos = data, 0
+    for n, r in enumerate(raccords):
+        npts = r.rcv.size
+        tab = lambda t, v: pit.__setitem__(tab0 + n + nrac * t, v)
+        tab(T_NPTS, npts); tab(T_NINTERP, npts); tab(T_IBCTYPE, -1)
+        tab(T_NOD, r.zd); tab(T_NOR, r.zr); tab(T_NRCV, npts)
+        tab(T_NVARS, 5); tab(T_ROT, 0); tab(T_NFLU, 0)
+        tab(T_NTYPE, pos)
+        pit[pos] = 1; pit[pos + 1] = npts
+        pit[pos + 2:pos + 2 + npts] = INJECTION
+        pos += 2 + npts
+        tab(T_DONOR, pos); pit[pos:pos + npts] = r.dnr; pos += npts
+        tab(T_RCV, pos); pit[pos:pos + npts] = r.rcv; pos += npts
+        tab(T_COEF, rpos); prt[rpos:rpos + npts] = 1.0; rpos += npts
+    if pit.max(initial=0) < 2 ** 31:
+        pit = pit.astype(np.int32)
+    return pit, prt
+
+
+def from_param_int_tc(param_int_tc, param_real_tc, sizes):
+    """Injection raccords of a flattened connectivity, read as setInterpTransfersIntra reads it
+    for the intra-process transfers (setInterpTransfersFast.cpp:461-470 for the header,
+    :635-760 for a raccord).  ``sizes[n]`` is the field-array length of zone n.  Unsteady raccords,
+    immersed-boundary raccords, azimuthal periodicity, other variable counts, conservative
+    flux corrections and types other than injection raise NotImplementedError."""
+    p = np.asarray(param_int_tc).ravel().astype(np.int64)
+    c = np.asarray(param_real_tc, dtype=np.float64).ravel()
+    size_ibc = int(p[2])
+    size_id = int(p[3 + size_ibc])
+    ech = int(p[1 + size_ibc + size_id + 3])
+    nrac, nrac_inst, timelevel = int(p[ech + 1]), int(p[ech + 2]), int(p[ech + 3])
+    if nrac_inst:
+        raise NotImplementedError("unsteady raccords are not supported")
+    racs = []
+    for n in range(nrac):
+        sh = ech + 4 + 2 * timelevel + n
+        tab = lambda t: int(p[sh + nrac * t])
+        what = "raccord %d" % n
+        if tab(T_IBCTYPE) >= 0:
+            raise NotImplementedError("%s: immersed-boundary raccord (type %d)" % (what, tab(T_IBCTYPE)))
+        if tab(T_NVARS) != 5:
+            raise NotImplementedError("%s: %d transferred variables (5 are supported)" % (what, tab(T_NVARS)))
+        if tab(T_ROT):
+            raise NotImplementedError("%s: azimuthal periodicity" % what)
+        if tab(T_NFLU):
+            raise NotImplementedError("%s: conservative flux correction" % what)
+        npts = tab(T_NRCV)
+        pos = tab(T_NTYPE)
+        ngroups = int(p[pos])
+        counts = p[pos + 1:pos + 1 + ngroups]
+        if int(counts.sum()) != npts:
+            raise ValueError("%s: type groups hold %d points, the raccord %d" % (what, int(counts.sum()), npts))
+        typ = p[pos + 1 + ngroups:pos + 1 + ngroups + npts]
+        dnr = p[tab(T_DONOR):tab(T_DONOR) + npts].copy()
+        rcv = p[tab(T_RCV):tab(T_RCV) + npts].copy()
+        coef = c[tab(T_COEF):tab(T_COEF) + tab(T_NINTERP)]
+        zr, zd = tab(T_NOR), tab(T_NOD)
+        if not (0 <= zr < len(sizes) and 0 <= zd < len(sizes)):
+            raise ValueError("%s: zone numbers %d <- %d outside the %d zones" % (what, zr, zd, len(sizes)))
+        _check_injection(what, rcv, dnr, typ, coef, sizes[zr], sizes[zd])
+        racs.append(Raccord(zr, zd, rcv, dnr))
+    return racs
