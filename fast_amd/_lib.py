@@ -52,6 +52,7 @@ SIGNATURES = {
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
     "fm_flux_balance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
     "fm_residual_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "fm_residual_vjp_zones": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
 }
 PHASE_COE, PHASE_RESIDUAL, PHASE_LU, PHASE_BC, PHASE_VJP = range(5)
 OPT_LU_PRECISION = 5
@@ -229,6 +230,22 @@ class Context:
             raise ValueError("lam must hold 5 x NDIMDX values")
         check(self.lib.fm_residual_vjp(self.h, zone, level, dptr(lam), dptr(out)))
         return out
+
+    def residual_vjp_zones(self, lams, level=0):
+        """(dR/dW)^T lam of every zone with the transfers' transpose: lams[n] and the returned
+        arrays are 5 x NDIMDX per zone, in the order the zones were added."""
+        n = len(self.zones)
+        if len(lams) != n:
+            raise ValueError("one lam array per zone (%d)" % n)
+        lams = [np.ascontiguousarray(a, dtype=np.float64) for a in lams]
+        for z, a in enumerate(lams):
+            if a.size != 5 * self.zone_nd(z):
+                raise ValueError("lam of zone %d must hold 5 x NDIMDX values" % z)
+        outs = [np.zeros_like(a) for a in lams]
+        pl = (ctypes.c_void_p * n)(*[a.ctypes.data for a in lams])
+        po = (ctypes.c_void_p * n)(*[a.ctypes.data for a in outs])
+        check(self.lib.fm_residual_vjp_zones(self.h, level, pl, po))
+        return outs
 
     def transfer(self, level=0):
         check(self.lib.fm_transfer(self.h, level))

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <mutex>
+#include <iterator>
 #include <string>
 #include <vector>
 #include "fastmi.h"
@@ -86,6 +87,10 @@ struct fm_ctx {
     // injection transfers between zones (all raccords, flattened)
     TransferPoint* tr_pts;
     long tr_n;
+    // per raccord: first point, point count, and whether its transpose must run point by point
+    // (a donor or receiver cell repeated, or a receiver that is also a donor)
+    std::vector<long> tr_off, tr_cnt;
+    std::vector<char> tr_serial;
     // diagonal-batched LU-SGS over several zones: cached block lists (per set of LU boxes)
     // and the per-zone argument table, staged through two pinned host buffers
     struct LuList { bool cart; bool stream; std::vector<Box> boxes; LuWork* work; std::vector<int> offs; int maxrb; };
@@ -1053,6 +1058,60 @@ extern "C" int fm_residual_vjp(fm_ctx* c, int zone, int level, const double* lam
     return FM_OK;
 }
 
+extern "C" int fm_residual_vjp_zones(fm_ctx* c, int level, const double* const* lam, double* const* out) {
+    if (!c || level < 0 || level > 2 || !lam || !out) return fail(FM_E_ARG, "fm_residual_vjp_zones: bad arguments");
+    int nz = (int)c->zones.size();
+    if (nz > kMaxTransferZones) return fail(FM_E_UNSUPPORTED, "fm_residual_vjp_zones: more than %d zones", kMaxTransferZones);
+    size_t total = 0;
+    for (int n = 0; n < nz; n++) {
+        KZ& k = c->zones[n]->k;
+        if (!lam[n] || !out[n]) return fail(FM_E_ARG, "fm_residual_vjp_zones: zone %d: no array", n);
+        if (k.kflu != 1 || k.slope != 2 || k.iflow != 1)
+            return fail(FM_E_UNSUPPORTED, "fm_residual_vjp_zones: zone %d: AUSM+(P) o3 Euler zones only", n);
+        if (k.nf < 2 || k.nfk < 2)
+            return fail(FM_E_UNSUPPORTED, "fm_residual_vjp_zones: zone %d: the face stencil needs 2 ghost layers", n);
+        total += 5 * (size_t)k.nd;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    for (Zone* z : c->zones)
+        if (!z->adj_buf)
+            if (int rc = dmalloc(&z->adj_buf, vjp_buffer_doubles(z->k))) return rc;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * total * sizeof(double)));
+    TransferArgs g;
+    hipError_t e = hipSuccess;
+    size_t o = 0;
+    for (int n = 0; n < nz && e == hipSuccess; n++) {
+        Zone* z = c->zones[n];
+        size_t cnt = 5 * (size_t)z->k.nd;
+        VjpArgs a;
+        double* dl = d + o;
+        a.rop = z->ro[level]; a.lam = dl; a.out = d + total + o; a.buf = z->adj_buf;
+        g.f[n] = a.out; g.nd[n] = z->k.nd;
+        e = hipMemcpyAsync(dl, lam[n], cnt * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemsetAsync(a.out, 0, cnt * sizeof(double), st);
+        if (e == hipSuccess) { launch_residual_vjp(z->k, a, st); e = hipGetLastError(); }
+        o += cnt;
+    }
+    // the transfers' transpose, raccords in reverse order
+    for (int r = (int)c->tr_off.size() - 1; r >= 0 && e == hipSuccess; r--) {
+        launch_transfer_vjp(c->tr_pts + c->tr_off[r], c->tr_cnt[r], g, c->tr_serial[r], st);
+        e = hipGetLastError();
+    }
+    o = 0;
+    for (int n = 0; n < nz && e == hipSuccess; n++) {
+        size_t cnt = 5 * (size_t)c->zones[n]->k.nd;
+        e = hipMemcpyAsync(out[n], d + total + o, cnt * sizeof(double), hipMemcpyDeviceToHost, st);
+        o += cnt;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(FM_E_HIP, "fm_residual_vjp_zones: %s", hipGetErrorString(e));
+    return FM_OK;
+}
+
 extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int* no_d,
                                 const int64_t* nb_pts, const int64_t* rcv_pts, const int64_t* dnr_pts) {
     if (!c || nrac < 0 || (nrac > 0 && (!no_r || !no_d || !nb_pts || !rcv_pts || !dnr_pts)))
@@ -1061,18 +1120,31 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
     if ((long)c->zones.size() > kMaxTransferZones && nrac > 0)
         return fail(FM_E_UNSUPPORTED, "transfers between more than %d zones", kMaxTransferZones);
     std::vector<TransferPoint> h;
+    std::vector<long> roff, rcnt;
+    std::vector<char> rserial;
     long off = 0;
     for (int r = 0; r < nrac; r++) {
         Zone* zr = get_zone(c, no_r[r]);
         Zone* zd = get_zone(c, no_d[r]);
         if (!zr || !zd) return fail(FM_E_ARG, "raccord %d: bad zone %d <- %d", r, no_r[r], no_d[r]);
         if (nb_pts[r] < 0) return fail(FM_E_ARG, "raccord %d: %lld points", r, (long long)nb_pts[r]);
+        roff.push_back(off); rcnt.push_back((long)nb_pts[r]);
         for (int64_t q = 0; q < nb_pts[r]; q++, off++) {
             int64_t ir = rcv_pts[off], id = dnr_pts[off];
             if (ir < 0 || ir >= zr->k.nd || id < 0 || id >= zd->k.nd)
                 return fail(FM_E_ARG, "raccord %d point %lld: index outside the zone", r, (long long)q);
             h.push_back(TransferPoint{no_r[r], no_d[r], (long)ir, (long)id});
         }
+        std::vector<long> rv(rcv_pts + roff.back(), rcv_pts + off), dv(dnr_pts + roff.back(), dnr_pts + off);
+        std::sort(rv.begin(), rv.end()); std::sort(dv.begin(), dv.end());
+        bool serial = std::adjacent_find(rv.begin(), rv.end()) != rv.end() ||
+                      std::adjacent_find(dv.begin(), dv.end()) != dv.end();
+        if (!serial && no_r[r] == no_d[r]) {
+            std::vector<long> both;
+            std::set_intersection(rv.begin(), rv.end(), dv.begin(), dv.end(), std::back_inserter(both));
+            serial = !both.empty();
+        }
+        rserial.push_back(serial ? 1 : 0);
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->tr_pts) { HIPCHK(hipFree(c->tr_pts)); c->tr_pts = nullptr; }
@@ -1082,6 +1154,7 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
         HIPCHK(hipMemcpy(c->tr_pts, h.data(), h.size() * sizeof(TransferPoint), hipMemcpyHostToDevice));
         c->tr_n = (long)h.size();
     }
+    c->tr_off = roff; c->tr_cnt = rcnt; c->tr_serial = rserial;
     c->graph_epoch++;
     return FM_OK;
 }

@@ -1190,6 +1190,43 @@ __global__ void k_transfer(const TransferPoint* __restrict__ pts, long npts, Tra
     for (int v = 0; v < 5; v++) r[q.rcv + v * ndr] = d[q.dnr + v * ndd];
 }
 
+// Transpose of one raccord's injection: the gradient held by each receiver cell moves onto its
+// donor cell (g_d[dnr] += g_r[rcv], then g_r[rcv] = 0).  Point-parallel when the raccord's donor
+// and receiver cells are all distinct; otherwise one thread takes the points last to first, the
+// transpose of an injection whose later points overwrite the earlier ones.
+__device__ inline void fold_point(const TransferPoint& q, const TransferArgs& g) {
+    double* __restrict__ d = g.f[q.zd];
+    double* __restrict__ r = g.f[q.zr];
+    long ndd = g.nd[q.zd], ndr = g.nd[q.zr];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double x = r[q.rcv + v * ndr];
+        r[q.rcv + v * ndr] = 0.;
+        d[q.dnr + v * ndd] += x;
+    }
+}
+
+__global__ void k_transfer_vjp(const TransferPoint* __restrict__ pts, long npts, TransferArgs g) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    fold_point(pts[t], g);
+}
+
+__global__ void k_transfer_vjp_serial(const TransferPoint* __restrict__ pts, long npts, TransferArgs g) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (long t = npts - 1; t >= 0; t--) fold_point(pts[t], g);
+}
+
+void launch_transfer_vjp(const TransferPoint* pts, long npts, const TransferArgs& g, int serial, hipStream_t st) {
+    if (npts <= 0) return;
+    if (serial) {
+        hipLaunchKernelGGL(k_transfer_vjp_serial, dim3(1), dim3(64), 0, st, pts, npts, g);
+        return;
+    }
+    long nb = (npts + 255) / 256;
+    hipLaunchKernelGGL(k_transfer_vjp, dim3((unsigned)nb), dim3(256), 0, st, pts, npts, g);
+}
+
 void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a, hipStream_t st) {
     if (npts <= 0) return;
     long nb = (npts + 255) / 256;

@@ -127,6 +127,17 @@ int fm_flux_balance(fm_ctx* ctx, int zone, int level, double* drodm);
    doubles.  Zones: AUSM+(P), o3, Euler (FM_E_UNSUPPORTED otherwise).  Synchronous. */
 int fm_residual_vjp(fm_ctx* ctx, int zone, int level, const double* lam, double* out);
 
+/* The same product for all zones at once, with the ghost cells filled by the transfers of
+   fm_transfers_set: out[n] = (dR_n/dW_n)^T lam[n], then, raccords taken in reverse order, the
+   gradient on each receiver cell moved onto its donor cell (the transpose of the injection
+   rcv[rcvPts] = dnr[donorPts] of setInterpTransfersFast.cpp:440-760; a periodic box joined to
+   itself by raccords is covered the same way).  out[n] is then the gradient with respect to the
+   cells that the raccords do not fill: the interior, and ghost cells of physical boundaries, which
+   the caller chains through its boundary conditions.  lam[n], out[n]: 5 x NDIMDX doubles of zone n,
+   for every zone of the context in the order they were added.  Zones: AUSM+(P), o3, Euler.
+   Synchronous. */
+int fm_residual_vjp_zones(fm_ctx* ctx, int level, const double* const* lam, double* const* out);
+
 /* Injection raccords between zones (1-to-1 matches), replacing any set before.  Raccord r
    fills nb_pts[r] receiver cells of zone no_r[r] from donor cells of zone no_d[r]; rcv_pts and
    dnr_pts hold the points of all raccords one after the other, as 0-based linear indices into

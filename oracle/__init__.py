@@ -144,6 +144,22 @@ class _Backend:
             raise RuntimeError("%s residual_vjp failed (%d)" % (self.prefix, rc))
         return out
 
+    def residual_vjp_zones(self, zones, racs, lams, rops=None):
+        """(dR/dW)^T lam of every zone, then the transpose of the injection transfers
+        rcv[rcvPts] = dnr[donorPts] (FastC/Com/setInterpTransfersFast.cpp:440-760), raccords last
+        to first: the gradient on a receiver cell moves onto its donor cell.  The reference has
+        no such routine; this is the transpose of its transfer, point by point."""
+        rops = [z.ro for z in zones] if rops is None else rops
+        outs = [self.residual_vjp(z, lam, rop) for z, lam, rop in zip(zones, lams, rops)]
+        for r in reversed(racs):
+            ndr, ndd = zones[r.zr].nd, zones[r.zd].nd
+            for q in range(len(r.rcv) - 1, -1, -1):
+                for e in range(5):
+                    x = outs[r.zr][r.rcv[q] + e * ndr]
+                    outs[r.zr][r.rcv[q] + e * ndr] = 0.0
+                    outs[r.zd][r.dnr[q] + e * ndd] += x
+        return outs
+
     def apply_bc(self, z, rop, nstep=1):
         f = getattr(self.lib, self.prefix + "apply_bc")
         rc = f(_i(z.param_int), _d(z.param_real), ctypes.c_int(nstep), _d(rop), _d(z.ti),
