@@ -126,3 +126,32 @@ def test_residual_vjp_zones_gpu(orc, split, extra):
     for z, a, b in zip(zo, got, want):
         assert np.array_equal(a, b), z.name
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_flux_balance_zones_autograd(orc):
+    """torch.autograd through flux_balance_zones: the forward is the oracle's balance after the
+    transfers, the gradient of sum(lam * R) the oracle's transposed product with the fold."""
+    import torch
+    from fast_amd import _lib, adjoint
+    zo, racs = make("tiled")
+    zg, _ = make("tiled")
+    for z in zo:
+        orc.init_metric(z)
+    ctx = _lib.Context(0)
+    for z in zg:
+        ctx.add_zone(z)
+    ctx.set_transfers(racs)
+    Ws = [torch.tensor(z.ro, dtype=torch.float64, requires_grad=True) for z in zo]
+    Rs = adjoint.flux_balance_zones(Ws, ctx)
+    base = filled(zo, racs, [z.ro for z in zo])
+    for z, R, b in zip(zo, Rs, base):
+        assert np.array_equal(z.interior(R.detach().numpy()), z.interior(orc.flux_balance(z, b))), z.name
+    rng = np.random.default_rng(5)
+    lams = [rng.standard_normal(5 * z.nd) for z in zo]
+    J = sum((R * torch.from_numpy(l)).sum() for R, l in zip(Rs, lams))
+    gs = torch.autograd.grad(J, Ws)
+    want = orc.residual_vjp_zones(zo, racs, [l * interior_mask(z) for l, z in zip(lams, zo)], base)
+    for z, g, w in zip(zo, gs, want):
+        assert np.allclose(g.numpy(), w, rtol=1e-12, atol=1e-12), z.name
+    ctx.close()
