@@ -91,6 +91,10 @@ struct fm_ctx {
     // (a donor or receiver cell repeated, or a receiver that is also a donor)
     std::vector<long> tr_off, tr_cnt;
     std::vector<char> tr_serial;
+    // a receiver cell filled by more than one point: the transfers then run raccord by raccord,
+    // in order, and a raccord that repeats a receiver runs its points in order (the last wins)
+    bool tr_in_order = false;
+    std::vector<char> tr_rdup;
     // diagonal-batched LU-SGS over several zones: cached block lists (per set of LU boxes)
     // and the per-zone argument table, staged through two pinned host buffers
     struct LuList { bool cart; bool stream; std::vector<Box> boxes; LuWork* work; std::vector<int> offs; int maxrb; };
@@ -659,7 +663,12 @@ static int transfer(fm_ctx* c, double* const* fields) {
     TransferArgs a;
     memset(&a, 0, sizeof a);
     for (size_t n = 0; n < c->zones.size(); n++) { a.f[n] = fields[n]; a.nd[n] = c->zones[n]->k.nd; }
-    launch_transfer(c->tr_pts, c->tr_n, a, c->stream);
+    if (!c->tr_in_order) {
+        launch_transfer(c->tr_pts, c->tr_n, a, c->stream);
+    } else {
+        for (size_t r = 0; r < c->tr_off.size(); r++)
+            launch_transfer_part(c->tr_pts + c->tr_off[r], c->tr_cnt[r], a, c->tr_rdup[r], c->stream);
+    }
     HIPCHK(hipGetLastError());
     return FM_OK;
 }
@@ -1121,7 +1130,7 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
         return fail(FM_E_UNSUPPORTED, "transfers between more than %d zones", kMaxTransferZones);
     std::vector<TransferPoint> h;
     std::vector<long> roff, rcnt;
-    std::vector<char> rserial;
+    std::vector<char> rserial, rdup;
     long off = 0;
     for (int r = 0; r < nrac; r++) {
         Zone* zr = get_zone(c, no_r[r]);
@@ -1137,8 +1146,9 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
         }
         std::vector<long> rv(rcv_pts + roff.back(), rcv_pts + off), dv(dnr_pts + roff.back(), dnr_pts + off);
         std::sort(rv.begin(), rv.end()); std::sort(dv.begin(), dv.end());
-        bool serial = std::adjacent_find(rv.begin(), rv.end()) != rv.end() ||
-                      std::adjacent_find(dv.begin(), dv.end()) != dv.end();
+        bool dup_r = std::adjacent_find(rv.begin(), rv.end()) != rv.end();
+        rdup.push_back(dup_r ? 1 : 0);
+        bool serial = dup_r || std::adjacent_find(dv.begin(), dv.end()) != dv.end();
         if (!serial && no_r[r] == no_d[r]) {
             std::vector<long> both;
             std::set_intersection(rv.begin(), rv.end(), dv.begin(), dv.end(), std::back_inserter(both));
@@ -1154,7 +1164,12 @@ extern "C" int fm_transfers_set(fm_ctx* c, int nrac, const int* no_r, const int*
         HIPCHK(hipMemcpy(c->tr_pts, h.data(), h.size() * sizeof(TransferPoint), hipMemcpyHostToDevice));
         c->tr_n = (long)h.size();
     }
-    c->tr_off = roff; c->tr_cnt = rcnt; c->tr_serial = rserial;
+    c->tr_off = roff; c->tr_cnt = rcnt; c->tr_serial = rserial; c->tr_rdup = rdup;
+    std::vector<std::pair<int, long>> recv;
+    recv.reserve(h.size());
+    for (const TransferPoint& q : h) recv.emplace_back(q.zr, q.rcv);
+    std::sort(recv.begin(), recv.end());
+    c->tr_in_order = std::adjacent_find(recv.begin(), recv.end()) != recv.end();
     c->graph_epoch++;
     return FM_OK;
 }

@@ -1227,6 +1227,25 @@ void launch_transfer_vjp(const TransferPoint* pts, long npts, const TransferArgs
     hipLaunchKernelGGL(k_transfer_vjp, dim3((unsigned)nb), dim3(256), 0, st, pts, npts, g);
 }
 
+__global__ void k_transfer_serial(const TransferPoint* __restrict__ pts, long npts, TransferArgs a) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (long t = 0; t < npts; t++) {
+        TransferPoint q = pts[t];
+        long ndd = a.nd[q.zd], ndr = a.nd[q.zr];
+        for (int v = 0; v < 5; v++) a.f[q.zr][q.rcv + v * ndr] = a.f[q.zd][q.dnr + v * ndd];
+    }
+}
+
+// One raccord's points; point by point, first to last, when it repeats a receiver cell.
+void launch_transfer_part(const TransferPoint* pts, long npts, const TransferArgs& a, int serial, hipStream_t st) {
+    if (npts <= 0) return;
+    if (serial) {
+        hipLaunchKernelGGL(k_transfer_serial, dim3(1), dim3(64), 0, st, pts, npts, a);
+        return;
+    }
+    launch_transfer(pts, npts, a, st);
+}
+
 void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a, hipStream_t st) {
     if (npts <= 0) return;
     long nb = (npts + 255) / 256;

@@ -110,6 +110,7 @@ struct TransferArgs {
     long nd[kMaxTransferZones];     // its NDIMDX
 };
 void launch_transfer(const TransferPoint* pts, long npts, const TransferArgs& a, hipStream_t st);
+void launch_transfer_part(const TransferPoint* pts, long npts, const TransferArgs& a, int serial, hipStream_t st);
 void launch_transfer_vjp(const TransferPoint* pts, long npts, const TransferArgs& g, int serial, hipStream_t st);
 
 // Wall efforts (bceffort.for): per face of a window, 21 fields and the 12 contributions to the

@@ -142,7 +142,8 @@ int fm_residual_vjp_zones(fm_ctx* ctx, int level, const double* const* lam, doub
    fills nb_pts[r] receiver cells of zone no_r[r] from donor cells of zone no_d[r]; rcv_pts and
    dnr_pts hold the points of all raccords one after the other, as 0-based linear indices into
    the zones' field arrays (the rcvPts / donorPts of setInterpTransfersFast.cpp:753-756).
-   nrac = 0 removes every raccord.  At most 128 zones.  fm_compute then runs, at every
+   nrac = 0 removes every raccord.  A receiver cell that several points fill takes the value of the
+   last of them, raccords and points in the order given.  At most 128 zones.  fm_compute then runs, at every
    sub-iteration, the right-hand side and LU-SGS of all zones, the transfers, and the boundary
    conditions of all zones (gsdr3.cpp:205-444). */
 int fm_transfers_set(fm_ctx* ctx, int nrac, const int* no_r, const int* no_d,

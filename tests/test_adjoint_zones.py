@@ -155,3 +155,23 @@ def test_flux_balance_zones_autograd(orc):
     for z, g, w in zip(zo, gs, want):
         assert np.allclose(g.numpy(), w, rtol=1e-12, atol=1e-12), z.name
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("split,extra", [("tiled", False), ("tiled", True)])
+def test_transfers_overwrite_in_raccord_order(split, extra):
+    """fm_transfer fills the ghost cells as the raccords applied in order do: with a raccord
+    that overwrites ghosts of an earlier one (and repeats its donors), the later value wins."""
+    from fast_amd import _lib
+    zo, racs = make(split, extra)
+    zg, _ = make(split, extra)
+    want = filled(zo, racs, [z.ro for z in zo])
+    ctx = _lib.Context(0)
+    ids = [ctx.add_zone(z) for z in zg]
+    ctx.set_transfers(racs)
+    ctx.transfer(0)
+    for i, z, w in zip(ids, zg, want):
+        got = np.zeros_like(z.ro)
+        ctx.get(i, _lib.FIELD_RO, got)
+        assert np.array_equal(got, w), z.name
+    ctx.close()
