@@ -1089,6 +1089,7 @@ extern "C" int fm_residual_vjp_zones(fm_ctx* c, int level, const double* const* 
     double* d = nullptr;
     HIPCHK(hipMalloc(&d, 2 * total * sizeof(double)));
     TransferArgs g;
+    memset(&g, 0, sizeof g);
     hipError_t e = hipSuccess;
     size_t o = 0;
     for (int n = 0; n < nz && e == hipSuccess; n++) {
